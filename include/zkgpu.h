@@ -1,0 +1,158 @@
+/*
+ * zkgpu.h — C ABI of libzkgpu.so, the MI355X (gfx950) BN254 MSM + Fr NTT library.
+ *
+ * This is the drop-in boundary for zerotruss/zk-gpu's public Rust API (src/lib.rs:1-6). Each
+ * entry point names the reference item it replaces (file:line in the reference repository).
+ * The reference has no C ABI of its own; a Rust facade binds these symbols one-to-one (the
+ * binding is shown in INTEGRATION.md).
+ *
+ * Conventions
+ *  - Field elements are 8 little-endian u32 words, plain integers below the modulus
+ *    (README.md:77-78; src/host.rs:104-110). G1 affine points are 16 words (x, y); G2 affine
+ *    points are 32 words (x.a, x.b, y.a, y.b) with Fq2 = Fq[u]/(u^2+1); infinity is all zeros
+ *    (src/group.rs:57-69).
+ *  - Every function returns ZKG_OK (0) or an error code; zkg_last_error() then holds a message
+ *    for the calling thread. Size and range violations use the reference's panic strings
+ *    (src/msm.rs:97-98,275-281; src/ntt.rs:27,54-59), so a facade can re-raise them verbatim.
+ *  - All work on a device is queued in FIFO order on one HIP stream. zkg_msm_run,
+ *    zkg_ntt_forward and zkg_ntt_inverse return before the GPU finishes; zkg_buffer_read and
+ *    zkg_wait synchronise (src/gpu.rs:151-199).
+ *  - Plans are not re-entrant: a plan owns its workspace and its result buffer
+ *    (src/msm.rs:25-64,286-298; CONTRIBUTING.md:83-84).
+ */
+#ifndef ZKGPU_H
+#define ZKGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zkg_device zkg_device;
+typedef struct zkg_buffer zkg_buffer;
+typedef struct zkg_msm zkg_msm;
+typedef struct zkg_ntt zkg_ntt;
+
+enum {
+    ZKG_OK = 0,
+    ZKG_ERR_INVALID = 1,  /* argument out of range */
+    ZKG_ERR_SIZE = 2,     /* buffer smaller than the plan needs */
+    ZKG_ERR_DEVICE = 3,   /* no GPU / HIP runtime error */
+    ZKG_ERR_NOMEM = 4,    /* device allocation failed or over budget */
+    ZKG_ERR_INTERNAL = 5
+};
+
+enum { ZKG_G1 = 1, ZKG_G2 = 2 }; /* Group::{G1, G2}  (src/group.rs:10-14) */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* zkg_last_error(void);
+/* ABI version (major * 10000 + minor * 100 + patch). */
+int zkg_version(void);
+
+/* ------------------------------------------------------------------ device and buffers */
+
+/* Gpu::new (src/gpu.rs:51-55). Fails with "GPU init failed" when no device exists. */
+int zkg_device_open(int ordinal, zkg_device** out);
+int zkg_device_close(zkg_device* dev);
+/* Compute units, shader clock (kHz), HBM bytes and gfx arch name of the device. */
+int zkg_device_info(zkg_device* dev, int* cu_count, int* clock_khz, size_t* total_mem, char* name,
+                    size_t name_len);
+/* No counterpart in the reference (wgpu hides its queue): the HIP stream (hipStream_t) on which
+ * the library queues all work for this device, so callers can record events or order their
+ * own work against it. */
+void* zkg_device_stream(zkg_device* dev);
+/* Gpu::buffer (src/gpu.rs:57-64): device buffer initialised with n_words u32. */
+int zkg_buffer_create(zkg_device* dev, const uint32_t* words, size_t n_words, zkg_buffer** out);
+/* Gpu::empty_buffer (src/gpu.rs:66-69): zero-filled buffer of n_words u32. */
+int zkg_buffer_empty(zkg_device* dev, size_t n_words, zkg_buffer** out);
+/* Gpu::read (src/gpu.rs:192-199): blocking read of the first n_words u32 (synchronises). */
+int zkg_buffer_read(zkg_buffer* buf, size_t n_words, uint32_t* out);
+/* wgpu Queue::write_buffer as used by callers that refill a buffer: queued host->device copy. */
+int zkg_buffer_write(zkg_buffer* buf, size_t offset_words, const uint32_t* words, size_t n_words);
+/* wgpu::Buffer::size: capacity in bytes. */
+size_t zkg_buffer_size(const zkg_buffer* buf);
+/* No counterpart in the reference: wrap caller-owned device memory (e.g. a torch tensor) as a
+ * buffer without copying; the memory must outlive the handle and is never freed by the library. */
+int zkg_buffer_wrap(zkg_device* dev, void* device_ptr, size_t bytes, zkg_buffer** out);
+void* zkg_buffer_device_ptr(const zkg_buffer* buf);
+int zkg_buffer_free(zkg_buffer* buf);
+/* Gpu::wait (src/gpu.rs:151-155): block until all queued work on the device is done. */
+int zkg_wait(zkg_device* dev);
+
+/* ------------------------------------------------------------------ MSM */
+
+/* Msm::default_window (src/msm.rs:76-80): clamp(ilog2(max(n,1)) - 5, 11, 16). */
+uint32_t zkg_msm_default_window(size_t n);
+/* Msm::for_group (src/msm.rs:95-180); Msm::new (:86) is group = ZKG_G1, item_points = 0;
+ * Msm::with_item_points (:91) passes item_points. item_points = 0 picks the size automatically.
+ * Errors: n = 0 -> "an MSM needs at least one point"; c outside 9..=20 ->
+ * "window size c must be in 9..=20, got {c}". */
+int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points, zkg_msm** out);
+int zkg_msm_destroy(zkg_msm* plan);
+/* Msm::group (src/msm.rs:267-269). */
+int zkg_msm_group(const zkg_msm* plan);
+/* Window size and point count of a plan. */
+uint32_t zkg_msm_window(const zkg_msm* plan);
+size_t zkg_msm_points(const zkg_msm* plan);
+/* Msm::upload_bases (src/msm.rs:182-188): plain affine coordinates -> device buffer in the
+ * library's Montgomery form. Group-agnostic: n_words is 8 per coordinate. */
+int zkg_msm_upload_bases(zkg_device* dev, const uint32_t* plain_xy, size_t n_words, zkg_buffer** out);
+/* Msm::upload_bases_montgomery_256 (src/msm.rs:190-197): coordinates already in Montgomery form
+ * with R = 2^256 (the snarkjs/ffjavascript form). The library's R is 2^256, so this is a copy. */
+int zkg_msm_upload_bases_montgomery_256(zkg_device* dev, const uint32_t* xy, size_t n_words, zkg_buffer** out);
+/* Convert an existing device buffer of plain coordinates to the library's form in place
+ * (what upload_bases does after its copy; for bases produced on the device). */
+int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words);
+/* Msm::run (src/msm.rs:284-298): queue sum(scalars[i] * bases[i]). *result receives the
+ * plan-owned buffer holding one XYZZ point (32 words G1, 64 words G2, Montgomery form), valid
+ * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...". */
+int zkg_msm_run(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
+/* Msm::profile (src/msm.rs:321-342): run once, synchronising around every stage; writes up to
+ * `cap` (stage name, milliseconds) pairs, names as NUL-separated text in `names`. */
+int zkg_msm_profile(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, char* names,
+                    size_t names_len, double* ms, size_t cap, size_t* count);
+/* xyzz_to_affine (src/msm.rs:345-354): 32 result words -> plain affine x, y (16 words).
+ * *is_finite = 0 for the point at infinity (the reference's None); out is then zeroed. */
+int zkg_xyzz_to_affine(const uint32_t* words32, uint32_t* out_xy16, int* is_finite);
+/* xyzz_to_affine_g2 (src/msm.rs:356-373): 64 result words -> (x.a, x.b, y.a, y.b). */
+int zkg_xyzz_to_affine_g2(const uint32_t* words64, uint32_t* out_xy32, int* is_finite);
+
+/* ------------------------------------------------------------------ NTT */
+
+/* Ntt::new (src/ntt.rs:24-40): plan for 2^log_n; "log_n must be in 10..=27" otherwise. */
+int zkg_ntt_create(zkg_device* dev, uint32_t log_n, zkg_ntt** out);
+int zkg_ntt_destroy(zkg_ntt* plan);
+/* Ntt::log_n (src/ntt.rs:21). */
+uint32_t zkg_ntt_log_n(const zkg_ntt* plan);
+/* Ntt::forward (src/ntt.rs:42-45): in place, natural order in and out. Errors:
+ * "data buffer holds {} bytes; a 2^{} transform needs {}". */
+int zkg_ntt_forward(zkg_ntt* plan, zkg_buffer* data);
+/* Ntt::inverse (src/ntt.rs:47-50): in place, includes the 1/n scaling. */
+int zkg_ntt_inverse(zkg_ntt* plan, zkg_buffer* data);
+
+/* ------------------------------------------------------------------ test and benchmark hooks
+ * No counterpart in the reference's public API. They play the role of the reference's WGSL test
+ * harness (tests/curve.rs:8-30) and field benchmark (examples/field_bench.rs:10-97). */
+
+/* Element-wise field op on `count` plain elements: field 0 = Fq, 1 = Fr, 2 = Fq2 (16 words);
+ * op 0 = mul, 1 = add, 2 = sub, 3 = square. Synchronous; host arrays in and out. */
+int zkg_test_field_op(zkg_device* dev, int field, int op, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                      size_t count);
+/* Point ops on `count` pairs of plain affine points (infinity = zeros), results as plain affine:
+ * op 0 = P+Q (XYZZ add), 1 = 2P (XYZZ double), 2 = P+Q (mixed add), 3 = (P+Q)+2P. */
+int zkg_test_point_op(zkg_device* dev, int group, int op, const uint32_t* p, const uint32_t* q, uint32_t* out,
+                      size_t count);
+/* Field-multiply throughput: `threads` lanes each run `chain` dependent Montgomery products,
+ * `ways` independent chains per lane. Reports kernel milliseconds (HIP events). */
+int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t chain, uint32_t ways, double* ms);
+/* Fixed-base multiples out[i] = ks[i] * generator (plain affine, infinity = zeros), for building
+ * large synthetic inputs on the device. ks: n scalars (8 words each). */
+int zkg_util_generator_multiples(zkg_device* dev, int group, const zkg_buffer* ks, size_t n, zkg_buffer* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKGPU_H */

@@ -1,0 +1,377 @@
+// C ABI (include/zkgpu.h): device, buffers, NTT, and host-side affine conversion.
+// Every entry point catches library exceptions and turns them into a status code plus a
+// thread-local message, so no C++ exception crosses the boundary.
+#include "../../include/zkgpu.h"
+
+#include <string.h>
+
+#include <string>
+
+#include "api_types.hpp"
+#include "host_math.hpp"
+#include "ntt.hpp"
+#include "runtime.hpp"
+
+struct zkg_ntt {
+    zkg::NttPlan* plan;
+};
+
+namespace zkg {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+}  // namespace zkg
+
+extern "C" {
+
+const char* zkg_last_error(void) { return zkg::g_last_error.c_str(); }
+
+int zkg_version(void) { return 100; }
+
+int zkg_device_open(int ordinal, zkg_device** out) {
+    ZKG_REQUIRE(out, "null output pointer");
+    ZKG_GUARD_BEGIN
+    zkg::Device* d = zkg::device_open(ordinal);
+    *out = new zkg_device{d};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_device_close(zkg_device* dev) {
+    if (!dev) return zkg::ZKG_OK;
+    zkg::device_close(dev->d);
+    delete dev;
+    return zkg::ZKG_OK;
+}
+
+int zkg_device_info(zkg_device* dev, int* cu_count, int* clock_khz, size_t* total_mem, char* name,
+                    size_t name_len) {
+    ZKG_REQUIRE(dev, "null device");
+    if (cu_count) *cu_count = dev->d->cu_count;
+    if (clock_khz) *clock_khz = dev->d->clock_khz;
+    if (total_mem) *total_mem = dev->d->total_mem;
+    if (name && name_len) {
+        strncpy(name, dev->d->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return zkg::ZKG_OK;
+}
+
+void* zkg_device_stream(zkg_device* dev) { return dev ? (void*)dev->d->stream : nullptr; }
+
+int zkg_buffer_create(zkg_device* dev, const uint32_t* words, size_t n_words, zkg_buffer** out) {
+    ZKG_REQUIRE(dev && out && (words || n_words == 0), "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::Buffer* b = zkg::buffer_alloc(dev->d, 4 * n_words);
+    if (n_words) {
+        hipError_t e = hipMemcpyAsync(b->ptr, words, 4 * n_words, hipMemcpyHostToDevice, dev->d->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(dev->d->stream);  // `words` may be freed on return
+        if (e != hipSuccess) {
+            zkg::buffer_free(b);
+            return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("upload: ") + hipGetErrorString(e));
+        }
+    }
+    *out = new zkg_buffer{b};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_buffer_empty(zkg_device* dev, size_t n_words, zkg_buffer** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::Buffer* b = zkg::buffer_alloc(dev->d, 4 * n_words);
+    if (n_words) {
+        hipError_t e = hipMemsetAsync(b->ptr, 0, 4 * n_words, dev->d->stream);
+        if (e != hipSuccess) {
+            zkg::buffer_free(b);
+            return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("memset: ") + hipGetErrorString(e));
+        }
+    }
+    *out = new zkg_buffer{b};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_buffer_wrap(zkg_device* dev, void* device_ptr, size_t bytes, zkg_buffer** out) {
+    ZKG_REQUIRE(dev && out && (device_ptr || bytes == 0), "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::Buffer* b = new zkg::Buffer();
+    b->dev = dev->d;
+    b->ptr = device_ptr;
+    b->bytes = bytes;
+    b->owned = false;
+    *out = new zkg_buffer{b};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_buffer_read(zkg_buffer* buf, size_t n_words, uint32_t* out) {
+    ZKG_REQUIRE(buf && (out || n_words == 0), "null argument");
+    if (4 * n_words > buf->b->bytes)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "read of " + std::to_string(4 * n_words) + " bytes from a buffer of " +
+                                                 std::to_string(buf->b->bytes));
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(buf->b->dev);
+    ZKG_HIP(hipMemcpyAsync(out, buf->b->ptr, 4 * n_words, hipMemcpyDeviceToHost, buf->b->dev->stream));
+    hipError_t e = hipStreamSynchronize(buf->b->dev->stream);
+    if (e != hipSuccess) return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_buffer_write(zkg_buffer* buf, size_t offset_words, const uint32_t* words, size_t n_words) {
+    ZKG_REQUIRE(buf && (words || n_words == 0), "null argument");
+    if (4 * (offset_words + n_words) > buf->b->bytes) return zkg::fail(zkg::ZKG_ERR_SIZE, "write past the end of a buffer");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(buf->b->dev);
+    ZKG_HIP(hipMemcpyAsync(static_cast<uint32_t*>(buf->b->ptr) + offset_words, words, 4 * n_words,
+                           hipMemcpyHostToDevice, buf->b->dev->stream));
+    ZKG_HIP(hipStreamSynchronize(buf->b->dev->stream));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+size_t zkg_buffer_size(const zkg_buffer* buf) { return buf ? buf->b->bytes : 0; }
+
+void* zkg_buffer_device_ptr(const zkg_buffer* buf) { return buf ? buf->b->ptr : nullptr; }
+
+int zkg_buffer_free(zkg_buffer* buf) {
+    if (!buf) return zkg::ZKG_OK;
+    zkg::buffer_free(buf->b);
+    delete buf;
+    return zkg::ZKG_OK;
+}
+
+int zkg_wait(zkg_device* dev) {
+    ZKG_REQUIRE(dev, "null device");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    ZKG_HIP(hipStreamSynchronize(dev->d->stream));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+// ------------------------------------------------------------------ NTT
+
+int zkg_ntt_create(zkg_device* dev, uint32_t log_n, zkg_ntt** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::NttPlan* p = new zkg::NttPlan(dev->d, log_n);
+    *out = new zkg_ntt{p};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_ntt_destroy(zkg_ntt* plan) {
+    if (!plan) return zkg::ZKG_OK;
+    (void)hipSetDevice(plan->plan->dev->ordinal);
+    (void)hipStreamSynchronize(plan->plan->dev->stream);
+    delete plan->plan;
+    delete plan;
+    return zkg::ZKG_OK;
+}
+
+uint32_t zkg_ntt_log_n(const zkg_ntt* plan) { return plan ? plan->plan->log_n : 0; }
+
+static int ntt_run(zkg_ntt* plan, zkg_buffer* data, bool inverse) {
+    ZKG_REQUIRE(plan && data, "null argument");
+    const size_t need = (size_t)32 << plan->plan->log_n;
+    if (data->b->bytes < need)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "data buffer holds " + std::to_string(data->b->bytes) +
+                                                " bytes; a 2^" + std::to_string(plan->plan->log_n) +
+                                                " transform needs " + std::to_string(need));
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(plan->plan->dev);
+    plan->plan->run(static_cast<uint32_t*>(data->b->ptr), inverse, plan->plan->dev->stream);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_ntt_forward(zkg_ntt* plan, zkg_buffer* data) { return ntt_run(plan, data, false); }
+int zkg_ntt_inverse(zkg_ntt* plan, zkg_buffer* data) { return ntt_run(plan, data, true); }
+
+// ------------------------------------------------------------------ affine conversion (host)
+// The library's Montgomery radix is 2^256: x = X * 2^-256 mod p. Then (x/zz, y/zzz).
+
+int zkg_xyzz_to_affine(const uint32_t* w, uint32_t* out, int* is_finite) {
+    ZKG_REQUIRE(w && out && is_finite, "null argument");
+    const zkg::host::Modulus& F = zkg::host::fq();
+    zkg::host::U256 c[4];
+    for (int i = 0; i < 4; ++i) c[i] = F.from_mont(zkg::host::from_words32(w + 8 * i));
+    if (zkg::host::is_zero(c[2])) {
+        memset(out, 0, 16 * sizeof(uint32_t));
+        *is_finite = 0;
+        return zkg::ZKG_OK;
+    }
+    zkg::host::to_words32(out, F.mul(c[0], F.inverse(c[2])));
+    zkg::host::to_words32(out + 8, F.mul(c[1], F.inverse(c[3])));
+    *is_finite = 1;
+    return zkg::ZKG_OK;
+}
+
+// Fq2 = Fq[u]/(u^2+1); 1/(a+bu) = (a-bu)/(a^2+b^2)   (src/msm.rs:376-387)
+static void fq2_mul(const zkg::host::Modulus& F, const zkg::host::U256 x[2], const zkg::host::U256 y[2],
+                    zkg::host::U256 o[2]) {
+    zkg::host::U256 ac = F.mul(x[0], y[0]), bd = F.mul(x[1], y[1]);
+    zkg::host::U256 ad = F.mul(x[0], y[1]), bc = F.mul(x[1], y[0]);
+    o[0] = F.sub(ac, bd);
+    o[1] = F.add(ad, bc);
+}
+
+static void fq2_inv(const zkg::host::Modulus& F, const zkg::host::U256 x[2], zkg::host::U256 o[2]) {
+    zkg::host::U256 nrm = F.add(F.mul(x[0], x[0]), F.mul(x[1], x[1]));
+    zkg::host::U256 ni = F.inverse(nrm);
+    o[0] = F.mul(x[0], ni);
+    o[1] = F.sub(zkg::host::U256{{0, 0, 0, 0}}, F.mul(x[1], ni));
+}
+
+int zkg_xyzz_to_affine_g2(const uint32_t* w, uint32_t* out, int* is_finite) {
+    ZKG_REQUIRE(w && out && is_finite, "null argument");
+    const zkg::host::Modulus& F = zkg::host::fq();
+    zkg::host::U256 c[4][2];
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 2; ++k) c[i][k] = F.from_mont(zkg::host::from_words32(w + 8 * (2 * i + k)));
+    if (zkg::host::is_zero(c[2][0]) && zkg::host::is_zero(c[2][1])) {
+        memset(out, 0, 32 * sizeof(uint32_t));
+        *is_finite = 0;
+        return zkg::ZKG_OK;
+    }
+    zkg::host::U256 zi[2], zzi[2], x[2], y[2];
+    fq2_inv(F, c[2], zi);
+    fq2_inv(F, c[3], zzi);
+    fq2_mul(F, c[0], zi, x);
+    fq2_mul(F, c[1], zzi, y);
+    zkg::host::to_words32(out, x[0]);
+    zkg::host::to_words32(out + 8, x[1]);
+    zkg::host::to_words32(out + 16, y[0]);
+    zkg::host::to_words32(out + 24, y[1]);
+    *is_finite = 1;
+    return zkg::ZKG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ MSM
+
+#include "msm.hpp"
+
+struct zkg_msm {
+    zkg::MsmPlan* plan;
+    zkg_buffer* result;  // plan-owned view of plan->result
+};
+
+extern "C" {
+
+uint32_t zkg_msm_default_window(size_t n) { return zkg::msm_default_window(n); }
+
+int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points, zkg_msm** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::MsmPlan* p = new zkg::MsmPlan(dev->d, group, n, c, item_points);
+    zkg::Buffer* rb = new zkg::Buffer();
+    rb->dev = dev->d;
+    rb->ptr = p->result;
+    rb->bytes = 4 * (size_t)p->xw;
+    rb->owned = false;
+    *out = new zkg_msm{p, new zkg_buffer{rb}};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_msm_destroy(zkg_msm* m) {
+    if (!m) return zkg::ZKG_OK;
+    (void)hipSetDevice(m->plan->dev->ordinal);
+    (void)hipStreamSynchronize(m->plan->dev->stream);
+    zkg::buffer_free(m->result->b);
+    delete m->result;
+    delete m->plan;
+    delete m;
+    return zkg::ZKG_OK;
+}
+
+int zkg_msm_group(const zkg_msm* m) { return m ? m->plan->group : 0; }
+uint32_t zkg_msm_window(const zkg_msm* m) { return m ? m->plan->c : 0; }
+size_t zkg_msm_points(const zkg_msm* m) { return m ? m->plan->n : 0; }
+
+static int upload(zkg_device* dev, const uint32_t* xy, size_t n_words, zkg_buffer** out, bool convert) {
+    ZKG_REQUIRE(dev && out && (xy || n_words == 0), "null argument");
+    ZKG_REQUIRE(n_words % 8 == 0, "coordinates are 8 words each");
+    int rc = zkg_buffer_create(dev, xy, n_words, out);
+    if (rc != zkg::ZKG_OK) return rc;
+    if (!convert || n_words == 0) return zkg::ZKG_OK;
+    ZKG_GUARD_BEGIN
+    zkg::msm_convert_bases(static_cast<uint32_t*>((*out)->b->ptr), n_words / 8, dev->d->stream);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_msm_upload_bases(zkg_device* dev, const uint32_t* plain_xy, size_t n_words, zkg_buffer** out) {
+    return upload(dev, plain_xy, n_words, out, true);
+}
+
+int zkg_msm_upload_bases_montgomery_256(zkg_device* dev, const uint32_t* xy, size_t n_words, zkg_buffer** out) {
+    return upload(dev, xy, n_words, out, false);
+}
+
+int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words) {
+    ZKG_REQUIRE(buf, "null argument");
+    ZKG_REQUIRE(n_words % 8 == 0, "coordinates are 8 words each");
+    if (4 * n_words > buf->b->bytes) return zkg::fail(zkg::ZKG_ERR_SIZE, "conversion past the end of a buffer");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(buf->b->dev);
+    zkg::msm_convert_bases(static_cast<uint32_t*>(buf->b->ptr), n_words / 8, buf->b->dev->stream);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+// Msm::check_buffers (src/msm.rs:271-282)
+static int msm_check(const zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars) {
+    ZKG_REQUIRE(m && bases && scalars, "null argument");
+    const uint64_t n = m->plan->n, point_bytes = 4ull * m->plan->aw;
+    if (scalars->b->bytes < n * 32)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "scalars buffer holds " + std::to_string(scalars->b->bytes) + " bytes; " +
+                                                std::to_string(n) + " scalars need " + std::to_string(n * 32));
+    if (bases->b->bytes < n * point_bytes)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(bases->b->bytes) + " bytes; " +
+                                                std::to_string(n) + " points need " + std::to_string(n * point_bytes));
+    return zkg::ZKG_OK;
+}
+
+int zkg_msm_run(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result) {
+    int rc = msm_check(m, bases, scalars);
+    if (rc != zkg::ZKG_OK) return rc;
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(m->plan->dev);
+    m->plan->run(static_cast<const uint32_t*>(bases->b->ptr), static_cast<const uint32_t*>(scalars->b->ptr),
+                 m->plan->dev->stream);
+    if (result) *result = m->result;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_msm_profile(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, char* names, size_t names_len,
+                    double* ms, size_t cap, size_t* count) {
+    int rc = msm_check(m, bases, scalars);
+    if (rc != zkg::ZKG_OK) return rc;
+    ZKG_REQUIRE(names && ms && count, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(m->plan->dev);
+    std::vector<std::pair<std::string, double>> st;
+    m->plan->run(static_cast<const uint32_t*>(bases->b->ptr), static_cast<const uint32_t*>(scalars->b->ptr),
+                 m->plan->dev->stream, &st);
+    size_t k = 0, off = 0;
+    for (auto& e : st) {
+        if (k >= cap || off + e.first.size() + 1 > names_len) break;
+        memcpy(names + off, e.first.c_str(), e.first.size() + 1);
+        off += e.first.size() + 1;
+        ms[k++] = e.second;
+    }
+    *count = k;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// BN254 multi-scalar multiplication (signed-digit Pippenger) for G1 and G2 on gfx950.
+//
+// Drop-in for the reference's Msm (src/msm.rs:25-342): same validation (n >= 1, 9 <= c <= 20),
+// same default window and work-item sizing, same result contract (one XYZZ point in the
+// library's Montgomery form, plan-owned, read through xyzz_to_affine).
+#pragma once
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace zkg {
+
+uint32_t msm_default_window(size_t n);          // clamp(ilog2(max(n,1)) - 5, 11, 16)
+uint32_t msm_auto_item_points(size_t n, uint32_t c);
+
+struct MsmPlan {
+    Device* dev = nullptr;
+    int group = 1;           // 1 = G1 (Fq), 2 = G2 (Fq2)
+    uint32_t n = 0, c = 0;
+    uint32_t W = 0;          // windows, ceil(255 / c)
+    uint32_t B = 0;          // buckets per window, 2^(c-1)
+    uint32_t CH = 0;         // points per work item
+    uint32_t L = 0;          // buckets per running-sum thread
+    uint32_t T = 0;          // running-sum threads per window, B / L
+    uint32_t xw = 0;         // u32 words per XYZZ point (32 or 64)
+    uint32_t aw = 0;         // u32 words per affine point (16 or 32)
+    size_t max_items = 0;    // upper bound on work items over all windows
+    uint32_t merge_rounds = 0;
+
+    // workspace (device)
+    uint32_t* counts = nullptr;      // W*B  points per (window, bucket)
+    uint32_t* cursors = nullptr;     // W*B  scatter cursors
+    uint32_t* starts = nullptr;      // W*B + 1  exclusive scan of counts (global over windows)
+    uint32_t* istarts = nullptr;     // W*B + 1  exclusive scan of items per slot
+    uint32_t* scan_tmp = nullptr;    // per-block partial sums of the scan
+    uint32_t* sorted = nullptr;      // W*n  point index | sign << 31, grouped by slot
+    uint32_t* item_slot = nullptr;   // max_items
+    uint32_t* item_pts = nullptr;    // max_items * xw  per-item partial sums
+    uint32_t* buckets = nullptr;     // W*B * xw
+    uint32_t* partials = nullptr;    // W*T * xw  running-sum results
+    uint32_t* wsums = nullptr;       // W * xw    window sums
+    uint32_t* result = nullptr;      // xw        final XYZZ point
+    void* result_handle = nullptr;   // zkg_buffer* wrapping `result` (owned by the C ABI layer)
+
+    MsmPlan(Device* d, int group, size_t n, uint32_t c, uint32_t item_points);
+    ~MsmPlan();
+    // Queue the MSM on `stream`. With `stage_ms` set, wait around every stage and record
+    // (stage name, milliseconds) as the reference's Msm::profile does.
+    void run(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream,
+             std::vector<std::pair<std::string, double>>* stage_ms = nullptr) const;
+    size_t workspace_bytes() const;
+};
+
+// Plain affine coordinates -> Montgomery form in place (`coords` elements of 8 words).
+void msm_convert_bases(uint32_t* xy, size_t coords, hipStream_t stream);
+
+}  // namespace zkg

@@ -1,0 +1,203 @@
+// Curve-arithmetic stages of the MSM (accumulate, merge, finalize, reduce, tree sum,
+// combine) as kernel templates over the coordinate field, with their launchers. Included only
+// by the translation units that instantiate them (msm_g1.hip, msm_g2_*.hip).
+#pragma once
+#include "curve.hpp"
+#include "msm.hpp"
+#include "msm_stages.hpp"
+
+namespace zkg {
+
+// ---------------------------------------------------------------- bucket accumulation
+
+template <class Fe>
+__global__ __launch_bounds__(256) void msm_accumulate_kernel(const uint32_t* __restrict__ bases,
+                                                            const uint32_t* __restrict__ sorted,
+                                                            const uint32_t* __restrict__ starts,
+                                                            const uint32_t* __restrict__ istarts,
+                                                            const uint32_t* __restrict__ counts, uint32_t S,
+                                                            uint32_t ch, uint32_t* __restrict__ item_slot,
+                                                            uint32_t* __restrict__ item_pts) {
+    typedef FeTraits<Fe> Tr;
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= istarts[S]) return;
+    // last slot whose first item is <= item (empty slots before it share its istarts value)
+    uint32_t lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (istarts[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t slot = lo;
+    const uint32_t skip = (item - istarts[slot]) * ch;
+    const uint32_t len = min(ch, counts[slot] - skip);
+    const uint32_t* entries = sorted + starts[slot] + skip;
+    Xyzz<Fe> acc = xyzz_inf<Fe>();
+    for (uint32_t j = 0; j < len; ++j) {
+        const uint32_t e = entries[j];
+        Affine<Fe> p = load_affine<Fe>(bases + (size_t)(2 * Tr::WORDS) * (e & 0x7fffffffu));
+        if (e >> 31) p.y = Tr::zero() - p.y;
+        acc = xyzz_madd(acc, p);
+    }
+    store_xyzz(item_pts + (size_t)(4 * Tr::WORDS) * item, acc);
+    item_slot[item] = slot;
+}
+
+template <class Fe>
+__global__ __launch_bounds__(256) void msm_merge_kernel(uint32_t* __restrict__ item_pts,
+                                                       const uint32_t* __restrict__ item_slot,
+                                                       const uint32_t* __restrict__ istarts, uint32_t S,
+                                                       uint32_t stride) {
+    typedef FeTraits<Fe> Tr;
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= istarts[S]) return;
+    const uint32_t slot = item_slot[item];
+    const uint32_t first = istarts[slot];
+    const uint32_t k = item - first, total = istarts[slot + 1] - first;
+    if ((total + stride - 1) / stride <= SUM_GROUP) return;
+    if (k % (stride * SUM_GROUP) != 0) return;
+    const size_t xw = 4 * Tr::WORDS;
+    Xyzz<Fe> acc = load_xyzz<Fe>(item_pts + xw * item);
+    for (uint32_t j = 1; j < SUM_GROUP; ++j) {
+        if (k + j * stride >= total) break;
+        acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (item + j * stride)));
+    }
+    store_xyzz(item_pts + xw * item, acc);
+}
+
+template <class Fe>
+__global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __restrict__ item_pts,
+                                                          const uint32_t* __restrict__ istarts, uint32_t S,
+                                                          uint32_t* __restrict__ buckets) {
+    typedef FeTraits<Fe> Tr;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= S) return;
+    const uint32_t first = istarts[slot], total = istarts[slot + 1] - first;
+    uint32_t stride = 1;
+    while ((total + stride - 1) / stride > SUM_GROUP) stride *= SUM_GROUP;
+    const size_t xw = 4 * Tr::WORDS;
+    Xyzz<Fe> acc = xyzz_inf<Fe>();
+    for (uint32_t k = 0; k < total; k += stride) acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (first + k)));
+    store_xyzz(buckets + xw * slot, acc);
+}
+
+// ---------------------------------------------------------------- bucket reduction
+
+template <class Fe>
+__device__ Xyzz<Fe> mul_small(const Xyzz<Fe>& p, uint32_t m, uint32_t bits) {
+    Xyzz<Fe> r = xyzz_inf<Fe>();
+    for (int k = (int)bits - 1; k >= 0; --k) {
+        if (!xyzz_is_inf(r)) r = xyzz_dbl(r);
+        if ((m >> k) & 1u) r = xyzz_add(r, p);
+    }
+    return r;
+}
+
+// Lane t of window w takes buckets b0 .. b0+L-1 (b0 = j*L) and returns
+// sum_k (k+1) bucket[b0+k] + b0 * sum_k bucket[b0+k]  =  sum_k (b0+k+1) bucket[b0+k].
+template <class Fe>
+__global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restrict__ buckets, uint32_t W,
+                                                        uint32_t B, uint32_t L, uint32_t c,
+                                                        uint32_t* __restrict__ partials) {
+    typedef FeTraits<Fe> Tr;
+    const uint32_t T = B / L;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= W * T) return;
+    const uint32_t w = t / T, j = t % T, b0 = j * L;
+    const size_t xw = 4 * Tr::WORDS;
+    const uint32_t* src = buckets + xw * ((size_t)w * B + b0);
+    Xyzz<Fe> run = xyzz_inf<Fe>(), tot = xyzz_inf<Fe>();
+    for (int k = (int)L - 1; k >= 0; --k) {
+        run = xyzz_add(run, load_xyzz<Fe>(src + xw * k));
+        tot = xyzz_add(tot, run);
+    }
+    store_xyzz(partials + xw * t, xyzz_add(tot, mul_small(run, b0, c)));
+}
+
+// One workgroup per window: strided serial sums, then an LDS tree.
+template <class Fe, int THREADS>
+__global__ __launch_bounds__(THREADS) void msm_window_sum_kernel(const uint32_t* __restrict__ partials, uint32_t T,
+                                                                uint32_t* __restrict__ wsums) {
+    typedef FeTraits<Fe> Tr;
+    constexpr int XW = 4 * Tr::WORDS;
+    __shared__ uint32_t lds[THREADS * XW];
+    const uint32_t w = blockIdx.x;
+    Xyzz<Fe> acc = xyzz_inf<Fe>();
+    for (uint32_t k = threadIdx.x; k < T; k += THREADS)
+        acc = xyzz_add(acc, load_xyzz<Fe>(partials + (size_t)XW * ((size_t)w * T + k)));
+    for (int off = THREADS / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x >= off && (int)threadIdx.x < 2 * off) store_xyzz(lds + XW * (threadIdx.x - off), acc);
+        __syncthreads();
+        if ((int)threadIdx.x < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XW * threadIdx.x));
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) store_xyzz(wsums + (size_t)XW * w, acc);
+}
+
+// result = sum_w 2^(c w) S_w. Lane w scales its window sum by c*w doublings, then the wave adds
+// the W scaled sums through an LDS tree. (A single lane running Horner's rule on wave-uniform
+// data would be compiled to scalar-ALU code and run several times slower.)
+template <class Fe>
+__global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restrict__ wsums, uint32_t W, uint32_t c,
+                                                        uint32_t* __restrict__ result) {
+    typedef FeTraits<Fe> Tr;
+    constexpr int XW = 4 * Tr::WORDS;
+    __shared__ uint32_t lds[64 * XW];
+    const uint32_t lane = threadIdx.x;
+    Xyzz<Fe> acc = xyzz_inf<Fe>();
+    if (lane < W) {
+        acc = load_xyzz<Fe>(wsums + (size_t)XW * lane);
+        for (uint32_t k = 0; k < c * lane; ++k) acc = xyzz_dbl(acc);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        if ((int)lane >= off && (int)lane < 2 * off) store_xyzz(lds + XW * (lane - off), acc);
+        __syncthreads();
+        if ((int)lane < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XW * lane));
+        __syncthreads();
+    }
+    if (lane == 0) store_xyzz(result, acc);
+}
+
+
+// ---------------------------------------------------------------- stage launchers
+// Each group's stages are instantiated in their own translation units (msm_g1.hip,
+// msm_g2_*.hip) so that the large, fully inlined G2 kernels compile in parallel.
+
+template <class Fe>
+void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
+    hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases, p.sorted,
+                       p.starts, p.istarts, p.counts, p.W * p.B, p.CH, p.item_slot, p.item_pts);
+}
+
+template <class Fe>
+void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
+    hipLaunchKernelGGL(msm_merge_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, p.item_pts,
+                       p.item_slot, p.istarts, p.W * p.B, stride);
+}
+
+template <class Fe>
+void launch_finalize(const MsmPlan& p, hipStream_t s) {
+    const uint32_t S = p.W * p.B;
+    hipLaunchKernelGGL(msm_finalize_kernel<Fe>, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.item_pts, p.istarts, S,
+                       p.buckets);
+}
+
+template <class Fe>
+void launch_reduce(const MsmPlan& p, hipStream_t s) {
+    hipLaunchKernelGGL(msm_reduce_kernel<Fe>, dim3(ceil_div((size_t)p.W * p.T, 256)), dim3(256), 0, s, p.buckets, p.W,
+                       p.B, p.L, p.c, p.partials);
+}
+
+template <class Fe>
+void launch_tree(const MsmPlan& p, hipStream_t s) {
+    // 64 KB of LDS per workgroup either way: 512 G1 points or 256 G2 points
+    constexpr int TH = FeTraits<Fe>::WORDS == 8 ? 512 : 256;
+    hipLaunchKernelGGL((msm_window_sum_kernel<Fe, TH>), dim3(p.W), dim3(TH), 0, s, p.partials, p.T, p.wsums);
+}
+
+template <class Fe>
+void launch_combine(const MsmPlan& p, hipStream_t s) {
+    hipLaunchKernelGGL(msm_combine_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.c, p.result);
+}
+
+}  // namespace zkg

@@ -1,0 +1,283 @@
+// NTT kernels for gfx950 and the plan that sequences them.
+//
+// Algorithm (same mathematics as the reference's ntt.wgsl:34-74, re-laid out for CDNA4):
+// radix-2 decimation-in-frequency, log_n stages split into P passes of at most TILE_LOG stages.
+// A pass loads a tile of 2^TILE_LOG elements into LDS (up to 128 KB of the 160 KB per CU), runs
+// its stages there with one barrier per stage, and writes the tile back. Element (row, column)
+// of a tile sits at  ((c >> lm) << (lm + s)) + (c & (2^lm - 1)) + (row << lm)  for global column
+// c, so columns with consecutive c are adjacent in memory: loads/stores of a tile are runs of
+// whole 32-byte elements, coalesced across the wavefront.
+//
+// The DIF output is bit-reversed. Instead of a separate permutation, the last pass (lm = 0,
+// tiles are contiguous blocks) picks its blocks so that block ids are consecutive after bit
+// reversal: block c = bitrev(wg*G + col) lands at  bitrev_s(row) * 2^(log_n - s) + wg*G + col,
+// so each output row is again a coalesced run of G elements. That pass is the only out-of-place
+// one (its write set differs from its read set); all earlier passes run in place, so the plan
+// needs one n-element scratch buffer regardless of the pass count.
+//
+// Twiddles w^i (i < n/2) are precomputed in Montgomery form while data stays plain, so the
+// Montgomery product (a - b) * tw yields the plain (a - b) * w^i. The inverse transform uses
+// w^-i twiddles and multiplies by n^-1 in the last pass's store.
+#include "field.hpp"
+#include "ntt.hpp"
+#include "host_math.hpp"
+
+#include <algorithm>
+
+namespace zkg {
+
+struct NttPassArgs {
+    const uint32_t* src;
+    uint32_t* dst;
+    const uint32_t* tw;
+    uint32_t lm, s, lg, log_n;
+    uint32_t last, scale_flag;
+    uint32_t scale[8];
+};
+
+__device__ __forceinline__ uint32_t bitrev_bits(uint32_t x, uint32_t bits) {
+    return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32u - bits));
+}
+
+// Tile slot -> global element index (non-last passes; in place).
+__device__ __forceinline__ size_t tile_addr(uint32_t wg, uint32_t row, uint32_t col, uint32_t lm,
+                                            uint32_t s, uint32_t lg) {
+    const size_t c = ((size_t)wg << lg) + col;
+    return ((c >> lm) << (lm + s)) + (c & ((1ull << lm) - 1)) + ((size_t)row << lm);
+}
+
+template <int TILE_LOG, int THREADS>
+__global__ __launch_bounds__(THREADS) void ntt_pass_kernel(NttPassArgs a) {
+    __shared__ uint4 tile[2 << TILE_LOG];  // 32 bytes per element
+    const uint32_t s = a.s, lm = a.lm, lg = a.lg, L = a.log_n;
+    const uint32_t tlog = s + lg;  // log2 of elements in this tile (<= TILE_LOG)
+    const uint32_t count = 1u << tlog;
+    const uint32_t wg = blockIdx.x;
+    const uint32_t smask = (1u << s) - 1, gmask = (1u << lg) - 1;
+
+    // ---- load (coalesced: consecutive e -> consecutive global addresses)
+    if (a.last) {
+        for (uint32_t e = threadIdx.x; e < count; e += THREADS) {
+            const uint32_t row = e & smask, col = e >> s;
+            const size_t c = bitrev_bits((wg << lg) + col, L - s);
+            const uint4* p = reinterpret_cast<const uint4*>(a.src + 8 * ((c << s) + row));
+            const uint32_t slot = (row << lg) + col;
+            tile[2 * slot] = p[0];
+            tile[2 * slot + 1] = p[1];
+        }
+    } else {
+        const uint32_t gj = min(lg, lm);  // log2 of adjacent columns sharing one row run
+        for (uint32_t e = threadIdx.x; e < count; e += THREADS) {
+            const uint32_t lo = e & ((1u << gj) - 1), row = (e >> gj) & smask, hi = e >> (gj + s);
+            const uint32_t col = (hi << gj) + lo;
+            const uint4* p = reinterpret_cast<const uint4*>(a.src + 8 * tile_addr(wg, row, col, lm, s, lg));
+            const uint32_t slot = (row << lg) + col;
+            tile[2 * slot] = p[0];
+            tile[2 * slot + 1] = p[1];
+        }
+    }
+    __syncthreads();
+
+    // ---- s radix-2 DIF stages in LDS
+    const size_t lmmask = (1ull << lm) - 1;
+    for (uint32_t st = 0; st < s; ++st) {
+        const uint32_t lh = s - 1 - st;
+        for (uint32_t q = threadIdx.x; q < count / 2; q += THREADS) {
+            const uint32_t col = q & gmask, r = q >> lg;
+            const uint32_t r_lo = r & ((1u << lh) - 1);
+            const uint32_t row_a = ((r >> lh) << (lh + 1)) + r_lo;
+            const uint32_t sa = (row_a << lg) + col, sb = sa + (1u << (lh + lg));
+            const size_t j0 = a.last ? 0 : ((((size_t)wg << lg) + col) & lmmask);
+            const size_t e = (((size_t)r_lo << lm) + j0) << (L - 1 - lh - lm);
+            Fr x, y;
+            const uint4 x0 = tile[2 * sa], x1 = tile[2 * sa + 1];
+            const uint4 y0 = tile[2 * sb], y1 = tile[2 * sb + 1];
+            x.w[0] = x0.x; x.w[1] = x0.y; x.w[2] = x0.z; x.w[3] = x0.w;
+            x.w[4] = x1.x; x.w[5] = x1.y; x.w[6] = x1.z; x.w[7] = x1.w;
+            y.w[0] = y0.x; y.w[1] = y0.y; y.w[2] = y0.z; y.w[3] = y0.w;
+            y.w[4] = y1.x; y.w[5] = y1.y; y.w[6] = y1.z; y.w[7] = y1.w;
+            const Fr t = fp_load<FrParams>(a.tw + 8 * e);
+            const Fr u = x + y;
+            const Fr v = (x - y) * t;
+            tile[2 * sa] = make_uint4(u.w[0], u.w[1], u.w[2], u.w[3]);
+            tile[2 * sa + 1] = make_uint4(u.w[4], u.w[5], u.w[6], u.w[7]);
+            tile[2 * sb] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+            tile[2 * sb + 1] = make_uint4(v.w[4], v.w[5], v.w[6], v.w[7]);
+        }
+        __syncthreads();
+    }
+
+    // ---- store
+    if (a.last) {
+        Fr scale;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) scale.w[i] = a.scale[i];
+        for (uint32_t e = threadIdx.x; e < count; e += THREADS) {
+            const uint32_t col = e & gmask, orow = e >> lg;
+            const uint32_t row = bitrev_bits(orow, s);
+            const uint32_t slot = (row << lg) + col;
+            uint4 v0 = tile[2 * slot], v1 = tile[2 * slot + 1];
+            if (a.scale_flag) {
+                Fr v;
+                v.w[0] = v0.x; v.w[1] = v0.y; v.w[2] = v0.z; v.w[3] = v0.w;
+                v.w[4] = v1.x; v.w[5] = v1.y; v.w[6] = v1.z; v.w[7] = v1.w;
+                v = v * scale;
+                v0 = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+                v1 = make_uint4(v.w[4], v.w[5], v.w[6], v.w[7]);
+            }
+            const size_t out = ((size_t)orow << (L - s)) + ((size_t)wg << lg) + col;
+            uint4* p = reinterpret_cast<uint4*>(a.dst + 8 * out);
+            p[0] = v0;
+            p[1] = v1;
+        }
+    } else {
+        const uint32_t gj = min(lg, lm);
+        for (uint32_t e = threadIdx.x; e < count; e += THREADS) {
+            const uint32_t lo = e & ((1u << gj) - 1), row = (e >> gj) & smask, hi = e >> (gj + s);
+            const uint32_t col = (hi << gj) + lo;
+            const uint32_t slot = (row << lg) + col;
+            uint4* p = reinterpret_cast<uint4*>(a.dst + 8 * tile_addr(wg, row, col, lm, s, lg));
+            p[0] = tile[2 * slot];
+            p[1] = tile[2 * slot + 1];
+        }
+    }
+}
+
+// tw[i] = hi[i >> lb] * lo[i & (2^lb - 1)] (Montgomery), i < count
+__global__ void ntt_twiddle_kernel(const uint32_t* __restrict__ hi, const uint32_t* __restrict__ lo,
+                                   uint32_t* __restrict__ out, uint32_t lb, size_t count) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const Fr h = fp_load<FrParams>(hi + 8 * (i >> lb));
+    const Fr l = fp_load<FrParams>(lo + 8 * (i & ((1ull << lb) - 1)));
+    fp_store(out + 8 * i, h * l);
+}
+
+// ---------------------------------------------------------------- plan
+
+int ntt_default_tile_log(uint32_t log_n) {
+    (void)log_n;
+    return 11;
+}
+
+static void launch_pass(int tile_log, const NttPassArgs& a, unsigned wgs, hipStream_t stream) {
+    switch (tile_log) {
+        case 10: hipLaunchKernelGGL((ntt_pass_kernel<10, 256>), dim3(wgs), dim3(256), 0, stream, a); break;
+        case 11: hipLaunchKernelGGL((ntt_pass_kernel<11, 512>), dim3(wgs), dim3(512), 0, stream, a); break;
+        case 12: hipLaunchKernelGGL((ntt_pass_kernel<12, 1024>), dim3(wgs), dim3(1024), 0, stream, a); break;
+        default: throw Error(ZKG_ERR_INTERNAL, "unsupported NTT tile size");
+    }
+    ZKG_HIP(hipGetLastError());
+}
+
+static void build_twiddles(uint32_t* out, uint32_t log_n, const host::U256& root, hipStream_t stream) {
+    const host::Modulus& F = host::fr();
+    const size_t count = (size_t)1 << (log_n - 1);
+    const uint32_t lb = (log_n - 1 + 1) / 2;  // ceil((log_n - 1) / 2)
+    const size_t lo_len = (size_t)1 << lb;
+    const size_t hi_len = (count + lo_len - 1) / lo_len;
+    std::vector<uint32_t> lo(8 * lo_len), hi(8 * hi_len);
+    const host::U256 wm = F.to_mont(root);
+    host::U256 x = F.one;  // Montgomery 1
+    for (size_t i = 0; i < lo_len; ++i) {
+        host::to_words32(&lo[8 * i], x);
+        x = F.mont(x, wm);
+    }
+    const host::U256 step = x;  // w^lo_len (Montgomery)
+    x = F.one;
+    for (size_t i = 0; i < hi_len; ++i) {
+        host::to_words32(&hi[8 * i], x);
+        x = F.mont(x, step);
+    }
+    uint32_t *dlo = nullptr, *dhi = nullptr;
+    ZKG_HIP(hipMalloc(&dlo, lo.size() * 4));
+    ZKG_HIP(hipMalloc(&dhi, hi.size() * 4));
+    ZKG_HIP(hipMemcpyAsync(dlo, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, stream));
+    ZKG_HIP(hipMemcpyAsync(dhi, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(ntt_twiddle_kernel, dim3(ceil_div(count, 256)), dim3(256), 0, stream, dhi, dlo, out, lb,
+                       count);
+    ZKG_HIP(hipGetLastError());
+    // The host vectors go out of scope: wait for the copies and the kernel before freeing.
+    ZKG_HIP(hipStreamSynchronize(stream));
+    ZKG_HIP(hipFree(dlo));
+    ZKG_HIP(hipFree(dhi));
+}
+
+NttPlan::NttPlan(Device* d, uint32_t ln, NttShape shape) : dev(d), log_n(ln) {
+    if (ln < 10 || ln > 27) throw Error(ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    tile_log = shape.tile_log ? shape.tile_log : ntt_default_tile_log(ln);
+    const size_t n = (size_t)1 << ln;
+    device_bind(d);
+    try {
+        ZKG_HIP(hipMalloc(&tw_fwd, 32 * (n / 2)));
+        ZKG_HIP(hipMalloc(&tw_inv, 32 * (n / 2)));
+        ZKG_HIP(hipMalloc(&scratch, 32 * n));
+        const host::Modulus& F = host::fr();
+        const host::U256 w = host::root_of_unity(ln);
+        build_twiddles(tw_fwd, ln, w, d->stream);
+        build_twiddles(tw_inv, ln, F.inverse(w), d->stream);
+        host::U256 nn = {{0, 0, 0, 0}};
+        nn.l[ln / 64] = 1ull << (ln % 64);
+        host::to_words32(scale, F.to_mont(F.inverse(nn)));
+    } catch (...) {
+        if (tw_fwd) (void)hipFree(tw_fwd);
+        if (tw_inv) (void)hipFree(tw_inv);
+        if (scratch) (void)hipFree(scratch);
+        throw;
+    }
+    // Split log_n stages over P passes of at most tile_log stages, as evenly as possible.
+    const uint32_t P = (ln + tile_log - 1) / tile_log;
+    uint32_t done = 0;
+    for (uint32_t i = 0; i < P; ++i) {
+        NttPass p;
+        p.s = ln / P + (i < ln % P ? 1 : 0);
+        done += p.s;
+        p.lm = ln - done;
+        p.last = (i == P - 1);
+        p.lg = std::min<uint32_t>(tile_log - p.s, ln - p.s);
+        p.wgs = 1u << (ln - p.s - p.lg);
+        p.in_place = !p.last || P == 1;
+        passes.push_back(p);
+    }
+}
+
+NttPlan::~NttPlan() {
+    (void)hipFree(tw_fwd);
+    (void)hipFree(tw_inv);
+    (void)hipFree(scratch);
+}
+
+size_t NttPlan::workspace_bytes() const { return (size_t)64 << log_n; }
+
+void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream) const {
+    const size_t P = passes.size();
+    for (size_t i = 0; i < P; ++i) {
+        const NttPass& p = passes[i];
+        NttPassArgs a;
+        // passes 0..P-3 in place on data; pass P-2 data -> scratch; pass P-1 scratch -> data
+        if (P == 1) {
+            a.src = data;
+            a.dst = data;
+        } else if (i + 2 < P) {
+            a.src = data;
+            a.dst = data;
+        } else if (i + 2 == P) {
+            a.src = data;
+            a.dst = scratch;
+        } else {
+            a.src = scratch;
+            a.dst = data;
+        }
+        a.tw = inverse ? tw_inv : tw_fwd;
+        a.lm = p.lm;
+        a.s = p.s;
+        a.lg = p.lg;
+        a.log_n = log_n;
+        a.last = p.last;
+        a.scale_flag = (inverse && p.last) ? 1 : 0;
+        for (int k = 0; k < 8; ++k) a.scale[k] = scale[k];
+        launch_pass(tile_log, a, p.wgs, stream);
+    }
+}
+
+}  // namespace zkg

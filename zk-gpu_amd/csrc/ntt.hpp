@@ -1,0 +1,47 @@
+// In-place, natural-order NTT over the BN254 scalar field Fr.
+//
+// Drop-in for the reference's Ntt (src/ntt.rs:16-68): Ntt::new(log_n) with 10 <= log_n <= 27,
+// forward = X_j = sum_i x_i w^(ij) with w = 5^((r-1) >> log_n), inverse uses w^-1 and folds in
+// the 1/n scaling; data is 8 little-endian u32 words per element, plain (not Montgomery) form.
+#pragma once
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace zkg {
+
+// One pass = `s` radix-2 decimation-in-frequency stages on LDS-resident tiles.
+struct NttPass {
+    uint32_t lm;     // log2 of the element stride inside a group
+    uint32_t s;      // stages in this pass
+    uint32_t lg;     // log2 of groups (columns) per workgroup
+    uint32_t last;   // last pass: bit-reversed, coalesced scatter (+ optional 1/n scale)
+    uint32_t wgs;    // workgroups launched
+    int in_place;    // reads and writes the same buffer
+};
+
+// Launch-shape choice for one transform size; the autotuner may override the default.
+struct NttShape {
+    int tile_log = 0;  // log2 of elements per workgroup tile (0 = library default)
+};
+
+struct NttPlan {
+    Device* dev = nullptr;
+    uint32_t log_n = 0;
+    int tile_log = 11;
+    uint32_t* tw_fwd = nullptr;   // w^i * 2^256 mod r, i < n/2
+    uint32_t* tw_inv = nullptr;   // w^-i * 2^256 mod r
+    uint32_t* scratch = nullptr;  // n elements, for the out-of-place last pass
+    uint32_t scale[8];            // n^-1 * 2^256 mod r (so a Montgomery product multiplies by n^-1)
+    std::vector<NttPass> passes;
+
+    NttPlan(Device* d, uint32_t log_n, NttShape shape = NttShape());
+    ~NttPlan();
+    // Queue a transform of `data` (device pointer holding at least 2^log_n elements) on `stream`.
+    void run(uint32_t* data, bool inverse, hipStream_t stream) const;
+    size_t workspace_bytes() const;
+};
+
+int ntt_default_tile_log(uint32_t log_n);
+
+}  // namespace zkg

@@ -1,0 +1,84 @@
+// Device runtime: one HIP device with one in-order stream, device buffers, and the error channel
+// of the C ABI.
+//
+// Mirrors the reference's device wrapper (src/gpu.rs:51-69 new/buffer/empty_buffer,
+// :151-199 read/wait): every call is queued in FIFO order on the device's stream and only
+// reading a buffer back or waiting synchronises with the host. Pipelines and bind groups have no
+// counterpart: kernels are compiled ahead of time for gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+namespace zkg {
+
+// Raised inside the library and turned into a status code + message at the C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+enum Status : int {
+    ZKG_OK = 0,
+    ZKG_ERR_INVALID = 1,   // argument out of range (same conditions as the reference's asserts)
+    ZKG_ERR_SIZE = 2,      // buffer smaller than the plan needs
+    ZKG_ERR_DEVICE = 3,    // HIP runtime error
+    ZKG_ERR_NOMEM = 4,     // allocation failure / over the admission budget
+    ZKG_ERR_INTERNAL = 5,
+};
+
+#define ZKG_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::zkg::Error(e_ == hipErrorOutOfMemory ? ::zkg::ZKG_ERR_NOMEM : ::zkg::ZKG_ERR_DEVICE, \
+                               std::string(#call) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+struct Device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    int clock_khz = 0;
+    size_t total_mem = 0;
+    std::string name;
+};
+
+struct Buffer {
+    Device* dev = nullptr;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    bool owned = true;  // false for buffers wrapping caller-owned device memory
+};
+
+Device* device_open(int ordinal);
+void device_close(Device* d);
+void device_bind(Device* d);  // make d's ordinal the calling thread's current HIP device
+
+Buffer* buffer_alloc(Device* d, size_t bytes);
+void buffer_free(Buffer* b);
+
+// Device scratch that grows on demand and is released with its owner.
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need) {
+        if (need <= bytes) return;
+        if (ptr) ZKG_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        ZKG_HIP(hipMalloc(&ptr, need));
+        bytes = need;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace zkg

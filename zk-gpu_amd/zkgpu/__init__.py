@@ -1,0 +1,15 @@
+"""zkgpu — BN254 MSM (G1, G2) and Fr NTT on AMD Instinct MI355X (gfx950).
+
+Python mirror of zerotruss/zk-gpu's public API (src/lib.rs:1-6: field, gpu, group, host, msm,
+ntt) over libzkgpu.so's C ABI (include/zkgpu.h). All compute runs in hand-written HIP kernels;
+importing this package does not touch the GPU, and every call fails loudly if the library is
+missing.
+"""
+from . import host
+from ._lib import ZkgError
+from .gpu import Buffer, Gpu
+from .group import Group
+from .msm import Msm, xyzz_to_affine, xyzz_to_affine_g2
+from .ntt import Ntt
+
+__all__ = ["Buffer", "Gpu", "Group", "Msm", "Ntt", "ZkgError", "host", "xyzz_to_affine", "xyzz_to_affine_g2"]

@@ -1,0 +1,105 @@
+"""Device and buffers — mirror of the reference's `gpu` module (src/gpu.rs).
+
+`Gpu()` / `Gpu.new()` opens the device (src/gpu.rs:51-55), `buffer`/`empty_buffer` create
+device buffers of u32 words (:57-69), `read` is the blocking readback (:192-199) and `wait`
+drains the queue (:151-155). Everything goes through libzkgpu.so's C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, u32_ptr
+
+
+class Buffer:
+    """A device buffer (the reference's wgpu::Buffer). Owned buffers are freed with the handle."""
+
+    def __init__(self, gpu: "Gpu", handle: int, owned: bool = True):
+        self._gpu = gpu  # keeps the device open while the buffer lives
+        self._h = ctypes.c_void_p(handle)
+        self._owned = owned
+
+    @property
+    def handle(self):
+        return self._h
+
+    def size(self) -> int:
+        """Capacity in bytes (wgpu::Buffer::size)."""
+        return int(_lib.load().zkg_buffer_size(self._h))
+
+    def device_ptr(self) -> int:
+        return int(_lib.load().zkg_buffer_device_ptr(self._h) or 0)
+
+    def free(self) -> None:
+        if self._owned and self._h:
+            _lib.load().zkg_buffer_free(self._h)
+        self._h = ctypes.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Gpu:
+    def __init__(self, ordinal: int = 0):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(lib.zkg_device_open(ordinal, ctypes.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def new() -> "Gpu":
+        return Gpu()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        cu, clk, mem = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        name = ctypes.create_string_buffer(64)
+        check(_lib.load().zkg_device_info(self._h, ctypes.byref(cu), ctypes.byref(clk), ctypes.byref(mem), name, 64))
+        return {"cu_count": cu.value, "clock_khz": clk.value, "total_mem": mem.value, "arch": name.value.decode()}
+
+    def stream(self) -> int:
+        """The hipStream_t (as an integer) on which the library queues this device's work."""
+        return int(_lib.load().zkg_device_stream(self._h) or 0)
+
+    def buffer(self, data) -> Buffer:
+        arr = np.ascontiguousarray(np.asarray(data, dtype=np.uint32).reshape(-1))
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_buffer_create(self._h, u32_ptr(arr), arr.size, ctypes.byref(h)))
+        return Buffer(self, h.value)
+
+    def empty_buffer(self, words: int) -> Buffer:
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_buffer_empty(self._h, int(words), ctypes.byref(h)))
+        return Buffer(self, h.value)
+
+    def wrap(self, device_ptr: int, nbytes: int) -> Buffer:
+        """View caller-owned device memory (e.g. a torch tensor) as a Buffer, without copying."""
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_buffer_wrap(self._h, ctypes.c_void_p(device_ptr), int(nbytes), ctypes.byref(h)))
+        return Buffer(self, h.value, owned=True)  # freeing the handle never frees wrapped memory
+
+    def write(self, buf: Buffer, data, offset_words: int = 0) -> None:
+        arr = np.ascontiguousarray(np.asarray(data, dtype=np.uint32).reshape(-1))
+        check(_lib.load().zkg_buffer_write(buf.handle, int(offset_words), u32_ptr(arr), arr.size))
+
+    def read(self, buf: Buffer, words: int) -> np.ndarray:
+        out = np.zeros(int(words), dtype=np.uint32)
+        check(_lib.load().zkg_buffer_read(buf.handle, int(words), u32_ptr(out)))
+        return out
+
+    def wait(self) -> None:
+        check(_lib.load().zkg_wait(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.load().zkg_device_close(self._h)
+            self._h = ctypes.c_void_p(0)

@@ -1,0 +1,122 @@
+"""MSM — mirror of the reference's `msm` module (src/msm.rs:25-387)."""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib, host
+from ._lib import check, u32_ptr
+from .gpu import Buffer
+from .group import Group
+
+Fq2Words = Tuple[host.U256, host.U256]
+
+
+class Msm:
+    """Plan for MSMs of n points in one group with window c (src/msm.rs:95-180). Not re-entrant:
+    the plan owns its workspace and the result buffer returned by `run`."""
+
+    def __init__(self, gpu, group: Group, n: int, c: int, item_points: Optional[int] = None):
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_msm_create(gpu.handle, group.value, int(n), int(c), int(item_points or 0),
+                                         ctypes.byref(h)))
+        self._h = h
+        self._gpu = gpu
+        self._group = group
+        self._result = None
+
+    @staticmethod
+    def default_window(n: int) -> int:
+        return int(_lib.load().zkg_msm_default_window(int(n)))
+
+    @staticmethod
+    def new(gpu, n: int, c: int) -> "Msm":
+        return Msm(gpu, Group.G1, n, c, None)
+
+    @staticmethod
+    def with_item_points(gpu, n: int, c: int, item_points: int) -> "Msm":
+        return Msm(gpu, Group.G1, n, c, item_points)
+
+    @staticmethod
+    def for_group(gpu, group: Group, n: int, c: int, item_points: Optional[int] = None) -> "Msm":
+        return Msm(gpu, group, n, c, item_points)
+
+    @staticmethod
+    def upload_bases(gpu, plain_xy) -> Buffer:
+        arr = np.ascontiguousarray(np.asarray(plain_xy, dtype=np.uint32).reshape(-1))
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_msm_upload_bases(gpu.handle, u32_ptr(arr), arr.size, ctypes.byref(h)))
+        return Buffer(gpu, h.value)
+
+    @staticmethod
+    def upload_bases_montgomery_256(gpu, xy) -> Buffer:
+        arr = np.ascontiguousarray(np.asarray(xy, dtype=np.uint32).reshape(-1))
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_msm_upload_bases_montgomery_256(gpu.handle, u32_ptr(arr), arr.size, ctypes.byref(h)))
+        return Buffer(gpu, h.value)
+
+    @staticmethod
+    def convert_bases(buf: Buffer, words: int) -> None:
+        """Convert plain coordinates already on the device to the library's form, in place."""
+        check(_lib.load().zkg_msm_convert_bases(buf.handle, int(words)))
+
+    def group(self) -> Group:
+        return self._group
+
+    def window(self) -> int:
+        return int(_lib.load().zkg_msm_window(self._h))
+
+    def run(self, gpu, bases: Buffer, scalars: Buffer) -> Buffer:
+        """Queue sum(scalars[i] * bases[i]); returns the plan-owned XYZZ result buffer."""
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_msm_run(self._h, bases.handle, scalars.handle, ctypes.byref(h)))
+        if self._result is None or self._result.handle.value != h.value:
+            self._result = Buffer(gpu, h.value, owned=False)
+        return self._result
+
+    def profile(self, gpu, bases: Buffer, scalars: Buffer):
+        """Run once with a wait around every stage; [(stage, milliseconds)] (src/msm.rs:321-342)."""
+        cap = 64
+        names = ctypes.create_string_buffer(4096)
+        ms = (ctypes.c_double * cap)()
+        count = ctypes.c_size_t()
+        check(_lib.load().zkg_msm_profile(self._h, bases.handle, scalars.handle, names, 4096, ms, cap,
+                                          ctypes.byref(count)))
+        labels = names.raw.split(b"\0")[: count.value]
+        return [(labels[i].decode(), ms[i]) for i in range(count.value)]
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.load().zkg_msm_destroy(self._h)
+                self._h = ctypes.c_void_p(0)
+        except Exception:
+            pass
+
+
+def _to_u256(words) -> host.U256:
+    return host.from_words32(words)
+
+
+def xyzz_to_affine(words) -> Optional[Tuple[host.U256, host.U256]]:
+    """The GPU's XYZZ result (32 words) -> plain affine (x, y) as [u64; 4] each; None = infinity."""
+    w = np.ascontiguousarray(np.asarray(words, dtype=np.uint32).reshape(-1)[:32])
+    out = np.zeros(16, dtype=np.uint32)
+    fin = ctypes.c_int()
+    check(_lib.load().zkg_xyzz_to_affine(u32_ptr(w), u32_ptr(out), ctypes.byref(fin)))
+    if not fin.value:
+        return None
+    return _to_u256(out[:8]), _to_u256(out[8:16])
+
+
+def xyzz_to_affine_g2(words) -> Optional[Tuple[Fq2Words, Fq2Words]]:
+    """The GPU's G2 XYZZ result (64 words) -> ((x.a, x.b), (y.a, y.b)); None = infinity."""
+    w = np.ascontiguousarray(np.asarray(words, dtype=np.uint32).reshape(-1)[:64])
+    out = np.zeros(32, dtype=np.uint32)
+    fin = ctypes.c_int()
+    check(_lib.load().zkg_xyzz_to_affine_g2(u32_ptr(w), u32_ptr(out), ctypes.byref(fin)))
+    if not fin.value:
+        return None
+    return (_to_u256(out[0:8]), _to_u256(out[8:16])), (_to_u256(out[16:24]), _to_u256(out[24:32]))
