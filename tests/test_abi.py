@@ -106,3 +106,18 @@ def test_device_open_without_gpu_fails_loudly():
 
     with pytest.raises(zkgpu.ZkgError, match="GPU init failed"):
         zkgpu.Gpu()
+
+
+def test_kernels_contain_no_device_calls():
+    """Every device helper is force-inlined; an out-of-line function needing relaxed long branches
+    lost its return address (s[30:31]) to them and never returned. Checks the built objects."""
+    import glob
+    import subprocess
+    import sys
+
+    objs = sorted(glob.glob(os.path.join(ROOT, "zk-gpu_amd", "_build", "obj", "*.hip.o")))
+    if not objs or not os.path.exists("/opt/rocm/llvm/bin/llvm-objdump"):
+        pytest.skip("object files or ROCm LLVM tools not present")
+    script = os.path.join(ROOT, "zk-gpu_amd", "check_code_objects.py")
+    r = subprocess.run([sys.executable, script] + objs, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

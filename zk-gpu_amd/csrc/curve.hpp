@@ -61,6 +61,18 @@ struct FeTraits<Fq2> {
     static ZKG_DEV Fq2 from_mont(const Fq2& x) { return Fq2{fp_from_mont(x.a), fp_from_mont(x.b)}; }
 };
 
+// Branch-free selects (v_cndmask per word), used to share one inlined point-operation body
+// between several operand choices.
+template <class P>
+ZKG_DEV Fp<P> fe_select(bool c, const Fp<P>& a, const Fp<P>& b) {
+    Fp<P> r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r.w[i] = c ? a.w[i] : b.w[i];
+    return r;
+}
+
+ZKG_DEV Fq2 fe_select(bool c, const Fq2& a, const Fq2& b) { return Fq2{fe_select(c, a.a, b.a), fe_select(c, a.b, b.b)}; }
+
 template <class Fe>
 struct Affine {
     Fe x, y;
@@ -169,6 +181,11 @@ ZKG_DEV Xyzz<Fe> xyzz_add(const Xyzz<Fe>& p, const Xyzz<Fe>& q) {
     const Fe x3 = (T::sqr(r) - ppp) - (qq + qq);
     const Fe y3 = r * (qq - x3) - s1 * ppp;
     return Xyzz<Fe>{x3, y3, (p.zz * q.zz) * ppsq, (p.zzz * q.zzz) * ppp};
+}
+
+template <class Fe>
+ZKG_DEV Xyzz<Fe> select_xyzz(bool c, const Xyzz<Fe>& a, const Xyzz<Fe>& b) {
+    return Xyzz<Fe>{fe_select(c, a.x, b.x), fe_select(c, a.y, b.y), fe_select(c, a.zz, b.zz), fe_select(c, a.zzz, b.zzz)};
 }
 
 // Memory format: affine = x, y (WORDS each); XYZZ = x, y, zz, zzz.

@@ -1,5 +1,10 @@
 // Device helpers shared by kernels: RAII device arrays, Fermat inversion, plain <-> Montgomery
 // affine conversion, and the group generators in Montgomery form.
+//
+// Every device function is force-inlined: the library's kernels contain no calls. An out-of-line
+// function large enough to need relaxed (s_setpc) branches has had its return address in
+// s[30:31] overwritten by those long jumps, so it never returned; the build checks that no code
+// object contains a call (check_code_objects.py).
 #pragma once
 #include "curve.hpp"
 #include "runtime.hpp"
@@ -36,7 +41,7 @@ struct FeTraits<Fr> {
 
 // x^(m-2) (Fermat inverse, Montgomery in and out); 0 maps to 0.
 template <class P>
-__device__ Fp<P> fp_inverse(const Fp<P>& x) {
+__device__ __forceinline__ Fp<P> fp_inverse(const Fp<P>& x) {
     Fp<P> acc = fp_one<P>();
     for (int word = 7; word >= 0; --word) {
         // m - 2: the low word of either BN254 modulus is >= 2, so there is no borrow
@@ -50,7 +55,7 @@ __device__ Fp<P> fp_inverse(const Fp<P>& x) {
 }
 
 // 1/(a + bu) = (a - bu)/(a^2 + b^2)
-__device__ inline Fq2 fq2_inverse(const Fq2& x) {
+__device__ __forceinline__ Fq2 fq2_inverse(const Fq2& x) {
     const Fq ni = fp_inverse(x.a * x.a + x.b * x.b);
     return Fq2{x.a * ni, fp_zero<FqParams>() - x.b * ni};
 }
@@ -60,7 +65,7 @@ __device__ __forceinline__ Fq2 fe_inverse(const Fq2& x) { return fq2_inverse(x);
 
 // XYZZ (Montgomery) -> plain affine words; infinity -> zeros.
 template <class Fe>
-__device__ void store_plain_affine(uint32_t* out, const Xyzz<Fe>& p) {
+__device__ __forceinline__ void store_plain_affine(uint32_t* out, const Xyzz<Fe>& p) {
     typedef FeTraits<Fe> T;
     if (xyzz_is_inf(p)) {
         for (int i = 0; i < 2 * T::WORDS; ++i) out[i] = 0;
@@ -73,17 +78,17 @@ __device__ void store_plain_affine(uint32_t* out, const Xyzz<Fe>& p) {
 }
 
 template <class Fe>
-__device__ Affine<Fe> load_plain_affine(const uint32_t* p) {
+__device__ __forceinline__ Affine<Fe> load_plain_affine(const uint32_t* p) {
     typedef FeTraits<Fe> T;
     const Affine<Fe> a = load_affine<Fe>(p);
     return Affine<Fe>{T::to_mont(a.x), T::to_mont(a.y)};
 }
 
 template <class Fe>
-__device__ Affine<Fe> generator();
+__device__ __forceinline__ Affine<Fe> generator();
 
 template <>
-__device__ inline Affine<Fq> generator<Fq>() {  // G1 = (1, 2)
+__device__ __forceinline__ Affine<Fq> generator<Fq>() {  // G1 = (1, 2)
     Fq x = fp_zero<FqParams>(), y = fp_zero<FqParams>();
     x.w[0] = 1;
     y.w[0] = 2;
@@ -91,7 +96,7 @@ __device__ inline Affine<Fq> generator<Fq>() {  // G1 = (1, 2)
 }
 
 template <>
-__device__ inline Affine<Fq2> generator<Fq2>() {
+__device__ __forceinline__ Affine<Fq2> generator<Fq2>() {
     // BN254 G2 generator: x.a, x.b, y.a, y.b as plain little-endian words.
     const uint32_t g[32] = {
         0xd992f6edu, 0x46debd5cu, 0xf75edaddu, 0x674322d4u, 0x5e5c4479u, 0x426a0066u, 0x121f1e76u, 0x1800deefu,

@@ -83,18 +83,11 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
 
 // ---------------------------------------------------------------- bucket reduction
 
-template <class Fe>
-__device__ Xyzz<Fe> mul_small(const Xyzz<Fe>& p, uint32_t m, uint32_t bits) {
-    Xyzz<Fe> r = xyzz_inf<Fe>();
-    for (int k = (int)bits - 1; k >= 0; --k) {
-        if (!xyzz_is_inf(r)) r = xyzz_dbl(r);
-        if ((m >> k) & 1u) r = xyzz_add(r, p);
-    }
-    return r;
-}
-
 // Lane t of window w takes buckets b0 .. b0+L-1 (b0 = j*L) and returns
 // sum_k (k+1) bucket[b0+k] + b0 * sum_k bucket[b0+k]  =  sum_k (b0+k+1) bucket[b0+k].
+// Each loop has a single point-addition site (operands chosen by selects), which keeps the
+// fully inlined kernel small: one add body for the running sums, one doubling and one add
+// body for b0 * run (double-and-add over c bits) and the final + tot.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restrict__ buckets, uint32_t W,
                                                         uint32_t B, uint32_t L, uint32_t c,
@@ -107,11 +100,22 @@ __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restr
     const size_t xw = 4 * Tr::WORDS;
     const uint32_t* src = buckets + xw * ((size_t)w * B + b0);
     Xyzz<Fe> run = xyzz_inf<Fe>(), tot = xyzz_inf<Fe>();
-    for (int k = (int)L - 1; k >= 0; --k) {
-        run = xyzz_add(run, load_xyzz<Fe>(src + xw * k));
-        tot = xyzz_add(tot, run);
+    // steps 2k: run += bucket[L-1-k]; steps 2k+1: tot += run
+    for (uint32_t step = 0; step < 2 * L; ++step) {
+        const bool odd = step & 1;
+        const Xyzz<Fe> x = select_xyzz(odd, tot, run);
+        const Xyzz<Fe> y = select_xyzz(odd, run, load_xyzz<Fe>(src + xw * (L - 1 - step / 2)));
+        const Xyzz<Fe> z = xyzz_add(x, y);
+        run = select_xyzz(odd, run, z);
+        tot = select_xyzz(odd, z, tot);
     }
-    store_xyzz(partials + xw * t, xyzz_add(tot, mul_small(run, b0, c)));
+    // r = b0 * run by double-and-add over c bits, then r + tot in one extra iteration
+    Xyzz<Fe> r = xyzz_inf<Fe>();
+    for (int k = (int)c - 1; k >= -1; --k) {
+        if (k >= 0 && !xyzz_is_inf(r)) r = xyzz_dbl(r);
+        if (k < 0 || ((b0 >> k) & 1u)) r = xyzz_add(r, select_xyzz(k < 0, tot, run));
+    }
+    store_xyzz(partials + xw * t, r);
 }
 
 // One workgroup per window: strided serial sums, then an LDS tree.
