@@ -157,6 +157,32 @@ def cpu_baseline(plain_bases_g1, ks, scalars, gpu_affine, g2, ntt_input, ntt_out
     }
 
 
+def copy_calibration():
+    """A plain copy kernel of known size (256 MiB read + 256 MiB write), so that rocprofv3's
+    FETCH_SIZE / WRITE_SIZE for it calibrate the same counters on the NTT kernel."""
+    import torch
+
+    src = torch.zeros(64 << 20, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.add(src, 1, out=dst)  # an elementwise kernel (a plain copy_ would go to the DMA engine)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 x elementwise kernel reading 256 MiB and writing 256 MiB")
+
+
+def ntt_only(gpu, args):
+    """NTT 2^24 forward only: the kernel whose HBM traffic the roofline reports."""
+    rng = np.random.default_rng(args.seed)
+    n = 1 << 24
+    x = uniform_fr(rng, n)
+    plan = Ntt(gpu, 24)
+    buf = gpu.buffer(x)
+    for _ in range(args.steps):
+        plan.forward(gpu, buf)
+    gpu.wait()
+    log(f"NTT 2^24 forward x {args.steps}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +190,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg (and its exactness gate)")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--workload", choices=["all", "ntt24", "msm"], default="all",
+                    help="ntt24/msm restrict the run to one kernel family (for rocprofv3 counter passes)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -173,6 +203,11 @@ def main():
     gpu = zkgpu.Gpu(0)
     info = gpu.info()
     log(f"device {info}")
+    if args.copy_calibration:
+        copy_calibration()
+    if args.workload == "ntt24":
+        ntt_only(gpu, args)
+        return
     ev = HipEvents(gpu.stream())
     rng = np.random.default_rng(args.seed)
     out = {}
@@ -216,7 +251,7 @@ def main():
     # ---------------- NTT 2^20, 2^22, 2^24 forward (+ inverse round trip gate)
     ntt = {}
     ntt24_in = ntt24_out = None
-    for log_n in (20, 22, 24):
+    for log_n in ((20, 22, 24) if args.workload == "all" else ()):
         n = 1 << log_n
         x = uniform_fr(rng, n)
         plan = Ntt(gpu, log_n)
@@ -249,6 +284,10 @@ def main():
     msm_gmulmod = fq_mults / (g1_ms * 1e-3) / 1e9
 
     cpu = None
+    if args.workload == "msm":
+        print(json.dumps({"g1_msm_2^20_ms": round(g1_ms, 4), "g2_msm_2^18_ms": round(g2_ms, 4),
+                          "stages": {k: round(v, 4) for k, v in stages}}), flush=True)
+        return
     if not args.no_cpu:
         log("CPU baseline (oracle) on the same inputs")
         cpu = cpu_baseline(plain_g1, ks1, sc1, g1_affine, (ks2, sc2, g2_affine), ntt24_in, ntt24_out, 24)

@@ -132,6 +132,24 @@ int zkg_ntt_forward(zkg_ntt* plan, zkg_buffer* data);
 /* Ntt::inverse (src/ntt.rs:47-50): in place, includes the 1/n scaling. */
 int zkg_ntt_inverse(zkg_ntt* plan, zkg_buffer* data);
 
+/* ------------------------------------------------------------------ batches
+ * No counterpart in the reference (one call at a time). A batch queues many MSMs and NTTs of
+ * uneven size as one unit: all NTT jobs run as grouped launches (one launch per pass phase, each
+ * workgroup finding its transform in a table), every MSM job runs on its own HIP stream, and the
+ * device stream waits for the whole batch, so it is ordered like one call. Jobs must use distinct
+ * data buffers; an MSM plan may appear only once per batch (plans are not re-entrant). */
+typedef struct zkg_batch zkg_batch;
+int zkg_batch_create(zkg_device* dev, zkg_batch** out);
+int zkg_batch_destroy(zkg_batch* batch);
+/* In-place transform of the first 2^log_n elements of `data` (10 <= log_n <= 27), as Ntt::forward
+ * / Ntt::inverse. Twiddle tables are built once per size and kept by the batch. */
+int zkg_batch_add_ntt(zkg_batch* batch, zkg_buffer* data, uint32_t log_n, int inverse);
+/* sum(scalars[i] * bases[i]) into `plan`'s result buffer, as zkg_msm_run (same size checks). */
+int zkg_batch_add_msm(zkg_batch* batch, zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars);
+/* Queue every job (asynchronous); the batch keeps its jobs and can be run again. */
+int zkg_batch_run(zkg_batch* batch);
+int zkg_batch_clear(zkg_batch* batch);
+
 /* ------------------------------------------------------------------ test and benchmark hooks
  * No counterpart in the reference's public API. They play the role of the reference's WGSL test
  * harness (tests/curve.rs:8-30) and field benchmark (examples/field_bench.rs:10-97). */

@@ -375,3 +375,68 @@ int zkg_msm_profile(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scala
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+#include "batch.hpp"
+
+struct zkg_batch {
+    zkg::Batch* b;
+};
+
+extern "C" {
+
+int zkg_batch_create(zkg_device* dev, zkg_batch** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    *out = new zkg_batch{new zkg::Batch(dev->d)};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_batch_destroy(zkg_batch* batch) {
+    if (!batch) return zkg::ZKG_OK;
+    delete batch->b;
+    delete batch;
+    return zkg::ZKG_OK;
+}
+
+int zkg_batch_add_ntt(zkg_batch* batch, zkg_buffer* data, uint32_t log_n, int inverse) {
+    ZKG_REQUIRE(batch && data, "null argument");
+    if (log_n < 10 || log_n > 27) return zkg::fail(zkg::ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    const size_t need = (size_t)32 << log_n;
+    if (data->b->bytes < need)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "data buffer holds " + std::to_string(data->b->bytes) + " bytes; a 2^" +
+                                                std::to_string(log_n) + " transform needs " + std::to_string(need));
+    ZKG_GUARD_BEGIN
+    batch->b->add_ntt(log_n, static_cast<uint32_t*>(data->b->ptr), inverse != 0);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_batch_add_msm(zkg_batch* batch, zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars) {
+    ZKG_REQUIRE(batch, "null argument");
+    int rc = msm_check(plan, bases, scalars);
+    if (rc != zkg::ZKG_OK) return rc;
+    ZKG_GUARD_BEGIN
+    batch->b->add_msm(plan->plan, static_cast<const uint32_t*>(bases->b->ptr),
+                      static_cast<const uint32_t*>(scalars->b->ptr));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_batch_run(zkg_batch* batch) {
+    ZKG_REQUIRE(batch, "null argument");
+    ZKG_GUARD_BEGIN
+    batch->b->run();
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_batch_clear(zkg_batch* batch) {
+    ZKG_REQUIRE(batch, "null argument");
+    batch->b->clear();
+    return zkg::ZKG_OK;
+}
+
+}  // extern "C"

@@ -136,6 +136,100 @@ __global__ __launch_bounds__(256) void msm_digits_kernel(const uint32_t* __restr
     }
 }
 
+// ---------------------------------------------------------------- LDS counting sort (c <= 16)
+// Device-scope atomics on the W*B global counters cost ~40 ns each at this scale (the counters
+// are shared by all XCDs), so for windows up to c = 16 the sort avoids them entirely:
+//   digits   one lane per scalar writes its W signed digits, digits[w][i] = bucket | neg << 31
+//            (NO_DIGIT for a zero digit), coalesced per window;
+//   hist     block (w, blk) builds an LDS histogram of its P points' buckets (LDS atomics) and
+//            writes it to table[w][blk][b] (coalesced);
+//   colscan  one lane per (w, b) turns its column of the table into exclusive offsets over the
+//            blocks and writes the slot total counts[w*B + b];
+//   (slot scan of counts -> starts, istarts as in the atomic path)
+//   place    block (w, blk) loads cursor[b] = starts[w*B + b] + table[w][blk][b] into LDS and
+//            places each point at sorted[atomicAdd_LDS(cursor[b], 1)].
+
+static constexpr uint32_t NO_DIGIT = 0xffffffffu;
+static constexpr int SORT_THREADS = 1024;
+
+__global__ __launch_bounds__(256) void msm_digits_store_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
+                                                              uint32_t c, uint32_t W, uint32_t* __restrict__ digits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t B = 1u << (c - 1);
+    uint32_t s[8];
+    const uint4* p = reinterpret_cast<const uint4*>(scalars + 8 * (size_t)i);
+    const uint4 a = p[0], b = p[1];
+    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+        const uint32_t v = window_bits(s, w * c, c) + carry;
+        uint32_t mag = v, neg = 0;
+        carry = 0;
+        if (v >= B) {
+            mag = (1u << c) - v;
+            neg = 1;
+            carry = 1;
+        }
+        digits[(size_t)w * n + i] = mag ? ((mag - 1) | (neg << 31)) : NO_DIGIT;
+    }
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void msm_hist_kernel(const uint32_t* __restrict__ digits, uint32_t n,
+                                                               uint32_t B, uint32_t P, uint32_t nblk,
+                                                               uint32_t* __restrict__ table) {
+    extern __shared__ uint32_t hist[];
+    const uint32_t blk = blockIdx.x, w = blockIdx.y;
+    for (uint32_t b = threadIdx.x; b < B; b += SORT_THREADS) hist[b] = 0;
+    __syncthreads();
+    const uint32_t lo = blk * P, hi = min(lo + P, n);
+    const uint32_t* d = digits + (size_t)w * n;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
+        const uint32_t e = d[i];
+        if (e != NO_DIGIT) atomicAdd(&hist[e & 0x7fffffffu], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = table + ((size_t)w * nblk + blk) * B;
+    for (uint32_t b = threadIdx.x; b < B; b += SORT_THREADS) row[b] = hist[b];
+}
+
+// table[w][blk][b] -> exclusive prefix over blk; counts[w*B + b] = column total
+__global__ __launch_bounds__(256) void msm_colscan_kernel(uint32_t* __restrict__ table, uint32_t W, uint32_t B,
+                                                         uint32_t nblk, uint32_t* __restrict__ counts) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= W * B) return;
+    const uint32_t w = t / B, b = t % B;
+    uint32_t* col = table + (size_t)w * nblk * B + b;
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < nblk; ++k) {
+        const uint32_t v = col[(size_t)k * B];
+        col[(size_t)k * B] = run;
+        run += v;
+    }
+    counts[t] = run;
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void msm_place_kernel(const uint32_t* __restrict__ digits, uint32_t n,
+                                                                uint32_t B, uint32_t P, uint32_t nblk,
+                                                                const uint32_t* __restrict__ table,
+                                                                const uint32_t* __restrict__ starts,
+                                                                uint32_t* __restrict__ sorted) {
+    extern __shared__ uint32_t cursor[];
+    const uint32_t blk = blockIdx.x, w = blockIdx.y;
+    const uint32_t* row = table + ((size_t)w * nblk + blk) * B;
+    const uint32_t* st = starts + (size_t)w * B;
+    for (uint32_t b = threadIdx.x; b < B; b += SORT_THREADS) cursor[b] = st[b] + row[b];
+    __syncthreads();
+    const uint32_t lo = blk * P, hi = min(lo + P, n);
+    const uint32_t* d = digits + (size_t)w * n;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
+        const uint32_t e = d[i];
+        if (e == NO_DIGIT) continue;
+        const uint32_t pos = atomicAdd(&cursor[e & 0x7fffffffu], 1u);
+        sorted[pos] = i | (e & 0x80000000u);
+    }
+}
+
 // ---------------------------------------------------------------- scan of counts and items
 // Three kernels: per-block exclusive scans (4096 slots per block), a single-block scan of the
 // block totals, and the add-back of block offsets. Both scans (points and items) run together.
@@ -259,7 +353,17 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     if (max_items >= 0xffffffffull) throw Error(ZKG_ERR_INVALID, "too many work items for one plan");
     for (uint64_t s = 1, m = ((uint64_t)n + CH - 1) / CH; s < m; s *= SUM_GROUP) ++merge_rounds;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    lds_sort = c <= 16;
+    sort_pts = std::max<uint32_t>(B, 8192);
+    nblk = (n + sort_pts - 1) / sort_pts;
     device_bind(d);
+    if (lds_sort) {
+        // up to 128 KB of LDS per sort block (c = 16); allow it for both LDS-sort kernels
+        ZKG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(msm_hist_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 4 * B));
+        ZKG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(msm_place_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 4 * B));
+    }
     try {
         ZKG_HIP(hipMalloc(&counts, 4 * S));
         ZKG_HIP(hipMalloc(&cursors, 4 * S));
@@ -267,6 +371,10 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
         ZKG_HIP(hipMalloc(&istarts, 4 * (S + 1)));
         ZKG_HIP(hipMalloc(&scan_tmp, 8 * nb));
         ZKG_HIP(hipMalloc(&sorted, 4 * (size_t)W * n));
+        if (lds_sort) {
+            ZKG_HIP(hipMalloc(&digits, 4 * (size_t)W * n));
+            ZKG_HIP(hipMalloc(&table, 4 * (size_t)W * nblk * B));
+        }
         ZKG_HIP(hipMalloc(&item_slot, 4 * max_items));
         ZKG_HIP(hipMalloc(&item_pts, 4 * max_items * xw));
         ZKG_HIP(hipMalloc(&buckets, 4 * S * xw));
@@ -281,16 +389,17 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
 }
 
 MsmPlan::~MsmPlan() {
-    for (uint32_t* p : {counts, cursors, starts, istarts, scan_tmp, sorted, item_slot, item_pts, buckets, partials,
-                        wsums, result})
+    for (uint32_t* p : {counts, cursors, starts, istarts, scan_tmp, sorted, digits, table, item_slot, item_pts, buckets,
+                        partials, wsums, result})
         if (p) (void)hipFree(p);
-    counts = cursors = starts = istarts = scan_tmp = sorted = item_slot = item_pts = buckets = partials = wsums =
-        result = nullptr;
+    counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_slot = item_pts = buckets =
+        partials = wsums = result = nullptr;
 }
 
 size_t MsmPlan::workspace_bytes() const {
     const size_t S = (size_t)W * B;
-    return 4 * (4 * S + 2 + (size_t)W * n + max_items * (1 + xw) + S * xw + (size_t)W * T * xw + (size_t)W * xw + xw);
+    return 4 * (4 * S + 2 + (size_t)W * n * (lds_sort ? 2 : 1) + (lds_sort ? (size_t)W * nblk * B : 0) +
+                max_items * (1 + xw) + S * xw + (size_t)W * T * xw + (size_t)W * xw + xw);
 }
 
 template <class Fe>
@@ -319,18 +428,31 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         t0 = std::chrono::steady_clock::now();
     }
     const unsigned pg = ceil_div(p.n, 256);
-    fill_u32(p.counts, 0, S, s);
-    fill_u32(p.cursors, 0, S, s);
-    hipLaunchKernelGGL((msm_digits_kernel<false>), dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.counts,
-                       (const uint32_t*)nullptr, (uint32_t*)nullptr);
+    if (p.lds_sort) {
+        hipLaunchKernelGGL(msm_digits_store_kernel, dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.digits);
+        hipLaunchKernelGGL(msm_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.n, p.B,
+                           p.sort_pts, p.nblk, p.table);
+        hipLaunchKernelGGL(msm_colscan_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.table, p.W, p.B, p.nblk,
+                           p.counts);
+    } else {
+        fill_u32(p.counts, 0, S, s);
+        fill_u32(p.cursors, 0, S, s);
+        hipLaunchKernelGGL((msm_digits_kernel<false>), dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.counts,
+                           (const uint32_t*)nullptr, (uint32_t*)nullptr);
+    }
     stage("count");
     hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(SCAN_THREADS), 0, s, p.counts, S, p.CH, p.starts, p.istarts,
                        p.scan_tmp);
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, p.scan_tmp, nb, S, p.starts, p.istarts);
     hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.scan_tmp, S, p.starts, p.istarts);
     stage("scan");
-    hipLaunchKernelGGL((msm_digits_kernel<true>), dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.cursors,
-                       p.starts, p.sorted);
+    if (p.lds_sort) {
+        hipLaunchKernelGGL(msm_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.n, p.B,
+                           p.sort_pts, p.nblk, p.table, p.starts, p.sorted);
+    } else {
+        hipLaunchKernelGGL((msm_digits_kernel<true>), dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.cursors,
+                           p.starts, p.sorted);
+    }
     stage("scatter");
     launch_accumulate<Fe>(p, bases, s);
     stage("accumulate");

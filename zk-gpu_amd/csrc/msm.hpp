@@ -28,6 +28,9 @@ struct MsmPlan {
     uint32_t aw = 0;         // u32 words per affine point (16 or 32)
     size_t max_items = 0;    // upper bound on work items over all windows
     uint32_t merge_rounds = 0;
+    bool lds_sort = false;   // c <= 16: LDS counting sort (no global atomics); else atomic counting sort
+    uint32_t sort_pts = 0;   // points per sort block
+    uint32_t nblk = 0;       // sort blocks per window
 
     // workspace (device)
     uint32_t* counts = nullptr;      // W*B  points per (window, bucket)
@@ -36,6 +39,8 @@ struct MsmPlan {
     uint32_t* istarts = nullptr;     // W*B + 1  exclusive scan of items per slot
     uint32_t* scan_tmp = nullptr;    // per-block partial sums of the scan
     uint32_t* sorted = nullptr;      // W*n  point index | sign << 31, grouped by slot
+    uint32_t* digits = nullptr;      // W*n  bucket | sign << 31 per (window, point)   (LDS sort only)
+    uint32_t* table = nullptr;       // W*nblk*B  per-block bucket counts -> offsets    (LDS sort only)
     uint32_t* item_slot = nullptr;   // max_items
     uint32_t* item_pts = nullptr;    // max_items * xw  per-item partial sums
     uint32_t* buckets = nullptr;     // W*B * xw

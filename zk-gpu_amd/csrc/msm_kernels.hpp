@@ -138,30 +138,120 @@ __global__ __launch_bounds__(THREADS) void msm_window_sum_kernel(const uint32_t*
     if (threadIdx.x == 0) store_xyzz(wsums + (size_t)XW * w, acc);
 }
 
-// result = sum_w 2^(c w) S_w. Lane w scales its window sum by c*w doublings, then the wave adds
-// the W scaled sums through an LDS tree. (A single lane running Horner's rule on wave-uniform
-// data would be compiled to scalar-ALU code and run several times slower.)
+// ---------------------------------------------------------------- window combine
+// result = sum_w 2^(c w) S_w by Horner's rule from the top window: (W-1)*c doublings in one
+// dependency chain, which no amount of parallelism across points can shorten. What can be
+// shortened is the latency of each step: the multiplications of one doubling fall into three
+// levels of independent products (V = U^2, X^2 | W = UV, S = XV, ZZ3 = V ZZ, M^2 |
+// M (S - X3), W Y, W ZZZ), so lanes 0..3 of the wave each compute one product per level and the
+// results are broadcast with lane shuffles. A doubling then costs three multiplication latencies
+// instead of nine; the additions of the chain use four levels of up to six products.
+
+template <class Fe>
+struct LaneOps;
+
+template <>
+struct LaneOps<Fq> {
+    static ZKG_DEV Fq shfl(const Fq& x, int src) {
+        Fq r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r.w[i] = __shfl(x.w[i], src);
+        return r;
+    }
+};
+
+template <>
+struct LaneOps<Fq2> {
+    static ZKG_DEV Fq2 shfl(const Fq2& x, int src) { return Fq2{LaneOps<Fq>::shfl(x.a, src), LaneOps<Fq>::shfl(x.b, src)}; }
+};
+
+// pick(lane, a0, a1, ...) = a_lane (the last candidate for lanes beyond the list)
+template <class Fe>
+ZKG_DEV Fe pick(uint32_t lane, const Fe& a0, const Fe& a1) { return fe_select(lane == 0, a0, a1); }
+template <class Fe>
+ZKG_DEV Fe pick(uint32_t lane, const Fe& a0, const Fe& a1, const Fe& a2) {
+    return fe_select(lane == 0, a0, fe_select(lane == 1, a1, a2));
+}
+template <class Fe>
+ZKG_DEV Fe pick(uint32_t lane, const Fe& a0, const Fe& a1, const Fe& a2, const Fe& a3) {
+    return fe_select(lane == 0, a0, fe_select(lane == 1, a1, fe_select(lane == 2, a2, a3)));
+}
+template <class Fe>
+ZKG_DEV Fe pick(uint32_t lane, const Fe& a0, const Fe& a1, const Fe& a2, const Fe& a3, const Fe& a4, const Fe& a5) {
+    return fe_select(lane == 0, a0,
+                     fe_select(lane == 1, a1, fe_select(lane == 2, a2, fe_select(lane == 3, a3, fe_select(lane == 4, a4, a5)))));
+}
+
+// 2P with the products of each level spread over lanes 0..3 (all lanes hold P).
+template <class Fe>
+ZKG_DEV Xyzz<Fe> xyzz_dbl_lanes(const Xyzz<Fe>& p, uint32_t lane) {
+    typedef FeTraits<Fe> T;
+    typedef LaneOps<Fe> Lx;
+    const Fe u = p.y + p.y;
+    Fe a = pick(lane, u, p.x);
+    Fe m = a * a;
+    const Fe v = Lx::shfl(m, 0), x2 = Lx::shfl(m, 1);
+    const Fe mm = (x2 + x2) + x2;
+    a = pick(lane, u, p.x, v, mm);
+    Fe b = pick(lane, v, v, p.zz, mm);
+    m = a * b;
+    const Fe w = Lx::shfl(m, 0), s = Lx::shfl(m, 1), zz3 = Lx::shfl(m, 2), m2 = Lx::shfl(m, 3);
+    const Fe x3 = m2 - (s + s);
+    a = pick(lane, mm, w, w);
+    b = pick(lane, s - x3, p.y, p.zzz);
+    m = a * b;
+    const Fe y3 = Lx::shfl(m, 0) - Lx::shfl(m, 1), zzz3 = Lx::shfl(m, 2);
+    (void)T::WORDS;
+    return Xyzz<Fe>{x3, y3, zz3, zzz3};
+}
+
+// P + Q (both XYZZ, all lanes hold both) with every special case, products spread over lanes.
+template <class Fe>
+ZKG_DEV Xyzz<Fe> xyzz_add_lanes(const Xyzz<Fe>& p, const Xyzz<Fe>& q, uint32_t lane) {
+    typedef FeTraits<Fe> T;
+    typedef LaneOps<Fe> Lx;
+    if (xyzz_is_inf(q)) return p;
+    if (xyzz_is_inf(p)) return q;
+    Fe a = pick(lane, p.x, q.x, p.y, q.y, p.zz, p.zzz);
+    Fe b = pick(lane, q.zz, p.zz, q.zzz, p.zzz, q.zz, q.zzz);
+    Fe m = a * b;
+    const Fe u1 = Lx::shfl(m, 0), u2 = Lx::shfl(m, 1), s1 = Lx::shfl(m, 2), s2 = Lx::shfl(m, 3);
+    const Fe zz12 = Lx::shfl(m, 4), zzz12 = Lx::shfl(m, 5);
+    const Fe pp = u2 - u1, r = s2 - s1;
+    if (T::is_zero(pp)) {
+        if (T::is_zero(r)) return xyzz_dbl_lanes(p, lane);
+        return xyzz_inf<Fe>();
+    }
+    a = pick(lane, pp, r);
+    m = a * a;
+    const Fe ppsq = Lx::shfl(m, 0), r2 = Lx::shfl(m, 1);
+    a = pick(lane, pp, u1, zz12);
+    m = a * ppsq;
+    const Fe ppp = Lx::shfl(m, 0), qq = Lx::shfl(m, 1), zz3 = Lx::shfl(m, 2);
+    const Fe x3 = (r2 - ppp) - (qq + qq);
+    a = pick(lane, r, s1, zzz12);
+    b = pick(lane, qq - x3, ppp, ppp);
+    m = a * b;
+    const Fe y3 = Lx::shfl(m, 0) - Lx::shfl(m, 1), zzz3 = Lx::shfl(m, 2);
+    return Xyzz<Fe>{x3, y3, zz3, zzz3};
+}
+
 template <class Fe>
 __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restrict__ wsums, uint32_t W, uint32_t c,
                                                         uint32_t* __restrict__ result) {
     typedef FeTraits<Fe> Tr;
     constexpr int XW = 4 * Tr::WORDS;
-    __shared__ uint32_t lds[64 * XW];
     const uint32_t lane = threadIdx.x;
-    Xyzz<Fe> acc = xyzz_inf<Fe>();
-    if (lane < W) {
-        acc = load_xyzz<Fe>(wsums + (size_t)XW * lane);
-        for (uint32_t k = 0; k < c * lane; ++k) acc = xyzz_dbl(acc);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        if ((int)lane >= off && (int)lane < 2 * off) store_xyzz(lds + XW * (lane - off), acc);
-        __syncthreads();
-        if ((int)lane < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XW * lane));
-        __syncthreads();
+    // every lane loads the same sums through a lane-dependent address (vector registers)
+    const uint32_t zero = lane >> 6;
+    Xyzz<Fe> acc = load_xyzz<Fe>(wsums + (size_t)XW * (W - 1 + zero));
+    for (int w = (int)W - 2; w >= 0; --w) {
+        for (uint32_t k = 0; k < c; ++k)
+            if (!xyzz_is_inf(acc)) acc = xyzz_dbl_lanes(acc, lane);
+        acc = xyzz_add_lanes(acc, load_xyzz<Fe>(wsums + (size_t)XW * (w + zero)), lane);
     }
     if (lane == 0) store_xyzz(result, acc);
 }
-
 
 // ---------------------------------------------------------------- stage launchers
 // Each group's stages are instantiated in their own translation units (msm_g1.hip,

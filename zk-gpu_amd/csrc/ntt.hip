@@ -26,15 +26,6 @@
 
 namespace zkg {
 
-struct NttPassArgs {
-    const uint32_t* src;
-    uint32_t* dst;
-    const uint32_t* tw;
-    uint32_t lm, s, lg, log_n;
-    uint32_t last, scale_flag;
-    uint32_t scale[8];
-};
-
 __device__ __forceinline__ uint32_t bitrev_bits(uint32_t x, uint32_t bits) {
     return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32u - bits));
 }
@@ -46,13 +37,12 @@ __device__ __forceinline__ size_t tile_addr(uint32_t wg, uint32_t row, uint32_t 
     return ((c >> lm) << (lm + s)) + (c & ((1ull << lm) - 1)) + ((size_t)row << lm);
 }
 
+// One pass on workgroup `wg`'s tile; `tile` is the workgroup's LDS (2 uint4 per element).
 template <int TILE_LOG, int THREADS>
-__global__ __launch_bounds__(THREADS) void ntt_pass_kernel(NttPassArgs a) {
-    __shared__ uint4 tile[2 << TILE_LOG];  // 32 bytes per element
+__device__ __forceinline__ void ntt_pass_body(const NttPassArgs& a, const uint32_t wg, uint4* __restrict__ tile) {
     const uint32_t s = a.s, lm = a.lm, lg = a.lg, L = a.log_n;
     const uint32_t tlog = s + lg;  // log2 of elements in this tile (<= TILE_LOG)
     const uint32_t count = 1u << tlog;
-    const uint32_t wg = blockIdx.x;
     const uint32_t smask = (1u << s) - 1, gmask = (1u << lg) - 1;
 
     // ---- load (coalesced: consecutive e -> consecutive global addresses)
@@ -143,6 +133,30 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_kernel(NttPassArgs a) {
     }
 }
 
+template <int TILE_LOG, int THREADS>
+__global__ __launch_bounds__(THREADS) void ntt_pass_kernel(NttPassArgs a) {
+    __shared__ uint4 tile[2 << TILE_LOG];  // 32 bytes per element
+    ntt_pass_body<TILE_LOG, THREADS>(a, blockIdx.x, tile);
+}
+
+// Many transforms in one launch: workgroup g runs tile g - wg_start[j] of job j, where j is the
+// last job with wg_start[j] <= g. Jobs may differ in size, direction and pass parameters.
+template <int TILE_LOG, int THREADS>
+__global__ __launch_bounds__(THREADS) void ntt_pass_batch_kernel(const NttPassArgs* __restrict__ jobs,
+                                                                const uint32_t* __restrict__ wg_start,
+                                                                uint32_t njobs) {
+    __shared__ uint4 tile[2 << TILE_LOG];
+    const uint32_t g = blockIdx.x;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (wg_start[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const NttPassArgs a = jobs[lo];
+    ntt_pass_body<TILE_LOG, THREADS>(a, g - wg_start[lo], tile);
+}
+
 // tw[i] = hi[i >> lb] * lo[i & (2^lb - 1)] (Montgomery), i < count
 __global__ void ntt_twiddle_kernel(const uint32_t* __restrict__ hi, const uint32_t* __restrict__ lo,
                                    uint32_t* __restrict__ out, uint32_t lb, size_t count) {
@@ -203,7 +217,7 @@ static void build_twiddles(uint32_t* out, uint32_t log_n, const host::U256& root
     ZKG_HIP(hipFree(dhi));
 }
 
-NttPlan::NttPlan(Device* d, uint32_t ln, NttShape shape) : dev(d), log_n(ln) {
+NttPlan::NttPlan(Device* d, uint32_t ln, NttShape shape, bool with_scratch) : dev(d), log_n(ln) {
     if (ln < 10 || ln > 27) throw Error(ZKG_ERR_INVALID, "log_n must be in 10..=27");
     tile_log = shape.tile_log ? shape.tile_log : ntt_default_tile_log(ln);
     const size_t n = (size_t)1 << ln;
@@ -211,7 +225,7 @@ NttPlan::NttPlan(Device* d, uint32_t ln, NttShape shape) : dev(d), log_n(ln) {
     try {
         ZKG_HIP(hipMalloc(&tw_fwd, 32 * (n / 2)));
         ZKG_HIP(hipMalloc(&tw_inv, 32 * (n / 2)));
-        ZKG_HIP(hipMalloc(&scratch, 32 * n));
+        if (with_scratch) ZKG_HIP(hipMalloc(&scratch, 32 * n));
         const host::Modulus& F = host::fr();
         const host::U256 w = host::root_of_unity(ln);
         build_twiddles(tw_fwd, ln, w, d->stream);
@@ -249,34 +263,124 @@ NttPlan::~NttPlan() {
 
 size_t NttPlan::workspace_bytes() const { return (size_t)64 << log_n; }
 
-void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream) const {
+NttPassArgs NttPlan::pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inverse) const {
     const size_t P = passes.size();
-    for (size_t i = 0; i < P; ++i) {
-        const NttPass& p = passes[i];
-        NttPassArgs a;
-        // passes 0..P-3 in place on data; pass P-2 data -> scratch; pass P-1 scratch -> data
-        if (P == 1) {
-            a.src = data;
-            a.dst = data;
-        } else if (i + 2 < P) {
-            a.src = data;
-            a.dst = data;
-        } else if (i + 2 == P) {
-            a.src = data;
-            a.dst = scratch;
-        } else {
-            a.src = scratch;
-            a.dst = data;
+    const NttPass& p = passes[i];
+    NttPassArgs a;
+    // passes 0..P-3 in place on data; pass P-2 data -> scratch; pass P-1 scratch -> data
+    if (P == 1 || i + 2 < P) {
+        a.src = data;
+        a.dst = data;
+    } else if (i + 2 == P) {
+        a.src = data;
+        a.dst = scr;
+    } else {
+        a.src = scr;
+        a.dst = data;
+    }
+    a.tw = inverse ? tw_inv : tw_fwd;
+    a.lm = p.lm;
+    a.s = p.s;
+    a.lg = p.lg;
+    a.log_n = log_n;
+    a.last = p.last;
+    a.scale_flag = (inverse && p.last) ? 1 : 0;
+    for (int k = 0; k < 8; ++k) a.scale[k] = scale[k];
+    return a;
+}
+
+void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream) const {
+    for (size_t i = 0; i < passes.size(); ++i) launch_pass(tile_log, pass_args(i, data, scratch, inverse), passes[i].wgs, stream);
+}
+
+// ---------------------------------------------------------------- batch
+
+static constexpr int BATCH_TILE_LOG = 11;
+
+NttBatch::NttBatch(Device* d) : dev_(d) {
+    device_bind(d);
+    ZKG_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
+}
+
+NttBatch::~NttBatch() {
+    (void)hipSetDevice(dev_->ordinal);
+    (void)hipEventSynchronize(staged_);
+    (void)hipEventDestroy(staged_);
+    if (staging_) (void)hipHostFree(staging_);
+    scratch_.release();
+    table_.release();
+}
+
+const NttPlan& NttBatch::plan(uint32_t log_n) {
+    auto it = plans_.find(log_n);
+    if (it == plans_.end()) {
+        NttShape shape;
+        shape.tile_log = BATCH_TILE_LOG;
+        it = plans_.emplace(log_n, std::unique_ptr<NttPlan>(new NttPlan(dev_, log_n, shape, false))).first;
+    }
+    return *it->second;
+}
+
+size_t NttBatch::add(uint32_t log_n, uint32_t* data, bool inverse) {
+    plan(log_n);  // validates the size and builds the twiddles once per size
+    jobs_.push_back(Job{log_n, data, inverse});
+    return jobs_.size() - 1;
+}
+
+void NttBatch::clear() { jobs_.clear(); }
+
+void NttBatch::run(hipStream_t stream) {
+    if (jobs_.empty()) return;
+    device_bind(dev_);
+    size_t scratch_elems = 0, max_passes = 0;
+    std::vector<size_t> offs(jobs_.size());
+    for (size_t j = 0; j < jobs_.size(); ++j) {
+        offs[j] = scratch_elems;
+        scratch_elems += (size_t)1 << jobs_[j].log_n;
+        max_passes = std::max(max_passes, plans_.at(jobs_[j].log_n)->passes.size());
+    }
+    // the previous run's upload must be done before the staging memory is rewritten
+    ZKG_HIP(hipEventSynchronize(staged_));
+    scratch_.ensure(32 * scratch_elems);
+    // table layout per phase: njobs NttPassArgs, then (njobs + 1) u32 workgroup starts (padded)
+    const size_t nj = jobs_.size();
+    const size_t start_slots = (4 * (nj + 1) + sizeof(NttPassArgs) - 1) / sizeof(NttPassArgs);
+    const size_t per_phase = nj + start_slots;
+    const size_t need = per_phase * max_passes;
+    if (need > staging_cap_) {
+        if (staging_) ZKG_HIP(hipHostFree(staging_));
+        staging_ = nullptr;
+        ZKG_HIP(hipHostMalloc(reinterpret_cast<void**>(&staging_), need * sizeof(NttPassArgs)));
+        staging_cap_ = need;
+    }
+    table_.ensure(need * sizeof(NttPassArgs));
+    std::vector<uint32_t> phase_jobs(max_passes, 0), phase_wgs(max_passes, 0);
+    for (size_t k = 0; k < max_passes; ++k) {
+        NttPassArgs* args = staging_ + k * per_phase;
+        uint32_t* starts = reinterpret_cast<uint32_t*>(args + nj);
+        uint32_t cnt = 0, wgs = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            const NttPlan& pl = *plans_.at(jobs_[j].log_n);
+            if (pl.passes.size() <= k) continue;
+            uint32_t* scr = static_cast<uint32_t*>(scratch_.ptr) + 8 * offs[j];
+            args[cnt] = pl.pass_args(k, jobs_[j].data, scr, jobs_[j].inverse);
+            starts[cnt] = wgs;
+            wgs += pl.passes[k].wgs;
+            ++cnt;
         }
-        a.tw = inverse ? tw_inv : tw_fwd;
-        a.lm = p.lm;
-        a.s = p.s;
-        a.lg = p.lg;
-        a.log_n = log_n;
-        a.last = p.last;
-        a.scale_flag = (inverse && p.last) ? 1 : 0;
-        for (int k = 0; k < 8; ++k) a.scale[k] = scale[k];
-        launch_pass(tile_log, a, p.wgs, stream);
+        starts[cnt] = wgs;
+        phase_jobs[k] = cnt;
+        phase_wgs[k] = wgs;
+    }
+    ZKG_HIP(hipMemcpyAsync(table_.ptr, staging_, need * sizeof(NttPassArgs), hipMemcpyHostToDevice, stream));
+    ZKG_HIP(hipEventRecord(staged_, stream));
+    for (size_t k = 0; k < max_passes; ++k) {
+        if (!phase_jobs[k]) continue;
+        const NttPassArgs* args = static_cast<const NttPassArgs*>(table_.ptr) + k * per_phase;
+        const uint32_t* starts = reinterpret_cast<const uint32_t*>(args + nj);
+        hipLaunchKernelGGL((ntt_pass_batch_kernel<BATCH_TILE_LOG, 512>), dim3(phase_wgs[k]), dim3(512), 0, stream, args,
+                           starts, phase_jobs[k]);
+        ZKG_HIP(hipGetLastError());
     }
 }
 

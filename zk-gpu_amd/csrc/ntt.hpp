@@ -4,11 +4,23 @@
 // forward = X_j = sum_i x_i w^(ij) with w = 5^((r-1) >> log_n), inverse uses w^-1 and folds in
 // the 1/n scaling; data is 8 little-endian u32 words per element, plain (not Montgomery) form.
 #pragma once
+#include <map>
+#include <memory>
 #include <vector>
 
 #include "runtime.hpp"
 
 namespace zkg {
+
+// Parameters of one NTT pass over one transform (kernel argument, or an entry of a batch table).
+struct NttPassArgs {
+    const uint32_t* src;
+    uint32_t* dst;
+    const uint32_t* tw;
+    uint32_t lm, s, lg, log_n;
+    uint32_t last, scale_flag;
+    uint32_t scale[8];
+};
 
 // One pass = `s` radix-2 decimation-in-frequency stages on LDS-resident tiles.
 struct NttPass {
@@ -35,11 +47,43 @@ struct NttPlan {
     uint32_t scale[8];            // n^-1 * 2^256 mod r (so a Montgomery product multiplies by n^-1)
     std::vector<NttPass> passes;
 
-    NttPlan(Device* d, uint32_t log_n, NttShape shape = NttShape());
+    // with_scratch = false: twiddles only (a batch supplies per-job scratch)
+    NttPlan(Device* d, uint32_t log_n, NttShape shape = NttShape(), bool with_scratch = true);
     ~NttPlan();
     // Queue a transform of `data` (device pointer holding at least 2^log_n elements) on `stream`.
     void run(uint32_t* data, bool inverse, hipStream_t stream) const;
+    // Pass i of a transform of `data` using `scr` as the out-of-place scratch.
+    NttPassArgs pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inverse) const;
     size_t workspace_bytes() const;
+};
+
+// Many NTTs of mixed sizes and directions in one grouped launch per pass phase: phase k runs
+// pass k of every job that has more than k passes, each workgroup finding its job in a prefix
+// table. Twiddle tables are shared per size; every job gets its own scratch region.
+class NttBatch {
+  public:
+    explicit NttBatch(Device* d);
+    ~NttBatch();
+    size_t add(uint32_t log_n, uint32_t* data, bool inverse);  // returns the job index
+    void clear();
+    size_t size() const { return jobs_.size(); }
+    void run(hipStream_t stream);
+    const NttPlan& plan(uint32_t log_n);
+
+  private:
+    struct Job {
+        uint32_t log_n;
+        uint32_t* data;
+        bool inverse;
+    };
+    Device* dev_;
+    std::vector<Job> jobs_;
+    std::map<uint32_t, std::unique_ptr<NttPlan>> plans_;
+    Scratch scratch_;                 // sum of the jobs' sizes, 32 bytes per element
+    Scratch table_;                   // device copy of the pass tables of all phases
+    NttPassArgs* staging_ = nullptr;  // pinned host staging of the tables
+    size_t staging_cap_ = 0;
+    hipEvent_t staged_ = nullptr;     // the last upload from staging has completed
 };
 
 int ntt_default_tile_log(uint32_t log_n);

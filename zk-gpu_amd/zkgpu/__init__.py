@@ -11,5 +11,6 @@ from .gpu import Buffer, Gpu
 from .group import Group
 from .msm import Msm, xyzz_to_affine, xyzz_to_affine_g2
 from .ntt import Ntt
+from .batch import Batch
 
-__all__ = ["Buffer", "Gpu", "Group", "Msm", "Ntt", "ZkgError", "host", "xyzz_to_affine", "xyzz_to_affine_g2"]
+__all__ = ["Batch", "Buffer", "Gpu", "Group", "Msm", "Ntt", "ZkgError", "host", "xyzz_to_affine", "xyzz_to_affine_g2"]

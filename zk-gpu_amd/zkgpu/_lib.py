@@ -53,6 +53,12 @@ SIGNATURES = {
     "zkg_ntt_log_n": (ctypes.c_uint32, [ctypes.c_void_p]),
     "zkg_ntt_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_ntt_inverse": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "zkg_batch_create": (ctypes.c_int, [ctypes.c_void_p, c_void_pp]),
+    "zkg_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_batch_add_ntt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int]),
+    "zkg_batch_add_msm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "zkg_batch_run": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_batch_clear": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_test_field_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_u32p, c_u32p, c_u32p,
                                          ctypes.c_size_t]),
     "zkg_test_point_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_u32p, c_u32p, c_u32p,
