@@ -265,7 +265,7 @@ def main():
         inv_ms, _ = time_calls(gpu, ev, lambda: plan.inverse(gpu, buf), max(5, args.steps), 2)
         gbps = 64.0 * n / (fwd_ms * 1e-3) / 1e9
         ntt[log_n] = {"forward_ms": round(fwd_ms, 4), "inverse_ms": round(inv_ms, 4), "hbm_GBps": round(gbps, 1),
-                      "passes": 2 if log_n > 11 else 1}
+                      "passes": plan.passes()}
         log(f"NTT 2^{log_n}: fwd {fwd_ms:.3f} ms inv {inv_ms:.3f} ms -> {gbps:.1f} GB/s")
         if log_n == 24:
             ntt24_in, ntt24_out = x, y

@@ -109,6 +109,8 @@ int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words);
  * plan-owned buffer holding one XYZZ point (32 words G1, 64 words G2, Montgomery form), valid
  * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...". */
 int zkg_msm_run(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
+/* The plan-owned result buffer (what zkg_msm_run returns), e.g. after a batch ran the plan. */
+int zkg_msm_result(zkg_msm* plan, zkg_buffer** result);
 /* Msm::profile (src/msm.rs:321-342): run once, synchronising around every stage; writes up to
  * `cap` (stage name, milliseconds) pairs, names as NUL-separated text in `names`. */
 int zkg_msm_profile(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, char* names,
@@ -126,6 +128,8 @@ int zkg_ntt_create(zkg_device* dev, uint32_t log_n, zkg_ntt** out);
 int zkg_ntt_destroy(zkg_ntt* plan);
 /* Ntt::log_n (src/ntt.rs:21). */
 uint32_t zkg_ntt_log_n(const zkg_ntt* plan);
+/* Kernel launches (LDS-tiled passes) per transform of this plan. */
+uint32_t zkg_ntt_passes(const zkg_ntt* plan);
 /* Ntt::forward (src/ntt.rs:42-45): in place, natural order in and out. Errors:
  * "data buffer holds {} bytes; a 2^{} transform needs {}". */
 int zkg_ntt_forward(zkg_ntt* plan, zkg_buffer* data);

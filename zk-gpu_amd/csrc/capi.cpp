@@ -175,6 +175,7 @@ int zkg_ntt_destroy(zkg_ntt* plan) {
 }
 
 uint32_t zkg_ntt_log_n(const zkg_ntt* plan) { return plan ? plan->plan->log_n : 0; }
+uint32_t zkg_ntt_passes(const zkg_ntt* plan) { return plan ? (uint32_t)plan->plan->passes.size() : 0; }
 
 static int ntt_run(zkg_ntt* plan, zkg_buffer* data, bool inverse) {
     ZKG_REQUIRE(plan && data, "null argument");
@@ -350,6 +351,12 @@ int zkg_msm_run(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, 
     if (result) *result = m->result;
     return zkg::ZKG_OK;
     ZKG_GUARD_END
+}
+
+int zkg_msm_result(zkg_msm* m, zkg_buffer** result) {
+    ZKG_REQUIRE(m && result, "null argument");
+    *result = m->result;
+    return zkg::ZKG_OK;
 }
 
 int zkg_msm_profile(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, char* names, size_t names_len,

@@ -43,6 +43,7 @@ SIGNATURES = {
     "zkg_msm_upload_bases_montgomery_256": (ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_size_t, c_void_pp]),
     "zkg_msm_convert_bases": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
     "zkg_msm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_void_pp]),
+    "zkg_msm_result": (ctypes.c_int, [ctypes.c_void_p, c_void_pp]),
     "zkg_msm_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
                                        ctypes.c_size_t, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_size_t)]),
@@ -51,6 +52,7 @@ SIGNATURES = {
     "zkg_ntt_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_void_pp]),
     "zkg_ntt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_ntt_log_n": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "zkg_ntt_passes": (ctypes.c_uint32, [ctypes.c_void_p]),
     "zkg_ntt_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_ntt_inverse": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_batch_create": (ctypes.c_int, [ctypes.c_void_p, c_void_pp]),

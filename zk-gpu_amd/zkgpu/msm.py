@@ -68,6 +68,14 @@ class Msm:
     def window(self) -> int:
         return int(_lib.load().zkg_msm_window(self._h))
 
+    def result_buffer(self, gpu) -> Buffer:
+        """The plan-owned result buffer (written by `run` or by a batch that ran this plan)."""
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_msm_result(self._h, ctypes.byref(h)))
+        if self._result is None or self._result.handle.value != h.value:
+            self._result = Buffer(gpu, h.value, owned=False)
+        return self._result
+
     def run(self, gpu, bases: Buffer, scalars: Buffer) -> Buffer:
         """Queue sum(scalars[i] * bases[i]); returns the plan-owned XYZZ result buffer."""
         h = ctypes.c_void_p()

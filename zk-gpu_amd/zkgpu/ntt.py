@@ -21,6 +21,10 @@ class Ntt:
     def new(gpu, log_n: int) -> "Ntt":
         return Ntt(gpu, log_n)
 
+    def passes(self) -> int:
+        """Kernel launches (LDS-tiled passes) per transform."""
+        return int(_lib.load().zkg_ntt_passes(self._h))
+
     def forward(self, gpu, data) -> None:
         """In-place forward transform: natural order in, natural order out (asynchronous)."""
         check(_lib.load().zkg_ntt_forward(self._h, data.handle))
