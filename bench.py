@@ -157,6 +157,270 @@ def cpu_baseline(plain_bases_g1, ks, scalars, gpu_affine, g2, ntt_input, ntt_out
     }
 
 
+def fast_ks(rng, n):
+    """n base multipliers k_i uniform in [0, 2^253) (< r), without rejection sampling."""
+    k = rng.integers(0, 1 << 32, size=(n, 8), dtype=np.uint64).astype(np.uint32)
+    k[:, 7] &= 0x1FFFFFFF
+    return k.reshape(-1)
+
+
+def witness_like(rng, n):
+    """Groth16-witness-like scalars: 40% in {0, 1}, 30% uniform below 2^16, 30% uniform in [0, r)."""
+    u = uniform_fr(rng, n).reshape(n, 8)
+    kind = rng.integers(0, 10, size=n)
+    out = np.zeros((n, 8), dtype=np.uint32)
+    out[kind < 4, 0] = rng.integers(0, 2, size=int((kind < 4).sum()), dtype=np.uint64).astype(np.uint32)
+    small = (kind >= 4) & (kind < 7)
+    out[small, 0] = rng.integers(0, 1 << 16, size=int(small.sum()), dtype=np.uint64).astype(np.uint32)
+    out[kind >= 7] = u[kind >= 7]
+    return out.reshape(-1)
+
+
+def library_bases(gpu, group, ks):
+    bb = device_bases(gpu, group, ks)
+    Msm.convert_bases(bb, 16 * group * (ks.size // 8))
+    return bb
+
+
+def batched_ntt_leg(gpu, ev, rng, args):
+    """64 polynomials, log2 sizes uniform in 12..22, half forward and half inverse: one batch
+    (one grouped launch per pass phase) against 64 sequential plain calls."""
+    from zkgpu import Batch
+
+    sizes = [int(k) for k in rng.integers(12, 23, size=64)]
+    inverse = [j % 2 == 1 for j in range(64)]
+    rng.shuffle(inverse)
+    pool = uniform_fr(rng, 1 << 22)
+    bufs_b = [gpu.buffer(pool[: 8 << k]) for k in sizes]
+    bufs_s = [gpu.buffer(pool[: 8 << k]) for k in sizes]
+    plans = {k: Ntt(gpu, k) for k in set(sizes)}
+    batch = Batch(gpu)
+    for k, inv, b in zip(sizes, inverse, bufs_b):
+        batch.add_ntt(b, k, inverse=inv)
+
+    def sequential():
+        for k, inv, b in zip(sizes, inverse, bufs_s):
+            (plans[k].inverse if inv else plans[k].forward)(gpu, b)
+
+    batch.run()
+    sequential()
+    gpu.wait()
+    for j, (k, bb, bs) in enumerate(zip(sizes, bufs_b, bufs_s)):
+        if not np.array_equal(gpu.read(bb, 8 << k), gpu.read(bs, 8 << k)):
+            raise SystemExit(f"batched NTT {j} (2^{k}) differs from the plain call: refusing to report a time")
+    steps = max(5, args.steps)
+    seq_ms, _ = time_calls(gpu, ev, sequential, steps, 2)
+    bat_ms, _ = time_calls(gpu, ev, batch.run, steps, 2)
+    elems = sum(1 << k for k in sizes)
+    res = {"polys": 64, "log_n": sizes, "elements": elems, "sequential_ms": round(seq_ms, 4),
+           "batched_ms": round(bat_ms, 4), "speedup": round(seq_ms / bat_ms, 3)}
+    log(f"batched NTT x64: sequential {seq_ms:.3f} ms, batched {bat_ms:.3f} ms ({seq_ms / bat_ms:.2f}x)")
+    return res
+
+
+def groth16_msm_leg(gpu, ev, rng, args, log_n=22):
+    """A Groth16 proof's MSM set at n = 2^22: A, B1, C (G1, n), H (G1, n - 1), B2 (G2, n) with
+    witness-like scalars (H: uniform), bases uploaded once. Batched (the five MSMs overlapped on
+    their own streams) against five plain calls in sequence."""
+    from zkgpu import Batch
+
+    n = 1 << log_n
+    c = Msm.default_window(n)
+    sets = [(1, n), (1, n), (1, n), (1, n - 1), (2, n)]
+    bases = [library_bases(gpu, g, fast_ks(rng, m)) for g, m in sets]
+    wit = gpu.buffer(witness_like(rng, n))
+    hsc = gpu.buffer(uniform_fr(rng, n - 1))
+    scal = [wit, wit, wit, hsc, wit]
+    plans = [Msm.for_group(gpu, Group(g), m, c) for g, m in sets]
+    batch = Batch(gpu)
+    for p, b, sc in zip(plans, bases, scal):
+        batch.add_msm(p, b, sc)
+
+    def plain():
+        for p, b, sc in zip(plans, bases, scal):
+            p.run(gpu, b, sc)
+
+    plain()
+    gpu.wait()
+    want = [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(plans, sets)]
+    batch.run()
+    gpu.wait()
+    got = [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(plans, sets)]
+    if not all(np.array_equal(a, b) for a, b in zip(want, got)):
+        raise SystemExit("batched Groth16 MSM set differs from the plain calls: refusing to report a time")
+    steps = max(3, args.steps // 5)
+    plain_ms, _ = time_calls(gpu, ev, plain, steps, 1)
+    bat_ms, _ = time_calls(gpu, ev, batch.run, steps, 1)
+    res = {"n": n, "c": c, "msms": "A,B1,C,H(n-1) G1 + B2 G2", "plain_ms": round(plain_ms, 3),
+           "batched_ms": round(bat_ms, 3), "speedup": round(plain_ms / bat_ms, 3)}
+    log(f"Groth16 MSM set 2^{log_n}: plain {plain_ms:.2f} ms, batched {bat_ms:.2f} ms ({plain_ms / bat_ms:.2f}x)")
+    del batch, plans, bases
+    return res
+
+
+def serving_leg(gpu, rng, args):
+    """Poisson arrivals of proof jobs over 8 device-resident proving keys (log_n uniform in
+    14..24): each job = 7 NTTs of size n (random directions) + the Groth16 MSM set of size n.
+    The same arrival schedule is served (a) one call at a time through the plain API and (b) by
+    the continuous-batching server; jobs/s at saturating load and p50/p99 latency at moderate load."""
+    import queue
+    import threading
+
+    from zkgpu import Server
+
+    n_keys, ring = 8, 3
+    log_ns = [int(k) for k in rng.integers(14, 25, size=n_keys)]
+    top = 1 << max(log_ns)
+    ks_pool = fast_ks(rng, top + 64)
+    poly_pool = uniform_fr(rng, top)
+    wit_pool = witness_like(rng, top)
+    keys = []
+    off = 0
+    for ln in log_ns:
+        n = 1 << ln
+        g1 = []
+        for m in (n, n, n, n - 1):
+            g1.append(library_bases(gpu, 1, ks_pool[8 * off: 8 * (off + m)]))
+            off = (off + 7) % 64
+        g2 = library_bases(gpu, 2, ks_pool[8 * off: 8 * (off + n)])
+        off = (off + 7) % 64
+        sets = [[gpu.buffer(poly_pool[: 8 * n]) for _ in range(7)] for _ in range(ring)]
+        keys.append({"log_n": ln, "g1": g1, "g2": g2, "wit": gpu.buffer(wit_pool[: 8 * n]), "sets": sets})
+    gpu.wait()
+    log(f"serving: 8 keys at log_n {log_ns} resident")
+
+    plain_plans = {}
+
+    def plans_for(ln):
+        if ln not in plain_plans:
+            n = 1 << ln
+            c = Msm.default_window(n)
+            plain_plans[ln] = (Ntt(gpu, ln), [Msm.new(gpu, n, c) for _ in range(3)] + [Msm.new(gpu, n - 1, c)]
+                               + [Msm.for_group(gpu, Group.G2, n, c)])
+        return plain_plans[ln]
+
+    def plain_job(key, mask, polys):
+        ntt, msms = plans_for(key["log_n"])
+        for p, b in enumerate(polys):
+            (ntt.inverse if (mask >> p) & 1 else ntt.forward)(gpu, b)
+        scal = [key["wit"]] * 3 + [polys[6], key["wit"]]
+        bases = key["g1"] + [key["g2"]]
+        res = [m.run(gpu, b, sc) for m, b, sc in zip(msms, bases, scal)]
+        return [gpu.read(r, 32 if i < 4 else 64) for i, r in enumerate(res)]
+
+    # service time of each key on the plain path (warm)
+    service = []
+    for key in keys:
+        plain_job(key, 0, key["sets"][0])
+        t0 = time.perf_counter()
+        plain_job(key, 0, key["sets"][0])
+        service.append(time.perf_counter() - t0)
+    mean_service = float(np.mean(service))
+    log(f"serving: plain service times {[round(1e3 * s, 2) for s in service]} ms, mean {1e3 * mean_service:.2f}")
+
+    jobs_n = args.serve_jobs
+    choice = [int(k) for k in rng.integers(0, n_keys, size=jobs_n)]
+    masks = [int(m) for m in rng.integers(0, 128, size=jobs_n)]
+    gaps = rng.exponential(1.0, size=jobs_n)
+
+    def schedule(rate):
+        t = np.cumsum(gaps / rate)
+        return t - t[0]
+
+    def run_plain(arrivals):
+        lat = []
+        t0 = time.perf_counter()
+        for j in range(jobs_n):
+            now = time.perf_counter() - t0
+            if now < arrivals[j]:
+                time.sleep(arrivals[j] - now)
+            key = keys[choice[j]]
+            plain_job(key, masks[j], key["sets"][j % ring])
+            lat.append((time.perf_counter() - t0 - arrivals[j]) * 1e3)
+        span = time.perf_counter() - t0
+        return {"jobs_per_s": round(jobs_n / span, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_ms": round(float(np.mean(lat)), 3)}
+
+    server = Server(gpu, args.serve_budget_gib << 30, max_inflight=args.serve_inflight)
+    kid = [server.add_key(k["log_n"], k["g1"], k["g2"]) for k in keys]
+
+    def run_server(arrivals):
+        server.reset_stats()
+        done_ev = [threading.Event() for _ in range(jobs_n)]
+        q = queue.Queue()
+        errors = []
+
+        def waiter():
+            try:
+                for _ in range(jobs_n):
+                    j, jid = q.get()
+                    server.wait(jid)
+                    done_ev[j].set()
+            except Exception as e:  # noqa: BLE001 - reported below
+                errors.append(e)
+                for e2 in done_ev:
+                    e2.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        last_use = {}
+        for j in range(jobs_n):
+            now = time.perf_counter() - t0
+            if now < arrivals[j]:
+                time.sleep(arrivals[j] - now)
+            key = keys[choice[j]]
+            slot = (choice[j], j % ring)
+            if slot in last_use:  # the ring's input buffers are reused once their job is done
+                done_ev[last_use[slot]].wait()
+            last_use[slot] = j
+            q.put((j, server.submit(kid[choice[j]], key["sets"][j % ring], masks[j], key["wit"])))
+        th.join()
+        if errors:
+            raise errors[0]
+        st = server.stats()
+        return {"jobs_per_s": round(st["jobs_per_s"], 3), "p50_ms": round(st["p50_ms"], 3),
+                "p99_ms": round(st["p99_ms"], 3), "mean_ms": round(st["mean_ms"], 3),
+                "peak_inflight": st["peak_inflight"], "peak_GiB": round(st["peak_bytes"] / 2 ** 30, 2)}
+
+    # warm the server's slots for every key size
+    warm = [server.submit(kid[i], keys[i]["sets"][0], 0, keys[i]["wit"]) for i in range(n_keys)]
+    for w in warm:
+        server.wait(w)
+    # gate: a job served in the mix gives the same results as the same job alone on the plain path
+    key = keys[choice[0]]
+    for b in key["sets"][1]:
+        gpu.write(b, poly_pool[: 8 << key["log_n"]])
+    for b in key["sets"][2]:
+        gpu.write(b, poly_pool[: 8 << key["log_n"]])
+    gpu.wait()
+    mix = [server.submit(kid[i], keys[i]["sets"][0], masks[i], keys[i]["wit"]) for i in range(n_keys) if i != choice[0]]
+    jid = server.submit(kid[choice[0]], key["sets"][1], masks[0], key["wit"])
+    words, _ = server.wait(jid)
+    for m in mix:
+        server.wait(m)
+    alone = plain_job(key, masks[0], key["sets"][2])
+    served = [words[0:32], words[32:64], words[64:96], words[96:128], words[128:192]]
+    n = 1 << key["log_n"]
+    if not all(np.array_equal(affine_words(a, 1 if i < 4 else 2), affine_words(b, 1 if i < 4 else 2))
+               for i, (a, b) in enumerate(zip(served, alone))) or not all(
+            np.array_equal(gpu.read(x, 8 * n), gpu.read(y, 8 * n)) for x, y in zip(key["sets"][1], key["sets"][2])):
+        raise SystemExit("a served job differs from the same job run alone: refusing to report a time")
+
+    out = {"keys_log_n": log_ns, "jobs": jobs_n, "mean_plain_service_ms": round(1e3 * mean_service, 3),
+           "inflight": args.serve_inflight, "budget_GiB": args.serve_budget_gib}
+    for name, load in (("saturating", 4.0), ("moderate", 0.6)):
+        arrivals = schedule(load / mean_service)
+        p = run_plain(arrivals)
+        sv = run_server(arrivals)
+        out[name] = {"offered_jobs_per_s": round(load / mean_service, 3), "plain": p, "server": sv,
+                     "throughput_x": round(sv["jobs_per_s"] / p["jobs_per_s"], 3)}
+        log(f"serving {name}: plain {p}, server {sv}")
+    server.close()
+    plain_plans.clear()
+    return out
+
+
 def copy_calibration():
     """A plain copy kernel of known size (256 MiB read + 256 MiB write), so that rocprofv3's
     FETCH_SIZE / WRITE_SIZE for it calibrate the same counters on the NTT kernel."""
@@ -190,7 +454,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg (and its exactness gate)")
     ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--workload", choices=["all", "ntt24", "msm"], default="all",
+    ap.add_argument("--serve-jobs", type=int, default=32, help="proof jobs per serving run")
+    ap.add_argument("--serve-inflight", type=int, default=4, help="server: jobs on the device at once")
+    ap.add_argument("--serve-budget-gib", type=int, default=96, help="server: HBM budget (keys + workspaces)")
+    ap.add_argument("--workload", choices=["all", "ntt24", "msm", "batch", "serve"], default="all",
                     help="ntt24/msm restrict the run to one kernel family (for rocprofv3 counter passes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device-to-device copy kernel (reference for HBM counters)")
@@ -210,7 +477,15 @@ def main():
         return
     ev = HipEvents(gpu.stream())
     rng = np.random.default_rng(args.seed)
-    out = {}
+    if args.workload in ("batch", "serve"):
+        res = {}
+        if args.workload == "batch":
+            res["batched_ntt_64"] = batched_ntt_leg(gpu, ev, rng, args)
+            res["groth16_msm_set_2^22"] = groth16_msm_leg(gpu, ev, rng, args)
+        else:
+            res["serving"] = serving_leg(gpu, rng, args)
+        print(json.dumps(res), flush=True)
+        return
 
     # ---------------- G1 MSM 2^20 (the headline)
     n1 = 1 << 20
@@ -283,6 +558,12 @@ def main():
     fq_mults = n1 * W1 * 10  # upper bound: every digit nonzero
     msm_gmulmod = fq_mults / (g1_ms * 1e-3) / 1e9
 
+    batch_res = groth = serve = None
+    if args.workload == "all":
+        batch_res = batched_ntt_leg(gpu, ev, rng, args)
+        groth = groth16_msm_leg(gpu, ev, rng, args)
+        serve = serving_leg(gpu, rng, args)
+
     cpu = None
     if args.workload == "msm":
         print(json.dumps({"g1_msm_2^20_ms": round(g1_ms, 4), "g2_msm_2^18_ms": round(g2_ms, 4),
@@ -316,6 +597,9 @@ def main():
             "g2_msm_2^18_ms": round(g2_ms, 4),
             "ntt": {f"2^{k}": v for k, v in ntt.items()},
             "ntt_2^24_hbm_GBps": n24["hbm_GBps"],
+            "batched_ntt_64": batch_res,
+            "groth16_msm_set_2^22": groth,
+            "serving": serve,
             "field_mul_Gmulmod_per_s": round(mulmod_gps, 1),
             "g1_msm_fq_mults_per_s_G": round(msm_gmulmod, 1),
             "device": info,

@@ -154,6 +154,35 @@ int zkg_batch_add_msm(zkg_batch* batch, zkg_msm* plan, const zkg_buffer* bases, 
 int zkg_batch_run(zkg_batch* batch);
 int zkg_batch_clear(zkg_batch* batch);
 
+/* ------------------------------------------------------------------ proof-job server
+ * No counterpart in the reference. A device-resident request loop with continuous batching:
+ * proving keys stay in HBM; jobs are admitted in arrival order as workspace slots and the HBM
+ * budget allow (footprint per job: zkg_server_slot_bytes), and admitted jobs overlap on the
+ * device, each on its own streams. A job = 7 in-place NTTs of the key's size (bit k of
+ * inverse_mask: polynomial k inverse) + MSMs A, B1, C (G1, witness scalars), B2 (G2, witness
+ * scalars) and H (G1, n - 1 points, scalars = the first n - 1 outputs of polynomial 6).
+ * Results: 192 words = XYZZ of A, B1, C, H (32 each), then B2 (64), Montgomery form. */
+typedef struct zkg_server zkg_server;
+typedef struct {
+    uint64_t submitted, completed;
+    double p50_ms, p99_ms, mean_ms, jobs_per_s; /* latency = submit -> results in host memory */
+    size_t bytes_in_use, peak_bytes;
+    uint32_t inflight, peak_inflight;
+} zkg_server_stats;
+int zkg_server_create(zkg_device* dev, size_t mem_budget_bytes, uint32_t max_inflight, zkg_server** out);
+int zkg_server_destroy(zkg_server* server);
+/* g1[0..3]: A, B1, C (n points each) and H (n - 1 points); g2: B2 (n points); all in the
+ * library's form (zkg_msm_upload_bases*). The buffers must outlive the server. */
+int zkg_server_add_key(zkg_server* server, uint32_t log_n, zkg_buffer* const g1[4], zkg_buffer* g2, int* key_id);
+int zkg_server_slot_bytes(zkg_server* server, uint32_t log_n, size_t* bytes);
+int zkg_server_submit(zkg_server* server, int key_id, zkg_buffer* const polys[7], uint32_t inverse_mask,
+                      const zkg_buffer* witness, uint64_t* job_id);
+int zkg_server_done(zkg_server* server, uint64_t job_id, int* done);
+/* Blocks until the job finishes; results: 192 words (may be NULL); frees the job's record. */
+int zkg_server_wait(zkg_server* server, uint64_t job_id, uint32_t* results, double* latency_ms);
+int zkg_server_get_stats(zkg_server* server, zkg_server_stats* out);
+int zkg_server_reset_stats(zkg_server* server);
+
 /* ------------------------------------------------------------------ test and benchmark hooks
  * No counterpart in the reference's public API. They play the role of the reference's WGSL test
  * harness (tests/curve.rs:8-30) and field benchmark (examples/field_bench.rs:10-97). */

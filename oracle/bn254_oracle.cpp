@@ -689,6 +689,35 @@ void orc_poly_eval(const uint32_t* coeffs, size_t n, const uint32_t* x, uint32_t
     f_to_words(out, acc, F);
 }
 
+// sum a_i b_i mod r (plain words in, plain words out; every a_i, b_i < r). With bases k_i G this is
+// the scalar of the closed form sum s_i (k_i G) = (sum k_i s_i) G.
+void orc_dot_fr(const uint32_t* a, const uint32_t* b, size_t n, uint32_t* out, int threads) {
+    const Field& F = FR();
+    const int t = std::max(1, threads);
+    std::vector<Fe> part(t, Fe{{0, 0, 0, 0}});
+    std::vector<std::thread> pool;
+    for (int k = 0; k < t; ++k) {
+        const size_t lo = n * k / t, hi = n * (k + 1) / t;
+        pool.emplace_back([&, k, lo, hi] {
+            Fe acc = Fe{{0, 0, 0, 0}};
+            for (size_t i = lo; i < hi; ++i) {
+                Fe y, p;
+                for (int j = 0; j < 4; ++j) y.l[j] = (uint64_t)b[8 * i + 2 * j] | ((uint64_t)b[8 * i + 2 * j + 1] << 32);
+                f_mul(p, f_from_words(a + 8 * i, F), y, F);  // (a R) * b / R = a b, plain
+                f_add(acc, acc, p, F);
+            }
+            part[k] = acc;
+        });
+    }
+    for (auto& th : pool) th.join();
+    Fe acc = Fe{{0, 0, 0, 0}};
+    for (const Fe& x : part) f_add(acc, acc, x, F);
+    for (int j = 0; j < 4; ++j) {
+        out[2 * j] = (uint32_t)acc.l[j];
+        out[2 * j + 1] = (uint32_t)(acc.l[j] >> 32);
+    }
+}
+
 // Points: G1 affine 16 words (x, y), G2 affine 32 words (x.a, x.b, y.a, y.b); infinity = zeros.
 // group: 1 = G1, 2 = G2. Scalars: 8 words each, any 256-bit value.
 void orc_msm(int group, const uint32_t* bases, const uint32_t* scalars, size_t n, uint32_t* out, int threads) {

@@ -75,10 +75,11 @@ def lib():
         L.orc_multiples.argtypes = [i, u32p, u32p, sz, u32p, i]
         L.orc_point_add.argtypes = [i, u32p, u32p, u32p]
         L.orc_point_mul.argtypes = [i, u32p, u32p, u32p]
+        L.orc_dot_fr.argtypes = [u32p, u32p, sz, u32p, i]
         for f in ("orc_ntt", "orc_poly_eval", "orc_msm", "orc_msm_naive", "orc_multiples",
                   "orc_point_add", "orc_point_mul", "orc_field_mul", "orc_field_add",
                   "orc_field_sub", "orc_field_inv", "orc_field_pow", "orc_fq2_mul",
-                  "orc_fq2_inv", "orc_root_of_unity"):
+                  "orc_fq2_inv", "orc_root_of_unity", "orc_dot_fr"):
             getattr(L, f).restype = None
         _lib = L
     return _lib
@@ -268,8 +269,17 @@ def closed_form_msm(group: int, ks, scalars) -> np.ndarray:
     return point_mul(group, generator_words(group), t % R)
 
 
-def closed_form_scalar(ks_words, scalar_words) -> int:
-    """sum k_i s_i mod r from word arrays, vectorised over 16-bit pieces."""
+def closed_form_scalar(ks_words, scalar_words, threads: int | None = None) -> int:
+    """sum k_i s_i mod r (every k_i, s_i < r), multi-threaded in the C restatement."""
+    k, s = _u32(ks_words), _u32(scalar_words)
+    out = np.zeros(8, dtype=np.uint32)
+    lib().orc_dot_fr(k, s, k.size // 8, out, threads or default_threads())
+    return words_to_int(out)
+
+
+def closed_form_scalar_np(ks_words, scalar_words) -> int:
+    """sum k_i s_i mod r from word arrays in exact integer arithmetic, vectorised over 16-bit
+    pieces (independent of the field code: pins closed_form_scalar)."""
     k = _u32(ks_words).reshape(-1, 8)
     s = _u32(scalar_words).reshape(-1, 8)
     # Split into 16 sixteen-bit limbs; limb products are < 2^32 and sums over <= 2^31 rows fit u64.

@@ -133,9 +133,10 @@ def test_msm_defaults_at_2_17(gpu):
     assert np.array_equal(got, O.point_mul(1, O.generator_words(1), O.closed_form_scalar(ks, sc)))
 
 
-@pytest.mark.parametrize("group,log_n", [(1, 16), (1, 20), (2, 18)])
+@pytest.mark.parametrize("group,log_n", [(1, 16), (1, 20), (2, 18), (1, 21), (1, 22), (2, 22), (1, 24)])
 def test_msm_config_sizes_closed_form(gpu, group, log_n):
-    """G1 2^16 and 2^20, G2 2^18 at the default window, exact at full size."""
+    """G1 2^16, 2^20, G2 2^18 (the single-call sizes) and the Groth16-set / serving sizes up to
+    2^24 (window 16: the largest LDS sort) at the default window, exact at full size."""
     from zkgpu import Group, Msm
 
     if quick() and log_n > 16:

@@ -140,6 +140,24 @@ def test_multiples_and_closed_form(group):
     assert np.array_equal(O.msm(group, bases[: 16 * group * 40], sc[: 8 * 40]), O.msm_naive(group, bases[: 16 * group * 40], sc[: 8 * 40]))
     t = O.closed_form_scalar(ks, sc)
     assert t == sum(a * b for a, b in zip(O.words_to_ints(ks), O.words_to_ints(sc))) % O.R
+    assert O.closed_form_scalar_np(ks, sc) == t
+
+
+def test_closed_form_scalar_threads_and_edges():
+    """The threaded C dot product against exact integer arithmetic: r - 1 squared terms, zeros,
+    a length that does not split evenly over the threads, and one thread vs many."""
+    rng = np.random.default_rng(24)
+    n = 10007
+    ks = uniform_fr_words(rng, n)
+    sc = uniform_fr_words(rng, n)
+    ks[:8] = O.int_to_words(O.R - 1)
+    sc[:8] = O.int_to_words(O.R - 1)
+    sc[8:16] = 0
+    want = O.closed_form_scalar_np(ks, sc)
+    assert O.closed_form_scalar(ks, sc, threads=1) == want
+    assert O.closed_form_scalar(ks, sc, threads=7) == want
+    assert O.closed_form_scalar(ks[:8], sc[:8]) == (O.R - 1) ** 2 % O.R
+    assert O.closed_form_scalar(ks[:0], sc[:0]) == 0
 
 
 def test_msm_edge_cases_vs_naive():

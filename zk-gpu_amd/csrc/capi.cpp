@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "api_types.hpp"
 #include "host_math.hpp"
@@ -443,6 +444,133 @@ int zkg_batch_run(zkg_batch* batch) {
 int zkg_batch_clear(zkg_batch* batch) {
     ZKG_REQUIRE(batch, "null argument");
     batch->b->clear();
+    return zkg::ZKG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ proof-job server
+
+#include "serve.hpp"
+
+struct zkg_server {
+    zkg::Server* s;
+    zkg::Device* d;
+    std::vector<uint32_t> key_log_n;
+};
+
+extern "C" {
+
+int zkg_server_create(zkg_device* dev, size_t mem_budget_bytes, uint32_t max_inflight, zkg_server** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    *out = new zkg_server{new zkg::Server(dev->d, mem_budget_bytes, max_inflight), dev->d, {}};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_server_destroy(zkg_server* server) {
+    if (!server) return zkg::ZKG_OK;
+    delete server->s;
+    delete server;
+    return zkg::ZKG_OK;
+}
+
+int zkg_server_add_key(zkg_server* server, uint32_t log_n, zkg_buffer* const g1[4], zkg_buffer* g2, int* key_id) {
+    ZKG_REQUIRE(server && g1 && g2 && key_id, "null argument");
+    if (log_n < 10 || log_n > 27) return zkg::fail(zkg::ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    const size_t n = (size_t)1 << log_n;
+    for (int i = 0; i < 4; ++i) {
+        ZKG_REQUIRE(g1[i], "null argument");
+        const size_t pts = i == 3 ? n - 1 : n;
+        if (g1[i]->b->bytes < 64 * pts)
+            return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(g1[i]->b->bytes) + " bytes; " +
+                                                    std::to_string(pts) + " points need " + std::to_string(64 * pts));
+    }
+    if (g2->b->bytes < 128 * n)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(g2->b->bytes) + " bytes; " +
+                                                std::to_string(n) + " points need " + std::to_string(128 * n));
+    ZKG_GUARD_BEGIN
+    const uint32_t* p[4];
+    size_t bytes = g2->b->bytes;
+    for (int i = 0; i < 4; ++i) {
+        p[i] = static_cast<const uint32_t*>(g1[i]->b->ptr);
+        bytes += g1[i]->b->bytes;
+    }
+    *key_id = server->s->add_key(log_n, p, static_cast<const uint32_t*>(g2->b->ptr), bytes);
+    if (server->key_log_n.size() <= (size_t)*key_id) server->key_log_n.resize(*key_id + 1);
+    server->key_log_n[*key_id] = log_n;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_server_slot_bytes(zkg_server* server, uint32_t log_n, size_t* bytes) {
+    ZKG_REQUIRE(server && bytes, "null argument");
+    if (log_n < 10 || log_n > 27) return zkg::fail(zkg::ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    *bytes = server->s->slot_bytes(log_n);
+    return zkg::ZKG_OK;
+}
+
+int zkg_server_submit(zkg_server* server, int key_id, zkg_buffer* const polys[7], uint32_t inverse_mask,
+                      const zkg_buffer* witness, uint64_t* job_id) {
+    ZKG_REQUIRE(server && polys && witness && job_id, "null argument");
+    if (key_id < 0 || (size_t)key_id >= server->key_log_n.size()) return zkg::fail(zkg::ZKG_ERR_INVALID, "unknown proving key");
+    const uint32_t log_n = server->key_log_n[key_id];
+    const size_t need = (size_t)32 << log_n;
+    for (int p = 0; p < 7; ++p) {
+        ZKG_REQUIRE(polys[p], "null argument");
+        if (polys[p]->b->bytes < need)
+            return zkg::fail(zkg::ZKG_ERR_SIZE, "data buffer holds " + std::to_string(polys[p]->b->bytes) +
+                                                    " bytes; a 2^" + std::to_string(log_n) + " transform needs " +
+                                                    std::to_string(need));
+    }
+    if (witness->b->bytes < need)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "scalars buffer holds " + std::to_string(witness->b->bytes) + " bytes; " +
+                                                std::to_string((size_t)1 << log_n) + " scalars need " + std::to_string(need));
+    ZKG_GUARD_BEGIN
+    uint32_t* p[7];
+    for (int k = 0; k < 7; ++k) p[k] = static_cast<uint32_t*>(polys[k]->b->ptr);
+    *job_id = server->s->submit(key_id, p, inverse_mask, static_cast<const uint32_t*>(witness->b->ptr));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_server_done(zkg_server* server, uint64_t job_id, int* done) {
+    ZKG_REQUIRE(server && done, "null argument");
+    ZKG_GUARD_BEGIN
+    *done = server->s->done(job_id) ? 1 : 0;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_server_wait(zkg_server* server, uint64_t job_id, uint32_t* results, double* latency_ms) {
+    ZKG_REQUIRE(server, "null argument");
+    ZKG_GUARD_BEGIN
+    server->s->wait(job_id, results, latency_ms);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_server_get_stats(zkg_server* server, zkg_server_stats* out) {
+    ZKG_REQUIRE(server && out, "null argument");
+    const zkg::ServerStats st = server->s->stats();
+    out->submitted = st.submitted;
+    out->completed = st.completed;
+    out->p50_ms = st.p50_ms;
+    out->p99_ms = st.p99_ms;
+    out->mean_ms = st.mean_ms;
+    out->jobs_per_s = st.jobs_per_s;
+    out->bytes_in_use = st.bytes_in_use;
+    out->peak_bytes = st.peak_bytes;
+    out->inflight = st.inflight;
+    out->peak_inflight = st.peak_inflight;
+    return zkg::ZKG_OK;
+}
+
+int zkg_server_reset_stats(zkg_server* server) {
+    ZKG_REQUIRE(server, "null argument");
+    server->s->reset_stats();
     return zkg::ZKG_OK;
 }
 

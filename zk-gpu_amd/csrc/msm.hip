@@ -358,11 +358,14 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     nblk = (n + sort_pts - 1) / sort_pts;
     device_bind(d);
     if (lds_sort) {
-        // up to 128 KB of LDS per sort block (c = 16); allow it for both LDS-sort kernels
+        // 4 * B bytes of LDS per sort block, up to 128 KB at c = 16. The limit is a property of
+        // the kernel, shared by every plan: always raise it to the largest window's need, so a
+        // plan with a small window never lowers it under a plan with a large one.
+        const int lds_max = 4 << 15;
         ZKG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(msm_hist_kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 4 * B));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
         ZKG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(msm_place_kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 4 * B));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     }
     try {
         ZKG_HIP(hipMalloc(&counts, 4 * S));
@@ -396,10 +399,18 @@ MsmPlan::~MsmPlan() {
         partials = wsums = result = nullptr;
 }
 
-size_t MsmPlan::workspace_bytes() const {
-    const size_t S = (size_t)W * B;
-    return 4 * (4 * S + 2 + (size_t)W * n * (lds_sort ? 2 : 1) + (lds_sort ? (size_t)W * nblk * B : 0) +
-                max_items * (1 + xw) + S * xw + (size_t)W * T * xw + (size_t)W * xw + xw);
+size_t MsmPlan::workspace_bytes() const { return msm_workspace_bytes(group, n, c, CH); }
+
+size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points) {
+    const uint32_t W = (255 + c - 1) / c, B = 1u << (c - 1);
+    const uint32_t CH = item_points ? item_points : msm_auto_item_points(n, c);
+    const uint32_t xw = group == 1 ? 32 : 64, T = B / 4;
+    const size_t S = (size_t)W * B, max_items = (size_t)W * ((n + CH - 1) / CH + B);
+    const bool lds = c <= 16;
+    const size_t P = std::max<uint32_t>(B, 8192), nblk = (n + P - 1) / P;
+    const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    return 4 * (4 * S + 2 + 2 * nb + (size_t)W * n * (lds ? 2 : 1) + (lds ? (size_t)W * nblk * B : 0) +
+                max_items * (1 + (size_t)xw) + S * xw + (size_t)W * T * xw + (size_t)W * xw + xw);
 }
 
 template <class Fe>

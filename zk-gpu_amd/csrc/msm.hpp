@@ -14,6 +14,8 @@ namespace zkg {
 
 uint32_t msm_default_window(size_t n);          // clamp(ilog2(max(n,1)) - 5, 11, 16)
 uint32_t msm_auto_item_points(size_t n, uint32_t c);
+// Device bytes a plan for (group, n, c, item_points) allocates (item_points 0 = automatic).
+size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points);
 
 struct MsmPlan {
     Device* dev = nullptr;

@@ -311,12 +311,14 @@ NttBatch::~NttBatch() {
     table_.release();
 }
 
+int NttBatch::tile_log() { return BATCH_TILE_LOG; }
+
 const NttPlan& NttBatch::plan(uint32_t log_n) {
     auto it = plans_.find(log_n);
     if (it == plans_.end()) {
         NttShape shape;
         shape.tile_log = BATCH_TILE_LOG;
-        it = plans_.emplace(log_n, std::unique_ptr<NttPlan>(new NttPlan(dev_, log_n, shape, false))).first;
+        it = plans_.emplace(log_n, std::make_shared<NttPlan>(dev_, log_n, shape, false)).first;
     }
     return *it->second;
 }

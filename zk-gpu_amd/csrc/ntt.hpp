@@ -69,6 +69,9 @@ class NttBatch {
     size_t size() const { return jobs_.size(); }
     void run(hipStream_t stream);
     const NttPlan& plan(uint32_t log_n);
+    // Use an existing twiddle plan (built with the batch tile size) for its size.
+    void share_plan(const std::shared_ptr<NttPlan>& p) { plans_[p->log_n] = p; }
+    static int tile_log();
 
   private:
     struct Job {
@@ -78,7 +81,7 @@ class NttBatch {
     };
     Device* dev_;
     std::vector<Job> jobs_;
-    std::map<uint32_t, std::unique_ptr<NttPlan>> plans_;
+    std::map<uint32_t, std::shared_ptr<NttPlan>> plans_;
     Scratch scratch_;                 // sum of the jobs' sizes, 32 bytes per element
     Scratch table_;                   // device copy of the pass tables of all phases
     NttPassArgs* staging_ = nullptr;  // pinned host staging of the tables

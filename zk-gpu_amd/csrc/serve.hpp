@@ -1,0 +1,114 @@
+// Device-resident proof-job server with continuous batching.
+//
+// No counterpart in the reference (one call at a time). Proving keys (four G1 base sets A, B1,
+// C, H and one G2 set B2) stay resident in HBM. A job names a key and brings seven polynomials
+// (transformed in place, each in its own direction) and a witness (scalars of the A, B1, C and B2
+// MSMs); the H MSM takes the first n-1 outputs of the seventh transform as its scalars, as in a
+// Groth16 prover. A scheduler thread admits jobs in arrival order as soon as a workspace slot and
+// the HBM budget allow, so jobs that arrive while others run overlap on the device: every job
+// runs on its own streams (NTTs grouped into one launch per pass phase, then H; A, B1, C and B2
+// concurrently). Results (five XYZZ points) land in pinned host memory.
+#pragma once
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "msm.hpp"
+#include "ntt.hpp"
+
+namespace zkg {
+
+static constexpr int SERVE_POLYS = 7;
+static constexpr int SERVE_MSMS = 5;                  // A, B1, C, H (G1) and B2 (G2)
+static constexpr int SERVE_RESULT_WORDS = 4 * 32 + 64;  // XYZZ words: four G1 points, then the G2 point
+
+struct ServerStats {
+    uint64_t submitted = 0, completed = 0;
+    double p50_ms = 0, p99_ms = 0, mean_ms = 0, jobs_per_s = 0;
+    size_t bytes_in_use = 0, peak_bytes = 0;
+    uint32_t inflight = 0, peak_inflight = 0;
+};
+
+class Server {
+  public:
+    // mem_budget: HBM bytes the server may hold (keys + workspaces); max_inflight: jobs on the
+    // device at once. Every MSM uses default_window(n) for its key size.
+    Server(Device* d, size_t mem_budget, uint32_t max_inflight);
+    ~Server();
+
+    // bases: device buffers in the library's Montgomery form; g1[0..3] = A, B1, C (n points),
+    // H (n - 1 points); g2 = B2 (n points). Returns the key id.
+    int add_key(uint32_t log_n, const uint32_t* const g1[4], const uint32_t* g2, size_t key_bytes);
+    // Queue a job (returns its id); the buffers must stay valid until the job completes.
+    uint64_t submit(int key, uint32_t* const polys[SERVE_POLYS], uint32_t inverse_mask, const uint32_t* witness);
+    // Block until the job is done; copies its SERVE_RESULT_WORDS result words and latency.
+    void wait(uint64_t job, uint32_t* results, double* latency_ms);
+    bool done(uint64_t job);
+    ServerStats stats();
+    void reset_stats();
+    // Per-job device footprint for a key of 2^log_n points (workspace slot).
+    size_t slot_bytes(uint32_t log_n);
+
+  private:
+    struct Key {
+        uint32_t log_n;
+        const uint32_t* g1[4];
+        const uint32_t* g2;
+    };
+    struct Slot {
+        uint32_t log_n = 0;
+        std::unique_ptr<MsmPlan> plans[SERVE_MSMS];
+        std::unique_ptr<NttBatch> ntt;
+        hipStream_t streams[SERVE_MSMS] = {};
+        hipEvent_t ev_ntt = nullptr, ev_done[SERVE_MSMS] = {};
+        uint32_t* host = nullptr;  // pinned results
+        size_t bytes = 0;
+        bool busy = false;
+        uint64_t last_use = 0;
+    };
+    struct Job {
+        uint64_t id;
+        int key;
+        uint32_t* polys[SERVE_POLYS];
+        uint32_t inverse_mask;
+        const uint32_t* witness;
+        hipEvent_t ready = nullptr;  // recorded on the device stream at submit
+        double t_submit = 0, t_done = 0;
+        Slot* slot = nullptr;
+        bool finished = false;
+        uint32_t results[SERVE_RESULT_WORDS];
+    };
+
+    void loop();
+    bool try_admit(Job& j);           // called with mu_ held
+    void launch(Job& j);              // called with mu_ held
+    Slot* acquire_slot(uint32_t log_n);
+    std::unique_ptr<Slot> make_slot(uint32_t log_n);
+    void destroy_slot(Slot* s);
+    uint32_t window_for(uint32_t log_n);
+
+    Device* dev_;
+    size_t budget_, key_bytes_ = 0, slot_bytes_total_ = 0, peak_bytes_ = 0;
+    uint32_t max_inflight_, inflight_ = 0, peak_inflight_ = 0;
+    std::vector<Key> keys_;
+    std::vector<std::unique_ptr<Slot>> slots_;
+    std::map<uint32_t, std::shared_ptr<NttPlan>> twiddles_;
+    std::map<uint64_t, std::unique_ptr<Job>> jobs_;
+    std::deque<uint64_t> pending_;
+    std::vector<uint64_t> running_;
+    std::vector<double> latencies_;
+    uint64_t next_id_ = 1, use_clock_ = 0, submitted_ = 0;
+    double first_submit_ = -1, last_done_ = 0;
+    std::string error_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    bool stop_ = false;
+    std::thread worker_;
+};
+
+}  // namespace zkg

@@ -12,5 +12,6 @@ from .group import Group
 from .msm import Msm, xyzz_to_affine, xyzz_to_affine_g2
 from .ntt import Ntt
 from .batch import Batch
+from .serve import Server
 
-__all__ = ["Batch", "Buffer", "Gpu", "Group", "Msm", "Ntt", "ZkgError", "host", "xyzz_to_affine", "xyzz_to_affine_g2"]
+__all__ = ["Batch", "Buffer", "Gpu", "Group", "Msm", "Ntt", "Server", "ZkgError", "host", "xyzz_to_affine", "xyzz_to_affine_g2"]

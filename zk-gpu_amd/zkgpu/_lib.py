@@ -61,6 +61,17 @@ SIGNATURES = {
     "zkg_batch_add_msm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_batch_run": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_batch_clear": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, c_void_pp]),
+    "zkg_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_server_add_key": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p * 4, ctypes.c_void_p,
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "zkg_server_slot_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]),
+    "zkg_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p * 7, ctypes.c_uint32,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "zkg_server_done": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "zkg_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_u32p, ctypes.POINTER(ctypes.c_double)]),
+    "zkg_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "zkg_server_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_test_field_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_u32p, c_u32p, c_u32p,
                                          ctypes.c_size_t]),
     "zkg_test_point_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_u32p, c_u32p, c_u32p,

@@ -1,0 +1,100 @@
+"""Proof-job server: resident proving keys, continuous batching (no counterpart in the reference).
+
+A job = seven in-place NTTs of the key's size + the five Groth16 MSMs (A, B1, C and B2 over the
+witness, H over the first n - 1 outputs of the seventh transform). Jobs are admitted in arrival
+order while the HBM budget and the in-flight limit allow, and admitted jobs overlap on the device.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+RESULT_WORDS = 4 * 32 + 64
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("jobs_per_s", ctypes.c_double),
+        ("bytes_in_use", ctypes.c_size_t),
+        ("peak_bytes", ctypes.c_size_t),
+        ("inflight", ctypes.c_uint32),
+        ("peak_inflight", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Server:
+    def __init__(self, gpu, mem_budget_bytes: int, max_inflight: int = 4):
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_server_create(gpu.handle, int(mem_budget_bytes), int(max_inflight), ctypes.byref(h)))
+        self._h = h
+        self._gpu = gpu
+        self._keep = {}  # key id -> bases buffers; job id -> its buffers
+
+    def add_key(self, log_n: int, g1, g2) -> int:
+        """g1 = (A, B1, C, H) bases buffers (H holds n - 1 points), g2 = B2 bases buffer."""
+        arr = (ctypes.c_void_p * 4)(*[b.handle for b in g1])
+        kid = ctypes.c_int()
+        check(_lib.load().zkg_server_add_key(self._h, int(log_n), arr, g2.handle, ctypes.byref(kid)))
+        self._keep[("key", kid.value)] = (tuple(g1), g2)
+        return kid.value
+
+    def slot_bytes(self, log_n: int) -> int:
+        out = ctypes.c_size_t()
+        check(_lib.load().zkg_server_slot_bytes(self._h, int(log_n), ctypes.byref(out)))
+        return out.value
+
+    def submit(self, key: int, polys, inverse_mask: int, witness) -> int:
+        assert len(polys) == 7
+        arr = (ctypes.c_void_p * 7)(*[p.handle for p in polys])
+        jid = ctypes.c_uint64()
+        check(_lib.load().zkg_server_submit(self._h, int(key), arr, int(inverse_mask), witness.handle,
+                                            ctypes.byref(jid)))
+        self._keep[("job", jid.value)] = (tuple(polys), witness)
+        return jid.value
+
+    def done(self, job: int) -> bool:
+        d = ctypes.c_int()
+        check(_lib.load().zkg_server_done(self._h, int(job), ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, job: int):
+        """Block until `job` finishes: (results as 192 u32 words, latency in ms)."""
+        out = np.zeros(RESULT_WORDS, dtype=np.uint32)
+        lat = ctypes.c_double()
+        try:
+            check(_lib.load().zkg_server_wait(self._h, int(job), _lib.u32_ptr(out), ctypes.byref(lat)))
+        finally:
+            self._keep.pop(("job", int(job)), None)
+        return out, lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(_lib.load().zkg_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        check(_lib.load().zkg_server_reset_stats(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.load().zkg_server_destroy(self._h)
+            self._h = ctypes.c_void_p(0)
+            self._keep = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
