@@ -198,6 +198,10 @@ int zkg_test_point_op(zkg_device* dev, int group, int op, const uint32_t* p, con
 /* Field-multiply throughput: `threads` lanes each run `chain` dependent Montgomery products,
  * `ways` independent chains per lane. Reports kernel milliseconds (HIP events). */
 int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t chain, uint32_t ways, double* ms);
+/* The MSM's GLV scalar split on the device: out[0..4n) = k1 of each scalar, out[4n..8n) = k2,
+ * each as |k| (127 bits, 4 words) with the sign in bit 127, such that s = k1 + k2 * lambda mod r
+ * (lambda = 0xb3c4d79d41a917585bfc41088d8daaa78b17ea66b99c90dd). Synchronous; host arrays. */
+int zkg_test_glv_split(zkg_device* dev, const uint32_t* scalars, size_t n, uint32_t* out);
 /* Fixed-base multiples out[i] = ks[i] * generator (plain affine, infinity = zeros), for building
  * large synthetic inputs on the device. ks: n scalars (8 words each). */
 int zkg_util_generator_multiples(zkg_device* dev, int group, const zkg_buffer* ks, size_t n, zkg_buffer* out);

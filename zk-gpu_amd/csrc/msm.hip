@@ -16,9 +16,13 @@
 //  finalize   each bucket sums its at most G remaining items.
 //  reduce     S_w = sum_b (b + 1) bucket[w][b]: L buckets per lane with running sums, plus b0 * run.
 //  tree sum   per-window sum of the lane results in one workgroup (LDS tree).
-//  combine    lane w scales S_w by c*w doublings, then a wave-wide LDS tree adds the W results.
+//  combine    Horner's rule over the windows from the top, one wave with lane-parallel products.
+// With GLV (the default), "count" first splits every scalar into two signed 127-bit halves and
+// computes phi(P) = (beta x, y) for every base; the stages then run over 2n virtual points with
+// ceil(128 / c) windows instead of ceil(255 / c).
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 
 #include "curve.hpp"
 #include "msm.hpp"
@@ -73,6 +77,150 @@ void msm_convert_bases(uint32_t* xy, size_t coords, hipStream_t stream) {
     ZKG_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- GLV split
+// lambda = 0xb3c4d79d41a917585bfc41088d8daaa78b17ea66b99c90dd is a cube root of unity mod r and
+// phi(x, y) = (beta x, y) maps P to lambda P, with beta a cube root of unity mod p:
+// beta = 0x59e26bcea0d48bacd4f263f1acdb5c4f5763473177fffffe on G1 and p - 1 minus that on G2.
+// The lattice {(a, b) : a + b lambda = 0 mod r} has the reduced basis v1 = (a1, b1),
+// v2 = (a2, b2) (extended Euclid on (r, lambda), stopped below sqrt(r); a1 b2 - a2 b1 = r):
+//   a1 = b2 = 0x89d3256894d213e3, b1 = -0x6f4d8248eeb859fc8211bbeb7d4f1128,
+//   a2 = 0x6f4d8248eeb859fd0be4e1541221250b.
+// Babai rounding with c1 = floor(s G1c / 2^256) ~ s b2 / r and c2 = floor(s G2c / 2^256) ~
+// -s b1 / r (G1c, G2c = those ratios times 2^256, rounded) gives (k1, k2) = (s, 0) - c1 v1 - c2 v2
+// with k1 + k2 lambda = s (mod r); each c_i is within 1.125 of the exact ratio, so
+// |k1|, |k2| <= 1.125 (a2 + a1) < 0.98 * 2^127 for every s < r.
+
+struct GlvConst {
+    static ZKG_HD constexpr uint32_t g1c(int i) { return i == 0 ? 0xc7e0b3d7u : i == 1 ? 0xd91d232eu : 0x2u; }
+    static ZKG_HD constexpr uint32_t g2c(int i) {
+        return i == 0 ? 0x391eb18eu : i == 1 ? 0x7a7bd9d4u : i == 2 ? 0xa773d2cfu : i == 3 ? 0x4ccef014u : 0x2u;
+    }
+    static ZKG_HD constexpr uint32_t a1(int i) { return i == 0 ? 0x94d213e3u : 0x89d32568u; }  // = b2
+    static ZKG_HD constexpr uint32_t a2(int i) {
+        return i == 0 ? 0x1221250bu : i == 1 ? 0x0be4e154u : i == 2 ? 0xeeb859fdu : 0x6f4d8248u;
+    }
+    static ZKG_HD constexpr uint32_t nb1(int i) {  // -b1
+        return i == 0 ? 0x7d4f1128u : i == 1 ? 0x8211bbebu : i == 2 ? 0xeeb859fcu : 0x6f4d8248u;
+    }
+    // beta * 2^256 mod p (the bases are in Montgomery form)
+    static ZKG_HD constexpr uint32_t beta_g1(int i) {
+        return i == 0 ? 0xd782e155u : i == 1 ? 0x71930c11u : i == 2 ? 0xffbe3323u : i == 3 ? 0xa6bb947cu
+             : i == 4 ? 0xd4741444u : i == 5 ? 0xaa303344u : i == 6 ? 0x26594943u : 0x2c3b3f0du;
+    }
+    static ZKG_HD constexpr uint32_t beta_g2(int i) {
+        return i == 0 ? 0x13e80b9cu : i == 1 ? 0x3350c88eu : i == 2 ? 0xdb5e56b9u : i == 3 ? 0x7dce557cu
+             : i == 4 ? 0xb615564au : i == 5 ? 0x6001b4b8u : i == 6 ? 0x020217e0u : 0x2682e617u;
+    }
+};
+
+// out[0..NB) = floor(s * g / 2^256), s of 8 words, g of NB words
+template <int NB, class G>
+ZKG_DEV void glv_mul_high(const uint32_t s[8], G g, uint32_t out[NB]) {
+    uint32_t prod[8 + NB];
+#pragma unroll
+    for (int k = 0; k < 8 + NB; ++k) prod[k] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const uint64_t t = (uint64_t)s[i] * g(j) + prod[i + j] + carry;
+            prod[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[i + NB] = (uint32_t)carry;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) out[k] = prod[8 + k];
+}
+
+// acc -= a * b (mod 2^160); a of NA words, b of NB words (a constant)
+template <int NA, int NB, class G>
+ZKG_DEV void glv_sub_mul(uint32_t acc[5], const uint32_t* a, G b) {
+    uint32_t prod[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (i + j < 5) {
+                const uint64_t t = (uint64_t)a[i] * b(j) + prod[i + j] + carry;
+                prod[i + j] = (uint32_t)t;
+                carry = t >> 32;
+            }
+        }
+        if (i + NB < 5) prod[i + NB] = (uint32_t)carry;
+    }
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[k] = __builtin_subc(acc[k], prod[k], borrow, &borrow);
+}
+
+// 160-bit two's complement -> |k| (127 bits) with the sign in bit 127
+ZKG_DEV void glv_store(uint32_t* dst, uint32_t k[5]) {
+    const uint32_t neg = k[4] >> 31;
+    if (neg) {
+        uint32_t carry = 1;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) k[i] = __builtin_addc(~k[i], 0u, carry, &carry);
+    }
+    *reinterpret_cast<uint4*>(dst) = make_uint4(k[0], k[1], k[2], (k[3] & 0x7fffffffu) | (neg << 31));
+}
+
+__global__ __launch_bounds__(256) void msm_glv_split_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
+                                                           uint32_t* __restrict__ split) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s[8];
+    const uint4* p = reinterpret_cast<const uint4*>(scalars + 8 * (size_t)i);
+    const uint4 a = p[0], b = p[1];
+    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    // any 256-bit scalar: reduce mod r first (2^256 < 6r)
+    for (int rep = 0; rep < 5; ++rep) {
+        uint32_t d[8], borrow = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d[k] = __builtin_subc(s[k], FrParams::mod(k), borrow, &borrow);
+        if (borrow) break;  // s < r
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] = d[k];
+    }
+    uint32_t c1[3], c2[5];
+    glv_mul_high<3>(s, GlvConst::g1c, c1);
+    glv_mul_high<5>(s, GlvConst::g2c, c2);
+    // k1 = s - c1 a1 - c2 a2;  k2 = c1 (-b1) - c2 b2   (mod 2^160; the true values are < 2^127)
+    uint32_t k1[5] = {s[0], s[1], s[2], s[3], s[4]};
+    glv_sub_mul<3, 2>(k1, c1, GlvConst::a1);
+    glv_sub_mul<5, 4>(k1, c2, GlvConst::a2);
+    uint32_t k2[5] = {0, 0, 0, 0, 0};
+    glv_sub_mul<5, 2>(k2, c2, GlvConst::a1);  // b2 = a1
+    uint32_t t[5] = {0, 0, 0, 0, 0};
+    glv_sub_mul<3, 4>(t, c1, GlvConst::nb1);  // t = -c1 (-b1)
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) k2[k] = __builtin_subc(k2[k], t[k], borrow, &borrow);
+    glv_store(split + 4 * (size_t)i, k1);
+    glv_store(split + 4 * ((size_t)n + i), k2);
+}
+
+void msm_glv_split(const uint32_t* scalars, uint32_t n, uint32_t* split, hipStream_t stream) {
+    if (!n) return;
+    hipLaunchKernelGGL(msm_glv_split_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, scalars, n, split);
+    ZKG_HIP(hipGetLastError());
+}
+
+// endo_x[i] = beta * x_i for every base (G2: both Fq components of x times beta)
+__global__ __launch_bounds__(256) void msm_endo_kernel(const uint32_t* __restrict__ bases, uint32_t n, int group,
+                                                      uint32_t* __restrict__ endo_x) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * group) return;
+    const size_t i = t / group, k = t % group;
+    Fq beta;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) beta.w[j] = group == 1 ? GlvConst::beta_g1(j) : GlvConst::beta_g2(j);
+    const Fq x = fp_load<FqParams>(bases + 16 * group * i + 8 * k);
+    fp_store(endo_x + 8 * t, x * beta);
+}
+
 // ---------------------------------------------------------------- digits: count and scatter
 
 __device__ __forceinline__ uint32_t window_bits(const uint32_t s[8], uint32_t pos, uint32_t c) {
@@ -82,9 +230,42 @@ __device__ __forceinline__ uint32_t window_bits(const uint32_t s[8], uint32_t po
     return v & ((1u << c) - 1);
 }
 
-template <bool SCATTER>
+// Scalar i of the digit stages: 8 plain words, or (GLV) a 127-bit magnitude with its sign in
+// bit 127; returns the sign.
+template <bool GLV>
+ZKG_DEV uint32_t load_digit_scalar(const uint32_t* __restrict__ scalars, uint32_t i, uint32_t s[8]) {
+    if (GLV) {
+        const uint4 a = *reinterpret_cast<const uint4*>(scalars + 4 * (size_t)i);
+        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w & 0x7fffffffu; s[4] = s[5] = s[6] = s[7] = 0;
+        return a.w >> 31;
+    }
+    const uint4* p = reinterpret_cast<const uint4*>(scalars + 8 * (size_t)i);
+    const uint4 a = p[0], b = p[1];
+    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    return 0;
+}
+
+// Signed digit of window value v (bits + carry). Full scalars follow the reference's rule
+// (v >= 2^(c-1) -> v - 2^c, carry 1), digits in [-2^(c-1), 2^(c-1)). GLV half-scalars let the
+// digit reach +2^(c-1) (v > 2^(c-1) wraps), so ceil(128 / c) windows hold any |k| < 2^127 with no
+// carry out of the top window. Either way |digit| <= 2^(c-1): bucket |digit| - 1.
+template <bool GLV>
+ZKG_DEV void signed_digit(uint32_t v, uint32_t width, uint32_t& mag, uint32_t& neg, uint32_t& carry) {
+    const uint32_t half = 1u << (width - 1);
+    const bool wrap = GLV ? v > half : v >= half;
+    mag = wrap ? (1u << width) - v : v;
+    neg = wrap ? 1u : 0u;
+    carry = neg;
+}
+
+// Bit offset and width of window w (see MsmPlan::wb, nw).
+ZKG_DEV uint32_t window_pos(uint32_t w, uint32_t wb, uint32_t nw) { return w * wb + min(w, nw); }
+ZKG_DEV uint32_t window_width(uint32_t w, uint32_t wb, uint32_t nw) { return wb + (w < nw ? 1u : 0u); }
+
+template <bool SCATTER, bool GLV>
 __global__ __launch_bounds__(256) void msm_digits_kernel(const uint32_t* __restrict__ scalars, uint32_t n, uint32_t c,
-                                                         uint32_t W, uint32_t* __restrict__ counter,
+                                                         uint32_t W, uint32_t wb, uint32_t nw,
+                                                         uint32_t* __restrict__ counter,
                                                          const uint32_t* __restrict__ starts,
                                                          uint32_t* __restrict__ sorted) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -92,25 +273,20 @@ __global__ __launch_bounds__(256) void msm_digits_kernel(const uint32_t* __restr
     const uint32_t B = 1u << (c - 1);
     const uint32_t lane = __lane_id();
     const unsigned long long lt_mask = (1ull << lane) - 1;
-    uint32_t s[8];
+    uint32_t s[8], sign = 0;
     if (valid) {
-        const uint4* p = reinterpret_cast<const uint4*>(scalars + 8 * (size_t)i);
-        const uint4 a = p[0], b = p[1];
-        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+        sign = load_digit_scalar<GLV>(scalars, i, s);
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) s[k] = 0;
     }
     uint32_t carry = 0;
     for (uint32_t w = 0; w < W; ++w) {
-        const uint32_t v = window_bits(s, w * c, c) + carry;
-        uint32_t mag = v, neg = 0;
-        carry = 0;
-        if (v >= B) {  // digit v - 2^c < 0
-            mag = (1u << c) - v;
-            neg = 1;
-            carry = 1;
-        }
+        const uint32_t width = window_width(w, wb, nw);
+        const uint32_t v = window_bits(s, window_pos(w, wb, nw), width) + carry;
+        uint32_t mag, neg;
+        signed_digit<GLV>(v, width, mag, neg, carry);
+        neg ^= sign;
         const bool has = valid && mag != 0;
         const uint32_t slot = w * B + mag - 1;
         const unsigned long long active = __ballot(has);
@@ -152,26 +328,21 @@ __global__ __launch_bounds__(256) void msm_digits_kernel(const uint32_t* __restr
 static constexpr uint32_t NO_DIGIT = 0xffffffffu;
 static constexpr int SORT_THREADS = 1024;
 
+template <bool GLV>
 __global__ __launch_bounds__(256) void msm_digits_store_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
-                                                              uint32_t c, uint32_t W, uint32_t* __restrict__ digits) {
+                                                              uint32_t W, uint32_t wb, uint32_t nw,
+                                                              uint32_t* __restrict__ digits) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t B = 1u << (c - 1);
     uint32_t s[8];
-    const uint4* p = reinterpret_cast<const uint4*>(scalars + 8 * (size_t)i);
-    const uint4 a = p[0], b = p[1];
-    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    const uint32_t sign = load_digit_scalar<GLV>(scalars, i, s);
     uint32_t carry = 0;
     for (uint32_t w = 0; w < W; ++w) {
-        const uint32_t v = window_bits(s, w * c, c) + carry;
-        uint32_t mag = v, neg = 0;
-        carry = 0;
-        if (v >= B) {
-            mag = (1u << c) - v;
-            neg = 1;
-            carry = 1;
-        }
-        digits[(size_t)w * n + i] = mag ? ((mag - 1) | (neg << 31)) : NO_DIGIT;
+        const uint32_t width = window_width(w, wb, nw);
+        const uint32_t v = window_bits(s, window_pos(w, wb, nw), width) + carry;
+        uint32_t mag, neg;
+        signed_digit<GLV>(v, width, mag, neg, carry);
+        digits[(size_t)w * n + i] = mag ? ((mag - 1) | ((neg ^ sign) << 31)) : NO_DIGIT;
     }
 }
 
@@ -341,21 +512,25 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     if (npts > 0x7fffffffu) throw Error(ZKG_ERR_INVALID, "an MSM holds at most 2^31 - 1 points");
     n = (uint32_t)npts;
     c = win;
-    W = (255 + c - 1) / c;
+    glv = msm_glv_enabled(npts);
+    nv = glv ? 2 * n : n;
+    W = glv ? (128 + c - 1) / c : (255 + c - 1) / c;
+    wb = glv ? 128 / W : c;
+    nw = glv ? 128 - wb * W : 0;
     B = 1u << (c - 1);
-    CH = item_points ? item_points : msm_auto_item_points(n, c);
+    CH = item_points ? item_points : msm_auto_item_points(nv, c);
     L = 4;
     T = B / L;
     aw = group == 1 ? 16 : 32;
     xw = 2 * aw;
     const size_t S = (size_t)W * B;
-    max_items = (size_t)W * (((size_t)n + CH - 1) / CH + B);
+    max_items = (size_t)W * (((size_t)nv + CH - 1) / CH + B);
     if (max_items >= 0xffffffffull) throw Error(ZKG_ERR_INVALID, "too many work items for one plan");
-    for (uint64_t s = 1, m = ((uint64_t)n + CH - 1) / CH; s < m; s *= SUM_GROUP) ++merge_rounds;
+    for (uint64_t s = 1, m = ((uint64_t)nv + CH - 1) / CH; s < m; s *= SUM_GROUP) ++merge_rounds;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
     lds_sort = c <= 16;
     sort_pts = std::max<uint32_t>(B, 8192);
-    nblk = (n + sort_pts - 1) / sort_pts;
+    nblk = (nv + sort_pts - 1) / sort_pts;
     device_bind(d);
     if (lds_sort) {
         // 4 * B bytes of LDS per sort block, up to 128 KB at c = 16. The limit is a property of
@@ -373,9 +548,13 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
         ZKG_HIP(hipMalloc(&starts, 4 * (S + 1)));
         ZKG_HIP(hipMalloc(&istarts, 4 * (S + 1)));
         ZKG_HIP(hipMalloc(&scan_tmp, 8 * nb));
-        ZKG_HIP(hipMalloc(&sorted, 4 * (size_t)W * n));
+        if (glv) {
+            ZKG_HIP(hipMalloc(&split, 16 * (size_t)nv));
+            ZKG_HIP(hipMalloc(&endo_x, 2 * (size_t)n * aw));
+        }
+        ZKG_HIP(hipMalloc(&sorted, 4 * (size_t)W * nv));
         if (lds_sort) {
-            ZKG_HIP(hipMalloc(&digits, 4 * (size_t)W * n));
+            ZKG_HIP(hipMalloc(&digits, 4 * (size_t)W * nv));
             ZKG_HIP(hipMalloc(&table, 4 * (size_t)W * nblk * B));
         }
         ZKG_HIP(hipMalloc(&item_slot, 4 * max_items));
@@ -392,25 +571,35 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
 }
 
 MsmPlan::~MsmPlan() {
-    for (uint32_t* p : {counts, cursors, starts, istarts, scan_tmp, sorted, digits, table, item_slot, item_pts, buckets,
-                        partials, wsums, result})
+    for (uint32_t* p : {split, endo_x, counts, cursors, starts, istarts, scan_tmp, sorted, digits, table, item_slot,
+                        item_pts, buckets, partials, wsums, result})
         if (p) (void)hipFree(p);
-    counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_slot = item_pts = buckets =
-        partials = wsums = result = nullptr;
+    split = endo_x = counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_slot = item_pts =
+        buckets = partials = wsums = result = nullptr;
 }
 
 size_t MsmPlan::workspace_bytes() const { return msm_workspace_bytes(group, n, c, CH); }
 
+bool msm_glv_enabled(size_t n) {
+    // virtual point indices (2n of them) must fit the 31-bit index field of a sorted entry
+    if (n > 0x3fffffffu) return false;
+    const char* e = std::getenv("ZKGPU_MSM_GLV");
+    return !(e && e[0] == '0');
+}
+
 size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points) {
-    const uint32_t W = (255 + c - 1) / c, B = 1u << (c - 1);
-    const uint32_t CH = item_points ? item_points : msm_auto_item_points(n, c);
+    const bool glv = msm_glv_enabled(n);
+    const size_t nv = glv ? 2 * n : n;
+    const uint32_t W = glv ? (128 + c - 1) / c : (255 + c - 1) / c, B = 1u << (c - 1);
+    const uint32_t CH = item_points ? item_points : msm_auto_item_points(nv, c);
     const uint32_t xw = group == 1 ? 32 : 64, T = B / 4;
-    const size_t S = (size_t)W * B, max_items = (size_t)W * ((n + CH - 1) / CH + B);
+    const size_t S = (size_t)W * B, max_items = (size_t)W * ((nv + CH - 1) / CH + B);
     const bool lds = c <= 16;
-    const size_t P = std::max<uint32_t>(B, 8192), nblk = (n + P - 1) / P;
+    const size_t P = std::max<uint32_t>(B, 8192), nblk = (nv + P - 1) / P;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    return 4 * (4 * S + 2 + 2 * nb + (size_t)W * n * (lds ? 2 : 1) + (lds ? (size_t)W * nblk * B : 0) +
-                max_items * (1 + (size_t)xw) + S * xw + (size_t)W * T * xw + (size_t)W * xw + xw);
+    return 4 * ((glv ? 4 * nv + n * xw / 2 : 0) + 4 * S + 2 + 2 * nb + (size_t)W * nv * (lds ? 2 : 1) +
+                (lds ? (size_t)W * nblk * B : 0) + max_items * (1 + (size_t)xw) + S * xw + (size_t)W * T * xw +
+                (size_t)W * xw + xw);
 }
 
 template <class Fe>
@@ -438,18 +627,35 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         ZKG_HIP(hipStreamSynchronize(s));
         t0 = std::chrono::steady_clock::now();
     }
-    const unsigned pg = ceil_div(p.n, 256);
+    // GLV: split every scalar into two signed 127-bit halves and compute phi(P_i) = (beta x_i, y_i)
+    const uint32_t* dscal = scalars;
+    if (p.glv) {
+        hipLaunchKernelGGL(msm_glv_split_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, scalars, p.n, p.split);
+        hipLaunchKernelGGL(msm_endo_kernel, dim3(ceil_div((size_t)p.n * p.group, 256)), dim3(256), 0, s, bases, p.n,
+                           p.group, p.endo_x);
+        dscal = p.split;
+    }
+    const unsigned pg = ceil_div(p.nv, 256);
     if (p.lds_sort) {
-        hipLaunchKernelGGL(msm_digits_store_kernel, dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.digits);
-        hipLaunchKernelGGL(msm_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.n, p.B,
+        if (p.glv)
+            hipLaunchKernelGGL(msm_digits_store_kernel<true>, dim3(pg), dim3(256), 0, s, dscal, p.nv, p.W, p.wb,
+                               p.nw, p.digits);
+        else
+            hipLaunchKernelGGL(msm_digits_store_kernel<false>, dim3(pg), dim3(256), 0, s, dscal, p.nv, p.W, p.wb,
+                               p.nw, p.digits);
+        hipLaunchKernelGGL(msm_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.nv, p.B,
                            p.sort_pts, p.nblk, p.table);
         hipLaunchKernelGGL(msm_colscan_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.table, p.W, p.B, p.nblk,
                            p.counts);
     } else {
         fill_u32(p.counts, 0, S, s);
         fill_u32(p.cursors, 0, S, s);
-        hipLaunchKernelGGL((msm_digits_kernel<false>), dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.counts,
-                           (const uint32_t*)nullptr, (uint32_t*)nullptr);
+        if (p.glv)
+            hipLaunchKernelGGL((msm_digits_kernel<false, true>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+                               p.wb, p.nw, p.counts, (const uint32_t*)nullptr, (uint32_t*)nullptr);
+        else
+            hipLaunchKernelGGL((msm_digits_kernel<false, false>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+                               p.wb, p.nw, p.counts, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     }
     stage("count");
     hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(SCAN_THREADS), 0, s, p.counts, S, p.CH, p.starts, p.istarts,
@@ -458,11 +664,14 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
     hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.scan_tmp, S, p.starts, p.istarts);
     stage("scan");
     if (p.lds_sort) {
-        hipLaunchKernelGGL(msm_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.n, p.B,
+        hipLaunchKernelGGL(msm_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.nv, p.B,
                            p.sort_pts, p.nblk, p.table, p.starts, p.sorted);
+    } else if (p.glv) {
+        hipLaunchKernelGGL((msm_digits_kernel<true, true>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+                           p.wb, p.nw, p.cursors, p.starts, p.sorted);
     } else {
-        hipLaunchKernelGGL((msm_digits_kernel<true>), dim3(pg), dim3(256), 0, s, scalars, p.n, p.c, p.W, p.cursors,
-                           p.starts, p.sorted);
+        hipLaunchKernelGGL((msm_digits_kernel<true, false>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+                           p.wb, p.nw, p.cursors, p.starts, p.sorted);
     }
     stage("scatter");
     launch_accumulate<Fe>(p, bases, s);

@@ -16,12 +16,23 @@ uint32_t msm_default_window(size_t n);          // clamp(ilog2(max(n,1)) - 5, 11
 uint32_t msm_auto_item_points(size_t n, uint32_t c);
 // Device bytes a plan for (group, n, c, item_points) allocates (item_points 0 = automatic).
 size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points);
+// Whether plans use the GLV split (default on; ZKGPU_MSM_GLV=0 turns it off for comparisons).
+bool msm_glv_enabled(size_t n);
 
 struct MsmPlan {
     Device* dev = nullptr;
     int group = 1;           // 1 = G1 (Fq), 2 = G2 (Fq2)
     uint32_t n = 0, c = 0;
-    uint32_t W = 0;          // windows, ceil(255 / c)
+    // GLV: s = k1 + k2*lambda (mod r) with |k1|, |k2| < 2^127, and lambda*P = phi(P) = (beta*x, y),
+    // so the MSM runs over nv = 2n virtual points (P_i with k1_i, phi(P_i) with k2_i) with
+    // 128-bit scalars: half the windows, half the bucket reduction and half the doubling chain.
+    bool glv = false;
+    uint32_t nv = 0;         // virtual points: 2n with GLV, else n
+    uint32_t W = 0;          // windows, ceil(128 / c) with GLV, else ceil(255 / c)
+    // Window w covers bits [w*wb + min(w, nw), +wb + (w < nw)): without GLV wb = c, nw = 0; with
+    // GLV the 128 bits are split evenly (c = 15: two windows of 15 bits and seven of 14), so no
+    // window is left with a few bits and a few heavy buckets.
+    uint32_t wb = 0, nw = 0;
     uint32_t B = 0;          // buckets per window, 2^(c-1)
     uint32_t CH = 0;         // points per work item
     uint32_t L = 0;          // buckets per running-sum thread
@@ -35,13 +46,15 @@ struct MsmPlan {
     uint32_t nblk = 0;       // sort blocks per window
 
     // workspace (device)
+    uint32_t* split = nullptr;       // nv * 4  GLV half-scalars: |k| in 127 bits, sign in bit 127
+    uint32_t* endo_x = nullptr;      // n * aw / 2  beta * x of every base (Montgomery form)
     uint32_t* counts = nullptr;      // W*B  points per (window, bucket)
     uint32_t* cursors = nullptr;     // W*B  scatter cursors
     uint32_t* starts = nullptr;      // W*B + 1  exclusive scan of counts (global over windows)
     uint32_t* istarts = nullptr;     // W*B + 1  exclusive scan of items per slot
     uint32_t* scan_tmp = nullptr;    // per-block partial sums of the scan
-    uint32_t* sorted = nullptr;      // W*n  point index | sign << 31, grouped by slot
-    uint32_t* digits = nullptr;      // W*n  bucket | sign << 31 per (window, point)   (LDS sort only)
+    uint32_t* sorted = nullptr;      // W*nv  point index | sign << 31, grouped by slot
+    uint32_t* digits = nullptr;      // W*nv  bucket | sign << 31 per (window, point)  (LDS sort only)
     uint32_t* table = nullptr;       // W*nblk*B  per-block bucket counts -> offsets    (LDS sort only)
     uint32_t* item_slot = nullptr;   // max_items
     uint32_t* item_pts = nullptr;    // max_items * xw  per-item partial sums
@@ -59,6 +72,10 @@ struct MsmPlan {
              std::vector<std::pair<std::string, double>>* stage_ms = nullptr) const;
     size_t workspace_bytes() const;
 };
+
+// GLV split of n scalars (any 256-bit values) into split[0..4n) = k1 of each, split[4n..8n) = k2
+// of each: |k| in the low 127 bits, its sign in bit 127, s = k1 + k2 lambda (mod r).
+void msm_glv_split(const uint32_t* scalars, uint32_t n, uint32_t* split, hipStream_t stream);
 
 // Plain affine coordinates -> Montgomery form in place (`coords` elements of 8 words).
 void msm_convert_bases(uint32_t* xy, size_t coords, hipStream_t stream);

@@ -10,8 +10,11 @@ namespace zkg {
 
 // ---------------------------------------------------------------- bucket accumulation
 
+// Sorted entries name virtual points v | sign << 31: v < n_real is base v, v >= n_real (GLV) is
+// phi(base v - n_real) = (endo_x[v - n_real], y). Without GLV n_real is 2^32 - 1.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_accumulate_kernel(const uint32_t* __restrict__ bases,
+                                                            const uint32_t* __restrict__ endo_x, uint32_t n_real,
                                                             const uint32_t* __restrict__ sorted,
                                                             const uint32_t* __restrict__ starts,
                                                             const uint32_t* __restrict__ istarts,
@@ -34,8 +37,12 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(const uint32_t* __r
     const uint32_t* entries = sorted + starts[slot] + skip;
     Xyzz<Fe> acc = xyzz_inf<Fe>();
     for (uint32_t j = 0; j < len; ++j) {
-        const uint32_t e = entries[j];
-        Affine<Fe> p = load_affine<Fe>(bases + (size_t)(2 * Tr::WORDS) * (e & 0x7fffffffu));
+        const uint32_t e = entries[j], v = e & 0x7fffffffu;
+        const bool endo = v >= n_real;
+        const uint32_t idx = endo ? v - n_real : v;
+        const uint32_t* yp = bases + (size_t)(2 * Tr::WORDS) * idx + Tr::WORDS;
+        const uint32_t* xp = endo ? endo_x + (size_t)Tr::WORDS * idx : yp - Tr::WORDS;
+        Affine<Fe> p{Tr::load(xp), Tr::load(yp)};
         if (e >> 31) p.y = Tr::zero() - p.y;
         acc = xyzz_madd(acc, p);
     }
@@ -76,8 +83,8 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
     uint32_t stride = 1;
     while ((total + stride - 1) / stride > SUM_GROUP) stride *= SUM_GROUP;
     const size_t xw = 4 * Tr::WORDS;
-    Xyzz<Fe> acc = xyzz_inf<Fe>();
-    for (uint32_t k = 0; k < total; k += stride) acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (first + k)));
+    Xyzz<Fe> acc = total ? load_xyzz<Fe>(item_pts + xw * first) : xyzz_inf<Fe>();
+    for (uint32_t k = stride; k < total; k += stride) acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (first + k)));
     store_xyzz(buckets + xw * slot, acc);
 }
 
@@ -90,7 +97,7 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
 // body for b0 * run (double-and-add over c bits) and the final + tot.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restrict__ buckets, uint32_t W,
-                                                        uint32_t B, uint32_t L, uint32_t c,
+                                                        uint32_t B, uint32_t L, uint32_t c, uint32_t wb, uint32_t nw,
                                                         uint32_t* __restrict__ partials) {
     typedef FeTraits<Fe> Tr;
     const uint32_t T = B / L;
@@ -98,6 +105,10 @@ __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restr
     if (t >= W * T) return;
     const uint32_t w = t / T, j = t % T, b0 = j * L;
     const size_t xw = 4 * Tr::WORDS;
+    if (b0 >= (1u << (wb + (w < nw ? 1u : 0u) - 1))) {  // beyond this window's digit range: empty
+        store_xyzz(partials + xw * t, xyzz_inf<Fe>());
+        return;
+    }
     const uint32_t* src = buckets + xw * ((size_t)w * B + b0);
     Xyzz<Fe> run = xyzz_inf<Fe>(), tot = xyzz_inf<Fe>();
     // steps 2k: run += bucket[L-1-k]; steps 2k+1: tot += run
@@ -139,7 +150,8 @@ __global__ __launch_bounds__(THREADS) void msm_window_sum_kernel(const uint32_t*
 }
 
 // ---------------------------------------------------------------- window combine
-// result = sum_w 2^(c w) S_w by Horner's rule from the top window: (W-1)*c doublings in one
+// result = sum_w 2^(pos_w) S_w (pos_w = bit offset of window w) by Horner's rule from the top
+// window: pos_(W-1) doublings (240 for c = 15 without GLV, 114 with it) in one
 // dependency chain, which no amount of parallelism across points can shorten. What can be
 // shortened is the latency of each step: the multiplications of one doubling fall into three
 // levels of independent products (V = U^2, X^2 | W = UV, S = XV, ZZ3 = V ZZ, M^2 |
@@ -237,8 +249,8 @@ ZKG_DEV Xyzz<Fe> xyzz_add_lanes(const Xyzz<Fe>& p, const Xyzz<Fe>& q, uint32_t l
 }
 
 template <class Fe>
-__global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restrict__ wsums, uint32_t W, uint32_t c,
-                                                        uint32_t* __restrict__ result) {
+__global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restrict__ wsums, uint32_t W, uint32_t wb,
+                                                        uint32_t nw, uint32_t* __restrict__ result) {
     typedef FeTraits<Fe> Tr;
     constexpr int XW = 4 * Tr::WORDS;
     const uint32_t lane = threadIdx.x;
@@ -246,7 +258,8 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
     const uint32_t zero = lane >> 6;
     Xyzz<Fe> acc = load_xyzz<Fe>(wsums + (size_t)XW * (W - 1 + zero));
     for (int w = (int)W - 2; w >= 0; --w) {
-        for (uint32_t k = 0; k < c; ++k)
+        const uint32_t width = wb + ((uint32_t)w < nw ? 1u : 0u);  // bits between window w and w + 1
+        for (uint32_t k = 0; k < width; ++k)
             if (!xyzz_is_inf(acc)) acc = xyzz_dbl_lanes(acc, lane);
         acc = xyzz_add_lanes(acc, load_xyzz<Fe>(wsums + (size_t)XW * (w + zero)), lane);
     }
@@ -259,8 +272,9 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
 
 template <class Fe>
 void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
-    hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases, p.sorted,
-                       p.starts, p.istarts, p.counts, p.W * p.B, p.CH, p.item_slot, p.item_pts);
+    hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
+                       (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts, p.counts,
+                       p.W * p.B, p.CH, p.item_slot, p.item_pts);
 }
 
 template <class Fe>
@@ -279,7 +293,7 @@ void launch_finalize(const MsmPlan& p, hipStream_t s) {
 template <class Fe>
 void launch_reduce(const MsmPlan& p, hipStream_t s) {
     hipLaunchKernelGGL(msm_reduce_kernel<Fe>, dim3(ceil_div((size_t)p.W * p.T, 256)), dim3(256), 0, s, p.buckets, p.W,
-                       p.B, p.L, p.c, p.partials);
+                       p.B, p.L, p.c, p.wb, p.nw, p.partials);
 }
 
 template <class Fe>
@@ -291,7 +305,7 @@ void launch_tree(const MsmPlan& p, hipStream_t s) {
 
 template <class Fe>
 void launch_combine(const MsmPlan& p, hipStream_t s) {
-    hipLaunchKernelGGL(msm_combine_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.c, p.result);
+    hipLaunchKernelGGL(msm_combine_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.wb, p.nw, p.result);
 }
 
 }  // namespace zkg

@@ -11,6 +11,7 @@
 #include "api_types.hpp"
 #include "curve.hpp"
 #include "device_util.hpp"
+#include "msm.hpp"
 #include "util_point.hpp"
 #include "runtime.hpp"
 
@@ -141,6 +142,22 @@ int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t c
     *ms = f;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_test_glv_split(zkg_device* dev, const uint32_t* scalars, size_t n, uint32_t* out) {
+    if (!dev || !scalars || !out) return zkg::fail(zkg::ZKG_ERR_INVALID, "null argument");
+    if (n == 0) return zkg::ZKG_OK;
+    if (n > 0x3fffffffu) return zkg::fail(zkg::ZKG_ERR_INVALID, "too many scalars");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    zkg::DeviceArray<uint32_t> ds(8 * n), dout(8 * n);
+    hipStream_t s = dev->d->stream;
+    ZKG_HIP(hipMemcpyAsync(ds.ptr, scalars, 32 * n, hipMemcpyHostToDevice, s));
+    zkg::msm_glv_split(ds.ptr, (uint32_t)n, dout.ptr, s);
+    ZKG_HIP(hipMemcpyAsync(out, dout.ptr, 32 * n, hipMemcpyDeviceToHost, s));
+    ZKG_HIP(hipStreamSynchronize(s));
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
