@@ -11,9 +11,9 @@
 //             one atomic per group of equal slots in a wave.
 //  accumulate one lane per work item (at most CH points of one bucket): a chain of mixed
 //             additions of the gathered affine bases into an XYZZ accumulator.
-//  merge r    heavy buckets: items k = 0 mod s*G absorb items k + s .. k + (G-1)s (s = G^r), so a
-//             bucket holding most of the points never becomes a serial tail.
-//  finalize   each bucket sums its at most G remaining items.
+//  merge r    heavy buckets (more than 16 items): items k = 0 mod 4s absorb items k + s, k + 2s,
+//             k + 3s (s = 4^r), so a bucket holding most of the points never becomes a serial tail.
+//  finalize   each bucket sums its at most 16 remaining items.
 //  reduce     S_w = sum_b (b + 1) bucket[w][b]: L buckets per lane with running sums, plus b0 * run.
 //  tree sum   per-window sum of the lane results in one workgroup (LDS tree).
 //  combine    Horner's rule over the windows from the top, one wave with lane-parallel products.
@@ -526,7 +526,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     const size_t S = (size_t)W * B;
     max_items = (size_t)W * (((size_t)nv + CH - 1) / CH + B);
     if (max_items >= 0xffffffffull) throw Error(ZKG_ERR_INVALID, "too many work items for one plan");
-    for (uint64_t s = 1, m = ((uint64_t)nv + CH - 1) / CH; s < m; s *= SUM_GROUP) ++merge_rounds;
+    for (uint64_t m = ((uint64_t)nv + CH - 1) / CH; m > FINAL_GROUP; m = (m + SUM_GROUP - 1) / SUM_GROUP) ++merge_rounds;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
     lds_sort = c <= 16;
     sort_pts = std::max<uint32_t>(B, 8192);
@@ -557,7 +557,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
             ZKG_HIP(hipMalloc(&digits, 4 * (size_t)W * nv));
             ZKG_HIP(hipMalloc(&table, 4 * (size_t)W * nblk * B));
         }
-        ZKG_HIP(hipMalloc(&item_slot, 4 * max_items));
+        ZKG_HIP(hipMalloc(&item_info, 8 * max_items));
         ZKG_HIP(hipMalloc(&item_pts, 4 * max_items * xw));
         ZKG_HIP(hipMalloc(&buckets, 4 * S * xw));
         ZKG_HIP(hipMalloc(&partials, 4 * (size_t)W * T * xw));
@@ -571,11 +571,13 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
 }
 
 MsmPlan::~MsmPlan() {
-    for (uint32_t* p : {split, endo_x, counts, cursors, starts, istarts, scan_tmp, sorted, digits, table, item_slot,
-                        item_pts, buckets, partials, wsums, result})
+    for (uint32_t* p : {split, endo_x, counts, cursors, starts, istarts, scan_tmp, sorted, digits, table, item_pts,
+                        buckets, partials, wsums, result})
         if (p) (void)hipFree(p);
-    split = endo_x = counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_slot = item_pts =
-        buckets = partials = wsums = result = nullptr;
+    if (item_info) (void)hipFree(item_info);
+    split = endo_x = counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_pts = buckets =
+        partials = wsums = result = nullptr;
+    item_info = nullptr;
 }
 
 size_t MsmPlan::workspace_bytes() const { return msm_workspace_bytes(group, n, c, CH); }
@@ -598,7 +600,7 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
     const size_t P = std::max<uint32_t>(B, 8192), nblk = (nv + P - 1) / P;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
     return 4 * ((glv ? 4 * nv + n * xw / 2 : 0) + 4 * S + 2 + 2 * nb + (size_t)W * nv * (lds ? 2 : 1) +
-                (lds ? (size_t)W * nblk * B : 0) + max_items * (1 + (size_t)xw) + S * xw + (size_t)W * T * xw +
+                (lds ? (size_t)W * nblk * B : 0) + max_items * (2 + (size_t)xw) + S * xw + (size_t)W * T * xw +
                 (size_t)W * xw + xw);
 }
 

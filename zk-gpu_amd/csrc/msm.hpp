@@ -56,7 +56,7 @@ struct MsmPlan {
     uint32_t* sorted = nullptr;      // W*nv  point index | sign << 31, grouped by slot
     uint32_t* digits = nullptr;      // W*nv  bucket | sign << 31 per (window, point)  (LDS sort only)
     uint32_t* table = nullptr;       // W*nblk*B  per-block bucket counts -> offsets    (LDS sort only)
-    uint32_t* item_slot = nullptr;   // max_items
+    uint2* item_info = nullptr;      // max_items  (position in its bucket, items in its bucket)
     uint32_t* item_pts = nullptr;    // max_items * xw  per-item partial sums
     uint32_t* buckets = nullptr;     // W*B * xw
     uint32_t* partials = nullptr;    // W*T * xw  running-sum results

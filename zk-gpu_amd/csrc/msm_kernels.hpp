@@ -2,6 +2,8 @@
 // combine) as kernel templates over the coordinate field, with their launchers. Included only
 // by the translation units that instantiate them (msm_g1.hip, msm_g2_*.hip).
 #pragma once
+#include <algorithm>
+
 #include "curve.hpp"
 #include "msm.hpp"
 #include "msm_stages.hpp"
@@ -13,13 +15,14 @@ namespace zkg {
 // Sorted entries name virtual points v | sign << 31: v < n_real is base v, v >= n_real (GLV) is
 // phi(base v - n_real) = (endo_x[v - n_real], y). Without GLV n_real is 2^32 - 1.
 template <class Fe>
-__global__ __launch_bounds__(256) void msm_accumulate_kernel(const uint32_t* __restrict__ bases,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void msm_accumulate_kernel(
+                                                            const uint32_t* __restrict__ bases,
                                                             const uint32_t* __restrict__ endo_x, uint32_t n_real,
                                                             const uint32_t* __restrict__ sorted,
                                                             const uint32_t* __restrict__ starts,
                                                             const uint32_t* __restrict__ istarts,
                                                             const uint32_t* __restrict__ counts, uint32_t S,
-                                                            uint32_t ch, uint32_t* __restrict__ item_slot,
+                                                            uint32_t ch, uint2* __restrict__ item_info,
                                                             uint32_t* __restrict__ item_pts) {
     typedef FeTraits<Fe> Tr;
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
@@ -47,21 +50,24 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(const uint32_t* __r
         acc = xyzz_madd(acc, p);
     }
     store_xyzz(item_pts + (size_t)(4 * Tr::WORDS) * item, acc);
-    item_slot[item] = slot;
+    // (position in the bucket, items in the bucket) for the merge rounds
+    item_info[item] = make_uint2(item - istarts[slot], istarts[slot + 1] - istarts[slot]);
 }
 
+// Round r (stride s = G^r): in a bucket with more than FINAL_GROUP partial sums left, item
+// k = 0 mod s*G absorbs items k + s .. k + (G-1)s. G = 4 keeps each round's chain at three
+// additions; typical buckets (up to FINAL_GROUP items) skip the rounds.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_merge_kernel(uint32_t* __restrict__ item_pts,
-                                                       const uint32_t* __restrict__ item_slot,
+                                                       const uint2* __restrict__ item_info,
                                                        const uint32_t* __restrict__ istarts, uint32_t S,
                                                        uint32_t stride) {
     typedef FeTraits<Fe> Tr;
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= istarts[S]) return;
-    const uint32_t slot = item_slot[item];
-    const uint32_t first = istarts[slot];
-    const uint32_t k = item - first, total = istarts[slot + 1] - first;
-    if ((total + stride - 1) / stride <= SUM_GROUP) return;
+    const uint2 info = item_info[item];
+    const uint32_t k = info.x, total = info.y;
+    if ((total + stride - 1) / stride <= FINAL_GROUP) return;
     if (k % (stride * SUM_GROUP) != 0) return;
     const size_t xw = 4 * Tr::WORDS;
     Xyzz<Fe> acc = load_xyzz<Fe>(item_pts + xw * item);
@@ -81,7 +87,7 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
     if (slot >= S) return;
     const uint32_t first = istarts[slot], total = istarts[slot + 1] - first;
     uint32_t stride = 1;
-    while ((total + stride - 1) / stride > SUM_GROUP) stride *= SUM_GROUP;
+    while ((total + stride - 1) / stride > FINAL_GROUP) stride *= SUM_GROUP;
     const size_t xw = 4 * Tr::WORDS;
     Xyzz<Fe> acc = total ? load_xyzz<Fe>(item_pts + xw * first) : xyzz_inf<Fe>();
     for (uint32_t k = stride; k < total; k += stride) acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (first + k)));
@@ -90,60 +96,65 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
 
 // ---------------------------------------------------------------- bucket reduction
 
-// Lane t of window w takes buckets b0 .. b0+L-1 (b0 = j*L) and returns
-// sum_k (k+1) bucket[b0+k] + b0 * sum_k bucket[b0+k]  =  sum_k (b0+k+1) bucket[b0+k].
-// Each loop has a single point-addition site (operands chosen by selects), which keeps the
-// fully inlined kernel small: one add body for the running sums, one doubling and one add
-// body for b0 * run (double-and-add over c bits) and the final + tot.
+// Lane t of window w takes buckets b0 .. b0+L-1 (b0 = j*L) and computes
+// sum_k (k+1) bucket[b0+k] + b0 * sum_k bucket[b0+k]  =  sum_k (b0+k+1) bucket[b0+k];
+// the lanes of a workgroup (all in one window) then add their results in an LDS tree, leaving
+// one partial sum per workgroup. Each loop has a single point-addition site (operands chosen by
+// selects), which keeps the fully inlined kernel small.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restrict__ buckets, uint32_t W,
                                                         uint32_t B, uint32_t L, uint32_t c, uint32_t wb, uint32_t nw,
                                                         uint32_t* __restrict__ partials) {
     typedef FeTraits<Fe> Tr;
-    const uint32_t T = B / L;
+    constexpr int XW = 4 * Tr::WORDS;
+    __shared__ uint32_t lds[128 * XW];
+    const uint32_t T = B / L;  // lanes per window, a multiple of blockDim.x
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= W * T) return;
     const uint32_t w = t / T, j = t % T, b0 = j * L;
-    const size_t xw = 4 * Tr::WORDS;
-    if (b0 >= (1u << (wb + (w < nw ? 1u : 0u) - 1))) {  // beyond this window's digit range: empty
-        store_xyzz(partials + xw * t, xyzz_inf<Fe>());
-        return;
-    }
-    const uint32_t* src = buckets + xw * ((size_t)w * B + b0);
-    Xyzz<Fe> run = xyzz_inf<Fe>(), tot = xyzz_inf<Fe>();
-    // steps 2k: run += bucket[L-1-k]; steps 2k+1: tot += run
-    for (uint32_t step = 0; step < 2 * L; ++step) {
-        const bool odd = step & 1;
-        const Xyzz<Fe> x = select_xyzz(odd, tot, run);
-        const Xyzz<Fe> y = select_xyzz(odd, run, load_xyzz<Fe>(src + xw * (L - 1 - step / 2)));
-        const Xyzz<Fe> z = xyzz_add(x, y);
-        run = select_xyzz(odd, run, z);
-        tot = select_xyzz(odd, z, tot);
-    }
-    // r = b0 * run by double-and-add over c bits, then r + tot in one extra iteration
     Xyzz<Fe> r = xyzz_inf<Fe>();
-    for (int k = (int)c - 1; k >= -1; --k) {
-        if (k >= 0 && !xyzz_is_inf(r)) r = xyzz_dbl(r);
-        if (k < 0 || ((b0 >> k) & 1u)) r = xyzz_add(r, select_xyzz(k < 0, tot, run));
+    if (b0 < (1u << (wb + (w < nw ? 1u : 0u) - 1))) {  // else beyond this window's digit range: empty
+        const uint32_t* src = buckets + (size_t)XW * ((size_t)w * B + b0);
+        Xyzz<Fe> run = xyzz_inf<Fe>(), tot = xyzz_inf<Fe>();
+        // steps 2k: run += bucket[L-1-k]; steps 2k+1: tot += run
+        for (uint32_t step = 0; step < 2 * L; ++step) {
+            const bool odd = step & 1;
+            const Xyzz<Fe> x = select_xyzz(odd, tot, run);
+            const Xyzz<Fe> y = select_xyzz(odd, run, load_xyzz<Fe>(src + XW * (L - 1 - step / 2)));
+            const Xyzz<Fe> z = xyzz_add(x, y);
+            run = select_xyzz(odd, run, z);
+            tot = select_xyzz(odd, z, tot);
+        }
+        // r = b0 * run by double-and-add over c bits, then r + tot in one extra iteration
+        for (int k = (int)c - 1; k >= -1; --k) {
+            if (k >= 0 && !xyzz_is_inf(r)) r = xyzz_dbl(r);
+            if (k < 0 || ((b0 >> k) & 1u)) r = xyzz_add(r, select_xyzz(k < 0, tot, run));
+        }
     }
-    store_xyzz(partials + xw * t, r);
+    for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XW * (threadIdx.x - off), r);
+        __syncthreads();
+        if (threadIdx.x < off) r = xyzz_add(r, load_xyzz<Fe>(lds + XW * threadIdx.x));
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) store_xyzz(partials + (size_t)XW * blockIdx.x, r);
 }
 
-// One workgroup per window: strided serial sums, then an LDS tree.
-template <class Fe, int THREADS>
-__global__ __launch_bounds__(THREADS) void msm_window_sum_kernel(const uint32_t* __restrict__ partials, uint32_t T,
-                                                                uint32_t* __restrict__ wsums) {
+// One wave per window adds the window's P workgroup partials (P <= 64 at c <= 16 needs no
+// serial part; larger windows add P / 64 per lane first), then an LDS tree of `top` levels.
+template <class Fe>
+__global__ __launch_bounds__(64) void msm_window_sum_kernel(const uint32_t* __restrict__ partials, uint32_t P,
+                                                           uint32_t top, uint32_t* __restrict__ wsums) {
     typedef FeTraits<Fe> Tr;
     constexpr int XW = 4 * Tr::WORDS;
-    __shared__ uint32_t lds[THREADS * XW];
+    __shared__ uint32_t lds[32 * XW];
     const uint32_t w = blockIdx.x;
-    Xyzz<Fe> acc = xyzz_inf<Fe>();
-    for (uint32_t k = threadIdx.x; k < T; k += THREADS)
-        acc = xyzz_add(acc, load_xyzz<Fe>(partials + (size_t)XW * ((size_t)w * T + k)));
-    for (int off = THREADS / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x >= off && (int)threadIdx.x < 2 * off) store_xyzz(lds + XW * (threadIdx.x - off), acc);
+    const uint32_t* src = partials + (size_t)XW * w * P;
+    Xyzz<Fe> acc = threadIdx.x < P ? load_xyzz<Fe>(src + XW * threadIdx.x) : xyzz_inf<Fe>();
+    for (uint32_t k = threadIdx.x + 64; k < P; k += 64) acc = xyzz_add(acc, load_xyzz<Fe>(src + XW * k));
+    for (uint32_t off = top; off > 0; off >>= 1) {
+        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XW * (threadIdx.x - off), acc);
         __syncthreads();
-        if ((int)threadIdx.x < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XW * threadIdx.x));
+        if (threadIdx.x < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XW * threadIdx.x));
         __syncthreads();
     }
     if (threadIdx.x == 0) store_xyzz(wsums + (size_t)XW * w, acc);
@@ -274,13 +285,13 @@ template <class Fe>
 void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
     hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
                        (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts, p.counts,
-                       p.W * p.B, p.CH, p.item_slot, p.item_pts);
+                       p.W * p.B, p.CH, p.item_info, p.item_pts);
 }
 
 template <class Fe>
 void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
     hipLaunchKernelGGL(msm_merge_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, p.item_pts,
-                       p.item_slot, p.istarts, p.W * p.B, stride);
+                       p.item_info, p.istarts, p.W * p.B, stride);
 }
 
 template <class Fe>
@@ -292,15 +303,17 @@ void launch_finalize(const MsmPlan& p, hipStream_t s) {
 
 template <class Fe>
 void launch_reduce(const MsmPlan& p, hipStream_t s) {
-    hipLaunchKernelGGL(msm_reduce_kernel<Fe>, dim3(ceil_div((size_t)p.W * p.T, 256)), dim3(256), 0, s, p.buckets, p.W,
-                       p.B, p.L, p.c, p.wb, p.nw, p.partials);
+    const uint32_t rb = std::min<uint32_t>(256, p.T);  // a workgroup never spans two windows
+    hipLaunchKernelGGL(msm_reduce_kernel<Fe>, dim3((size_t)p.W * p.T / rb), dim3(rb), 0, s, p.buckets, p.W, p.B, p.L,
+                       p.c, p.wb, p.nw, p.partials);
 }
 
 template <class Fe>
 void launch_tree(const MsmPlan& p, hipStream_t s) {
-    // 64 KB of LDS per workgroup either way: 512 G1 points or 256 G2 points
-    constexpr int TH = FeTraits<Fe>::WORDS == 8 ? 512 : 256;
-    hipLaunchKernelGGL((msm_window_sum_kernel<Fe, TH>), dim3(p.W), dim3(TH), 0, s, p.partials, p.T, p.wsums);
+    const uint32_t P = p.T / std::min<uint32_t>(256, p.T);  // workgroup partials per window
+    uint32_t top = 1;
+    while (top < std::min<uint32_t>(P, 64)) top <<= 1;
+    hipLaunchKernelGGL(msm_window_sum_kernel<Fe>, dim3(p.W), dim3(64), 0, s, p.partials, P, top / 2, p.wsums);
 }
 
 template <class Fe>
