@@ -224,6 +224,8 @@ def groth16_msm_leg(gpu, ev, rng, args, log_n=22):
     their own streams) against five plain calls in sequence."""
     from zkgpu import Batch
 
+    from zkgpu import tune
+
     n = 1 << log_n
     c = Msm.default_window(n)
     sets = [(1, n), (1, n), (1, n), (1, n - 1), (2, n)]
@@ -231,18 +233,22 @@ def groth16_msm_leg(gpu, ev, rng, args, log_n=22):
     wit = gpu.buffer(witness_like(rng, n))
     hsc = gpu.buffer(uniform_fr(rng, n - 1))
     scal = [wit, wit, wit, hsc, wit]
-    plans = [Msm.for_group(gpu, Group(g), m, c) for g, m in sets]
+    plain_plans = [Msm.for_group(gpu, Group(g), m, c) for g, m in sets]
+    t0 = time.perf_counter()
+    tuned = {g: tune.tune_msm(gpu, g, n) for g in (1, 2)}
+    tune_s = time.perf_counter() - t0
+    plans = [Msm.tuned(gpu, Group(g), m) for g, m in sets]
     batch = Batch(gpu)
     for p, b, sc in zip(plans, bases, scal):
         batch.add_msm(p, b, sc)
 
     def plain():
-        for p, b, sc in zip(plans, bases, scal):
+        for p, b, sc in zip(plain_plans, bases, scal):
             p.run(gpu, b, sc)
 
     plain()
     gpu.wait()
-    want = [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(plans, sets)]
+    want = [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(plain_plans, sets)]
     batch.run()
     gpu.wait()
     got = [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(plans, sets)]
@@ -251,10 +257,14 @@ def groth16_msm_leg(gpu, ev, rng, args, log_n=22):
     steps = max(3, args.steps // 5)
     plain_ms, _ = time_calls(gpu, ev, plain, steps, 1)
     bat_ms, _ = time_calls(gpu, ev, batch.run, steps, 1)
-    res = {"n": n, "c": c, "msms": "A,B1,C,H(n-1) G1 + B2 G2", "plain_ms": round(plain_ms, 3),
-           "batched_ms": round(bat_ms, 3), "speedup": round(plain_ms / bat_ms, 3)}
-    log(f"Groth16 MSM set 2^{log_n}: plain {plain_ms:.2f} ms, batched {bat_ms:.2f} ms ({plain_ms / bat_ms:.2f}x)")
-    del batch, plans, bases
+    res = {"n": n, "msms": "A,B1,C,H(n-1) G1 + B2 G2, witness-like scalars (H uniform)",
+           "plain": {"c": c, "ms": round(plain_ms, 3)},
+           "batched": {"ms": round(bat_ms, 3), "tuned_g1": tuned[1], "tuned_g2": tuned[2],
+                       "tuning_seconds": round(tune_s, 2)},
+           "speedup": round(plain_ms / bat_ms, 3)}
+    log(f"Groth16 MSM set 2^{log_n}: plain {plain_ms:.2f} ms, batched+tuned {bat_ms:.2f} ms "
+        f"({plain_ms / bat_ms:.2f}x); tuned {tuned}")
+    del batch, plans, plain_plans, bases
     return res
 
 
@@ -341,6 +351,14 @@ def serving_leg(gpu, rng, args):
         return {"jobs_per_s": round(jobs_n / span, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_ms": round(float(np.mean(lat)), 3)}
 
+    from zkgpu import tune
+
+    t0 = time.perf_counter()
+    for ln in sorted(set(log_ns)):
+        for g in (1, 2):
+            tune.tune_msm(gpu, g, 1 << ln)
+    tune_s = time.perf_counter() - t0
+    log(f"serving: tuned the key sizes in {tune_s:.1f} s")
     server = Server(gpu, args.serve_budget_gib << 30, max_inflight=args.serve_inflight)
     kid = [server.add_key(k["log_n"], k["g1"], k["g2"]) for k in keys]
 
@@ -408,8 +426,9 @@ def serving_leg(gpu, rng, args):
         raise SystemExit("a served job differs from the same job run alone: refusing to report a time")
 
     out = {"keys_log_n": log_ns, "jobs": jobs_n, "mean_plain_service_ms": round(1e3 * mean_service, 3),
-           "inflight": args.serve_inflight, "budget_GiB": args.serve_budget_gib}
-    for name, load in (("saturating", 4.0), ("moderate", 0.6)):
+           "inflight": args.serve_inflight, "budget_GiB": args.serve_budget_gib,
+           "server_msm_plans": "tuned per key size", "plain_msm_plans": "default_window", "tuning_seconds": round(tune_s, 2)}
+    for name, load in (("saturating", 4.0), ("moderate", 0.75)):
         arrivals = schedule(load / mean_service)
         p = run_plain(arrivals)
         sv = run_server(arrivals)
@@ -511,6 +530,16 @@ def main():
     g1_e2e = (time.perf_counter() - t0) * 1e3 / 3
     stages = m1.profile(gpu, bb1, sb1)
     log(f"G1 2^20: {g1_ms:.3f} ms (wall {g1_wall:.3f}, e2e {g1_e2e:.3f}); stages {stages}")
+    # the same single call with the autotuner's window / item / sort sizes for this size
+    from zkgpu import tune
+
+    tuned1 = tune.tune_msm(gpu, 1, n1)
+    mt = Msm.tuned(gpu, Group.G1, n1)
+    if not np.array_equal(affine_words(gpu.read(mt.run(gpu, bb1, sb1), 32), 1), g1_affine):
+        raise SystemExit("tuned G1 MSM 2^20 differs from the default plan: refusing to report a time")
+    g1_tuned_ms, _ = time_calls(gpu, ev, lambda: mt.run(gpu, bb1, sb1), args.steps, args.warmup)
+    log(f"G1 2^20 tuned {tuned1}: {g1_tuned_ms:.3f} ms")
+    del mt
 
     # ---------------- G2 MSM 2^18
     n2 = 1 << 18
@@ -566,7 +595,8 @@ def main():
 
     cpu = None
     if args.workload == "msm":
-        print(json.dumps({"g1_msm_2^20_ms": round(g1_ms, 4), "g2_msm_2^18_ms": round(g2_ms, 4),
+        print(json.dumps({"g1_msm_2^20_ms": round(g1_ms, 4), "g1_msm_2^20_tuned_ms": round(g1_tuned_ms, 4),
+                          "g2_msm_2^18_ms": round(g2_ms, 4),
                           "stages": {k: round(v, 4) for k, v in stages}}), flush=True)
         return
     if not args.no_cpu:
@@ -594,6 +624,8 @@ def main():
             "g1_msm_2^20_ms": round(g1_ms, 4),
             "g1_msm_2^20_e2e_ms": round(g1_e2e, 3),
             "g1_msm_2^20_stages_ms": {k: round(v, 4) for k, v in stages},
+            "g1_msm_2^20_tuned": {"ms": round(g1_tuned_ms, 4), "c": tuned1["c"], "item_points": tuned1["item_points"],
+                                  "sort_points": tuned1["sort_points"]},
             "g2_msm_2^18_ms": round(g2_ms, 4),
             "ntt": {f"2^{k}": v for k, v in ntt.items()},
             "ntt_2^24_hbm_GBps": n24["hbm_GBps"],

@@ -90,6 +90,12 @@ uint32_t zkg_msm_default_window(size_t n);
  * Errors: n = 0 -> "an MSM needs at least one point"; c outside 9..=20 ->
  * "window size c must be in 9..=20, got {c}". */
 int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points, zkg_msm** out);
+/* No counterpart in the reference: a plan whose window, work-item size and sort-block size come
+ * from the tuning table (zkg_tune_msm / zkg_tune_load) for this GPU and size; untuned sizes use
+ * default_window(n) with automatic item and sort sizes. Results equal any other plan's. */
+int zkg_msm_create_tuned(zkg_device* dev, int group, size_t n, zkg_msm** out);
+/* Points per work item the plan uses (the automatic choice when created with item_points = 0). */
+uint32_t zkg_msm_item_points(const zkg_msm* plan);
 int zkg_msm_destroy(zkg_msm* plan);
 /* Msm::group (src/msm.rs:267-269). */
 int zkg_msm_group(const zkg_msm* plan);
@@ -153,6 +159,28 @@ int zkg_batch_add_msm(zkg_batch* batch, zkg_msm* plan, const zkg_buffer* bases, 
 /* Queue every job (asynchronous); the batch keeps its jobs and can be run again. */
 int zkg_batch_run(zkg_batch* batch);
 int zkg_batch_clear(zkg_batch* batch);
+
+/* ------------------------------------------------------------------ autotuning
+ * No counterpart in the reference. A process-wide table, keyed by GPU architecture and size
+ * (MSM: group and ceil(log2 n); NTT: log_n), holds the measured best MSM window c, work-item
+ * size and counting-sort block size, and the NTT tile; zkg_msm_create_tuned, zkg_ntt_create and
+ * the server read it. ZKGPU_TUNE_FILE names a table file to load at first use. The tuned choices
+ * only change how work is split: every result is bit-identical to an untuned plan's. */
+/* Times candidates on synthetic inputs of n points (coordinate search: c in default_window(n)
+ * +-2 within 9..=16, then item size x0.5 / x2, then sort block x2 / x4), records and returns the
+ * best (0 = automatic for item_points / sort_points) and its time in milliseconds. */
+int zkg_tune_msm(zkg_device* dev, int group, size_t n, uint32_t* c, uint32_t* item_points, uint32_t* sort_points,
+                 double* ms);
+/* Times the LDS tiles 2^10, 2^11, 2^12 for a forward NTT of 2^log_n; records and returns the best. */
+int zkg_tune_ntt(zkg_device* dev, uint32_t log_n, uint32_t* tile_log, double* ms);
+int zkg_tune_lookup_msm(const char* arch, int group, size_t n, int* found, uint32_t* c, uint32_t* item_points,
+                        uint32_t* sort_points);
+int zkg_tune_lookup_ntt(const char* arch, uint32_t log_n, int* found, uint32_t* tile_log);
+/* Text file: "msm <arch> <group> <ceil log2 n> <c> <item points> <sort points> <ms>" and
+ * "ntt <arch> <log_n> <tile log2> <ms>" lines; '#' starts a comment line. load merges. */
+int zkg_tune_load(const char* path);
+int zkg_tune_save(const char* path);
+int zkg_tune_clear(void);
 
 /* ------------------------------------------------------------------ proof-job server
  * No counterpart in the reference. A device-resident request loop with continuous batching:

@@ -5,6 +5,7 @@
 
 #include <string.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -259,6 +260,7 @@ int zkg_xyzz_to_affine_g2(const uint32_t* w, uint32_t* out, int* is_finite) {
 // ------------------------------------------------------------------ MSM
 
 #include "msm.hpp"
+#include "tune.hpp"
 
 struct zkg_msm {
     zkg::MsmPlan* plan;
@@ -269,19 +271,40 @@ extern "C" {
 
 uint32_t zkg_msm_default_window(size_t n) { return zkg::msm_default_window(n); }
 
-int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points, zkg_msm** out) {
-    ZKG_REQUIRE(dev && out, "null argument");
-    ZKG_GUARD_BEGIN
-    zkg::MsmPlan* p = new zkg::MsmPlan(dev->d, group, n, c, item_points);
-    zkg::Buffer* rb = new zkg::Buffer();
-    rb->dev = dev->d;
+static zkg_msm* wrap_plan(zkg::Device* d, std::unique_ptr<zkg::MsmPlan> p) {
+    std::unique_ptr<zkg::Buffer> rb(new zkg::Buffer());
+    rb->dev = d;
     rb->ptr = p->result;
     rb->bytes = 4 * (size_t)p->xw;
     rb->owned = false;
-    *out = new zkg_msm{p, new zkg_buffer{rb}};
+    std::unique_ptr<zkg_buffer> rh(new zkg_buffer{rb.get()});
+    zkg_msm* m = new zkg_msm{p.get(), rh.get()};
+    p.release();
+    rb.release();
+    rh.release();
+    return m;
+}
+
+int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points, zkg_msm** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    *out = wrap_plan(dev->d, std::unique_ptr<zkg::MsmPlan>(new zkg::MsmPlan(dev->d, group, n, c, item_points)));
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
+
+int zkg_msm_create_tuned(zkg_device* dev, int group, size_t n, zkg_msm** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    if (n < 1) return zkg::fail(zkg::ZKG_ERR_INVALID, "an MSM needs at least one point");
+    const zkg::MsmTune t = zkg::msm_config(dev->d, group, n);
+    *out = wrap_plan(dev->d, std::unique_ptr<zkg::MsmPlan>(
+                                 new zkg::MsmPlan(dev->d, group, n, t.c, t.item_points, t.sort_points)));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+uint32_t zkg_msm_item_points(const zkg_msm* m) { return m ? m->plan->CH : 0; }
 
 int zkg_msm_destroy(zkg_msm* m) {
     if (!m) return zkg::ZKG_OK;
@@ -572,6 +595,85 @@ int zkg_server_reset_stats(zkg_server* server) {
     ZKG_REQUIRE(server, "null argument");
     server->s->reset_stats();
     return zkg::ZKG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ autotuning
+
+#include "tune.hpp"
+
+extern "C" {
+
+int zkg_tune_msm(zkg_device* dev, int group, size_t n, uint32_t* c, uint32_t* item_points, uint32_t* sort_points,
+                 double* ms) {
+    ZKG_REQUIRE(dev, "null argument");
+    ZKG_GUARD_BEGIN
+    const zkg::MsmTune t = zkg::tune_msm(dev->d, group, n);
+    if (c) *c = t.c;
+    if (item_points) *item_points = t.item_points;
+    if (sort_points) *sort_points = t.sort_points;
+    if (ms) *ms = t.ms;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_tune_ntt(zkg_device* dev, uint32_t log_n, uint32_t* tile_log, double* ms) {
+    ZKG_REQUIRE(dev, "null argument");
+    ZKG_GUARD_BEGIN
+    const zkg::NttTune t = zkg::tune_ntt(dev->d, log_n);
+    if (tile_log) *tile_log = (uint32_t)t.tile_log;
+    if (ms) *ms = t.ms;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_tune_lookup_msm(const char* arch, int group, size_t n, int* found, uint32_t* c, uint32_t* item_points,
+                        uint32_t* sort_points) {
+    ZKG_REQUIRE(arch && found, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::MsmTune t;
+    *found = zkg::tune_lookup_msm(arch, group, zkg::tune_size_key(n), &t) ? 1 : 0;
+    if (*found) {
+        if (c) *c = t.c;
+        if (item_points) *item_points = t.item_points;
+        if (sort_points) *sort_points = t.sort_points;
+    }
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_tune_lookup_ntt(const char* arch, uint32_t log_n, int* found, uint32_t* tile_log) {
+    ZKG_REQUIRE(arch && found, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::NttTune t;
+    *found = zkg::tune_lookup_ntt(arch, log_n, &t) ? 1 : 0;
+    if (*found && tile_log) *tile_log = (uint32_t)t.tile_log;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_tune_load(const char* path) {
+    ZKG_REQUIRE(path, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::tune_load(path);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_tune_save(const char* path) {
+    ZKG_REQUIRE(path, "null argument");
+    ZKG_GUARD_BEGIN
+    zkg::tune_save(path);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_tune_clear(void) {
+    ZKG_GUARD_BEGIN
+    zkg::tune_clear();
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
 }
 
 }  // extern "C"

@@ -505,7 +505,12 @@ __global__ __launch_bounds__(256) void scan_add_kernel(const uint32_t* __restric
 
 // ---------------------------------------------------------------- plan
 
-MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_points) : dev(d), group(grp) {
+uint32_t msm_plan_item_points(size_t n, uint32_t c) {
+    return msm_auto_item_points(msm_glv_enabled(n) ? 2 * n : n, c);
+}
+
+MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_points, uint32_t sort_points)
+    : dev(d), group(grp) {
     if (npts < 1) throw Error(ZKG_ERR_INVALID, "an MSM needs at least one point");
     if (win < 9 || win > 20) throw Error(ZKG_ERR_INVALID, "window size c must be in 9..=20, got " + std::to_string(win));
     if (grp != 1 && grp != 2) throw Error(ZKG_ERR_INVALID, "group must be G1 (1) or G2 (2)");
@@ -529,7 +534,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     for (uint64_t m = ((uint64_t)nv + CH - 1) / CH; m > FINAL_GROUP; m = (m + SUM_GROUP - 1) / SUM_GROUP) ++merge_rounds;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
     lds_sort = c <= 16;
-    sort_pts = std::max<uint32_t>(B, 8192);
+    sort_pts = sort_points ? std::max<uint32_t>(sort_points, B) : std::max<uint32_t>(B, 8192);
     nblk = (nv + sort_pts - 1) / sort_pts;
     device_bind(d);
     if (lds_sort) {
@@ -580,7 +585,7 @@ MsmPlan::~MsmPlan() {
     item_info = nullptr;
 }
 
-size_t MsmPlan::workspace_bytes() const { return msm_workspace_bytes(group, n, c, CH); }
+size_t MsmPlan::workspace_bytes() const { return msm_workspace_bytes(group, n, c, CH, sort_pts); }
 
 bool msm_glv_enabled(size_t n) {
     // virtual point indices (2n of them) must fit the 31-bit index field of a sorted entry
@@ -589,7 +594,7 @@ bool msm_glv_enabled(size_t n) {
     return !(e && e[0] == '0');
 }
 
-size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points) {
+size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points) {
     const bool glv = msm_glv_enabled(n);
     const size_t nv = glv ? 2 * n : n;
     const uint32_t W = glv ? (128 + c - 1) / c : (255 + c - 1) / c, B = 1u << (c - 1);
@@ -597,7 +602,8 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
     const uint32_t xw = group == 1 ? 32 : 64, T = B / 4;
     const size_t S = (size_t)W * B, max_items = (size_t)W * ((nv + CH - 1) / CH + B);
     const bool lds = c <= 16;
-    const size_t P = std::max<uint32_t>(B, 8192), nblk = (nv + P - 1) / P;
+    const size_t P = sort_points ? std::max<uint32_t>(sort_points, B) : std::max<uint32_t>(B, 8192);
+    const size_t nblk = (nv + P - 1) / P;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
     return 4 * ((glv ? 4 * nv + n * xw / 2 : 0) + 4 * S + 2 + 2 * nb + (size_t)W * nv * (lds ? 2 : 1) +
                 (lds ? (size_t)W * nblk * B : 0) + max_items * (2 + (size_t)xw) + S * xw + (size_t)W * T * xw +

@@ -14,8 +14,10 @@ namespace zkg {
 
 uint32_t msm_default_window(size_t n);          // clamp(ilog2(max(n,1)) - 5, 11, 16)
 uint32_t msm_auto_item_points(size_t n, uint32_t c);
-// Device bytes a plan for (group, n, c, item_points) allocates (item_points 0 = automatic).
-size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points);
+// The work-item size a plan for (n, c) picks when none is given.
+uint32_t msm_plan_item_points(size_t n, uint32_t c);
+// Device bytes a plan for (group, n, c, item_points, sort_points) allocates (0 = automatic).
+size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0);
 // Whether plans use the GLV split (default on; ZKGPU_MSM_GLV=0 turns it off for comparisons).
 bool msm_glv_enabled(size_t n);
 
@@ -64,7 +66,8 @@ struct MsmPlan {
     uint32_t* result = nullptr;      // xw        final XYZZ point
     void* result_handle = nullptr;   // zkg_buffer* wrapping `result` (owned by the C ABI layer)
 
-    MsmPlan(Device* d, int group, size_t n, uint32_t c, uint32_t item_points);
+    // item_points / sort_points 0 = automatic (a quarter of the mean bucket / max(2^(c-1), 8192))
+    MsmPlan(Device* d, int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0);
     ~MsmPlan();
     // Queue the MSM on `stream`. With `stage_ms` set, wait around every stage and record
     // (stage name, milliseconds) as the reference's Msm::profile does.

@@ -20,6 +20,7 @@
 // w^-i twiddles and multiplies by n^-1 in the last pass's store.
 #include "field.hpp"
 #include "ntt.hpp"
+#include "tune.hpp"
 #include "host_math.hpp"
 
 #include <algorithm>
@@ -169,9 +170,19 @@ __global__ void ntt_twiddle_kernel(const uint32_t* __restrict__ hi, const uint32
 
 // ---------------------------------------------------------------- plan
 
+// Measured on MI355X (tuning runs): 2^10-element tiles are fastest from 2^21 up (more, smaller
+// workgroups at 91 VGPRs), 2^12 at 2^20 (one pass fewer), 2^11 below.
 int ntt_default_tile_log(uint32_t log_n) {
-    (void)log_n;
+    if (log_n >= 21) return 10;
+    if (log_n == 20) return 12;
     return 11;
+}
+
+// The tile a plan uses when none is given: the tuning table's, else the default.
+static int ntt_tile_for(const Device* d, uint32_t log_n) {
+    NttTune t;
+    if (d && tune_lookup_ntt(d->name, log_n, &t)) return t.tile_log;
+    return ntt_default_tile_log(log_n);
 }
 
 static void launch_pass(int tile_log, const NttPassArgs& a, unsigned wgs, hipStream_t stream) {
@@ -219,7 +230,7 @@ static void build_twiddles(uint32_t* out, uint32_t log_n, const host::U256& root
 
 NttPlan::NttPlan(Device* d, uint32_t ln, NttShape shape, bool with_scratch) : dev(d), log_n(ln) {
     if (ln < 10 || ln > 27) throw Error(ZKG_ERR_INVALID, "log_n must be in 10..=27");
-    tile_log = shape.tile_log ? shape.tile_log : ntt_default_tile_log(ln);
+    tile_log = shape.tile_log ? shape.tile_log : ntt_tile_for(d, ln);
     const size_t n = (size_t)1 << ln;
     device_bind(d);
     try {
@@ -295,8 +306,6 @@ void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream) const {
 
 // ---------------------------------------------------------------- batch
 
-static constexpr int BATCH_TILE_LOG = 11;
-
 NttBatch::NttBatch(Device* d) : dev_(d) {
     device_bind(d);
     ZKG_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
@@ -311,20 +320,22 @@ NttBatch::~NttBatch() {
     table_.release();
 }
 
-int NttBatch::tile_log() { return BATCH_TILE_LOG; }
+int NttBatch::tile_for(const Device* d, uint32_t log_n) { return ntt_tile_for(d, log_n); }
 
-const NttPlan& NttBatch::plan(uint32_t log_n) {
+const NttPlan& NttBatch::plan(uint32_t log_n, int tile) {
     auto it = plans_.find(log_n);
-    if (it == plans_.end()) {
+    if (it == plans_.end() || it->second->tile_log != tile) {
         NttShape shape;
-        shape.tile_log = BATCH_TILE_LOG;
-        it = plans_.emplace(log_n, std::make_shared<NttPlan>(dev_, log_n, shape, false)).first;
+        shape.tile_log = tile;
+        auto p = std::make_shared<NttPlan>(dev_, log_n, shape, false);
+        if (it == plans_.end()) it = plans_.emplace(log_n, p).first;
+        else it->second = p;
     }
     return *it->second;
 }
 
 size_t NttBatch::add(uint32_t log_n, uint32_t* data, bool inverse) {
-    plan(log_n);  // validates the size and builds the twiddles once per size
+    if (log_n < 10 || log_n > 27) throw Error(ZKG_ERR_INVALID, "log_n must be in 10..=27");
     jobs_.push_back(Job{log_n, data, inverse});
     return jobs_.size() - 1;
 }
@@ -334,12 +345,17 @@ void NttBatch::clear() { jobs_.clear(); }
 void NttBatch::run(hipStream_t stream) {
     if (jobs_.empty()) return;
     device_bind(dev_);
+    // one kernel per pass phase serves every job, so one tile for the batch: the best tile of
+    // its largest transform (the tuning table's, else the default rule)
+    uint32_t top = 0;
+    for (const Job& j : jobs_) top = std::max(top, j.log_n);
+    const int tile = tile_for(dev_, top);
     size_t scratch_elems = 0, max_passes = 0;
     std::vector<size_t> offs(jobs_.size());
     for (size_t j = 0; j < jobs_.size(); ++j) {
         offs[j] = scratch_elems;
         scratch_elems += (size_t)1 << jobs_[j].log_n;
-        max_passes = std::max(max_passes, plans_.at(jobs_[j].log_n)->passes.size());
+        max_passes = std::max(max_passes, plan(jobs_[j].log_n, tile).passes.size());
     }
     // the previous run's upload must be done before the staging memory is rewritten
     ZKG_HIP(hipEventSynchronize(staged_));
@@ -380,8 +396,21 @@ void NttBatch::run(hipStream_t stream) {
         if (!phase_jobs[k]) continue;
         const NttPassArgs* args = static_cast<const NttPassArgs*>(table_.ptr) + k * per_phase;
         const uint32_t* starts = reinterpret_cast<const uint32_t*>(args + nj);
-        hipLaunchKernelGGL((ntt_pass_batch_kernel<BATCH_TILE_LOG, 512>), dim3(phase_wgs[k]), dim3(512), 0, stream, args,
-                           starts, phase_jobs[k]);
+        switch (tile) {
+            case 10:
+                hipLaunchKernelGGL((ntt_pass_batch_kernel<10, 256>), dim3(phase_wgs[k]), dim3(256), 0, stream, args,
+                                   starts, phase_jobs[k]);
+                break;
+            case 11:
+                hipLaunchKernelGGL((ntt_pass_batch_kernel<11, 512>), dim3(phase_wgs[k]), dim3(512), 0, stream, args,
+                                   starts, phase_jobs[k]);
+                break;
+            case 12:
+                hipLaunchKernelGGL((ntt_pass_batch_kernel<12, 1024>), dim3(phase_wgs[k]), dim3(1024), 0, stream, args,
+                                   starts, phase_jobs[k]);
+                break;
+            default: throw Error(ZKG_ERR_INTERNAL, "unsupported NTT tile size");
+        }
         ZKG_HIP(hipGetLastError());
     }
 }

@@ -68,10 +68,12 @@ class NttBatch {
     void clear();
     size_t size() const { return jobs_.size(); }
     void run(hipStream_t stream);
-    const NttPlan& plan(uint32_t log_n);
-    // Use an existing twiddle plan (built with the batch tile size) for its size.
+    // The plan (twiddles and pass structure) of one size at one tile.
+    const NttPlan& plan(uint32_t log_n, int tile);
+    // Use an existing twiddle plan for its size (rebuilt if a run needs another tile).
     void share_plan(const std::shared_ptr<NttPlan>& p) { plans_[p->log_n] = p; }
-    static int tile_log();
+    // The tile a batch whose largest transform is 2^log_n runs with.
+    static int tile_for(const Device* d, uint32_t log_n);
 
   private:
     struct Job {

@@ -1,5 +1,6 @@
 // Proof-job server (see serve.hpp).
 #include "serve.hpp"
+#include "tune.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -33,13 +34,12 @@ Server::~Server() {
         if (j.second->ready) (void)hipEventDestroy(j.second->ready);
 }
 
-uint32_t Server::window_for(uint32_t log_n) { return msm_default_window((size_t)1 << log_n); }
-
 size_t Server::slot_bytes(uint32_t log_n) {
     const size_t n = (size_t)1 << log_n;
-    const uint32_t c = window_for(log_n);
-    return 3 * msm_workspace_bytes(1, n, c, 0) + msm_workspace_bytes(1, n - 1, c, 0) + msm_workspace_bytes(2, n, c, 0) +
-           (size_t)SERVE_POLYS * 32 * n;
+    const MsmTune g1 = msm_config(dev_, 1, n), g2 = msm_config(dev_, 2, n);
+    return 3 * msm_workspace_bytes(1, n, g1.c, g1.item_points, g1.sort_points) +
+           msm_workspace_bytes(1, n - 1, g1.c, g1.item_points, g1.sort_points) +
+           msm_workspace_bytes(2, n, g2.c, g2.item_points, g2.sort_points) + (size_t)SERVE_POLYS * 32 * n;
 }
 
 int Server::add_key(uint32_t log_n, const uint32_t* const g1[4], const uint32_t* g2, size_t key_bytes) {
@@ -141,15 +141,16 @@ std::unique_ptr<Server::Slot> Server::make_slot(uint32_t log_n) {
     std::unique_ptr<Slot> s(new Slot());
     s->log_n = log_n;
     const size_t n = (size_t)1 << log_n;
-    const uint32_t c = window_for(log_n);
-    for (int k = 0; k < 3; ++k) s->plans[k].reset(new MsmPlan(dev_, 1, n, c, 0));
-    s->plans[3].reset(new MsmPlan(dev_, 1, n - 1, c, 0));
-    s->plans[4].reset(new MsmPlan(dev_, 2, n, c, 0));
+    // window, item and sort sizes from the tuning table (default_window when untuned)
+    const MsmTune g1 = msm_config(dev_, 1, n), g2 = msm_config(dev_, 2, n);
+    for (int k = 0; k < 3; ++k) s->plans[k].reset(new MsmPlan(dev_, 1, n, g1.c, g1.item_points, g1.sort_points));
+    s->plans[3].reset(new MsmPlan(dev_, 1, n - 1, g1.c, g1.item_points, g1.sort_points));
+    s->plans[4].reset(new MsmPlan(dev_, 2, n, g2.c, g2.item_points, g2.sort_points));
     s->ntt.reset(new NttBatch(dev_));
     auto tw = twiddles_.find(log_n);
     if (tw == twiddles_.end()) {
         NttShape shape;
-        shape.tile_log = NttBatch::tile_log();
+        shape.tile_log = NttBatch::tile_for(dev_, log_n);
         tw = twiddles_.emplace(log_n, std::make_shared<NttPlan>(dev_, log_n, shape, false)).first;
     }
     s->ntt->share_plan(tw->second);

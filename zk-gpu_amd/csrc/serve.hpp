@@ -37,7 +37,7 @@ struct ServerStats {
 class Server {
   public:
     // mem_budget: HBM bytes the server may hold (keys + workspaces); max_inflight: jobs on the
-    // device at once. Every MSM uses default_window(n) for its key size.
+    // device at once. MSMs use the tuned window for their size (default_window(n) untuned).
     Server(Device* d, size_t mem_budget, uint32_t max_inflight);
     ~Server();
 
@@ -90,7 +90,6 @@ class Server {
     Slot* acquire_slot(uint32_t log_n);
     std::unique_ptr<Slot> make_slot(uint32_t log_n);
     void destroy_slot(Slot* s);
-    uint32_t window_for(uint32_t log_n);
 
     Device* dev_;
     size_t budget_, key_bytes_ = 0, slot_bytes_total_ = 0, peak_bytes_ = 0;

@@ -62,6 +62,43 @@ extern template void launch_generator_multiples<Fq2>(const uint32_t*, uint32_t*,
 
 }  // namespace zkg
 
+namespace zkg {
+
+// Synthetic MSM inputs for tuning and benchmarks: bases k_i * G with k_i < 2^253 from a seeded
+// xorshift64* stream, converted in place to the library's Montgomery form, and scalars below
+// 2^252 from the same stream. `bases` holds 16 * group * n words, `scalars` 8 * n.
+void synthetic_msm_inputs(Device* d, int group, size_t n, uint64_t seed, uint32_t* bases, uint32_t* scalars) {
+    device_bind(d);
+    uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&x]() {
+        x ^= x >> 12;
+        x ^= x << 25;
+        x ^= x >> 27;
+        return (uint32_t)((x * 0x2545F4914F6CDD1Dull) >> 32);
+    };
+    std::vector<uint32_t> host(8 * n);
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 8; ++k) host[8 * i + k] = next();
+        host[8 * i + 7] &= 0x1fffffffu;
+    }
+    DeviceArray<uint32_t> ks(8 * n);
+    hipStream_t s = d->stream;
+    ZKG_HIP(hipMemcpyAsync(ks.ptr, host.data(), 32 * n, hipMemcpyHostToDevice, s));
+    if (group == 1) launch_generator_multiples<Fq>(ks.ptr, bases, n, s);
+    else launch_generator_multiples<Fq2>(ks.ptr, bases, n, s);
+    ZKG_HIP(hipGetLastError());
+    msm_convert_bases(bases, 2 * (size_t)group * n, s);
+    ZKG_HIP(hipStreamSynchronize(s));  // `host` is rewritten below
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 8; ++k) host[8 * i + k] = next();
+        host[8 * i + 7] &= 0x0fffffffu;
+    }
+    ZKG_HIP(hipMemcpyAsync(scalars, host.data(), 32 * n, hipMemcpyHostToDevice, s));
+    ZKG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace zkg
+
 extern "C" {
 
 int zkg_test_field_op(zkg_device* dev, int field, int op, const uint32_t* a, const uint32_t* b, uint32_t* out,

@@ -5,7 +5,7 @@ ntt) over libzkgpu.so's C ABI (include/zkgpu.h). All compute runs in hand-writte
 importing this package does not touch the GPU, and every call fails loudly if the library is
 missing.
 """
-from . import host
+from . import host, tune
 from ._lib import ZkgError
 from .gpu import Buffer, Gpu
 from .group import Group
@@ -14,4 +14,4 @@ from .ntt import Ntt
 from .batch import Batch
 from .serve import Server
 
-__all__ = ["Batch", "Buffer", "Gpu", "Group", "Msm", "Ntt", "Server", "ZkgError", "host", "xyzz_to_affine", "xyzz_to_affine_g2"]
+__all__ = ["Batch", "Buffer", "Gpu", "Group", "Msm", "Ntt", "Server", "ZkgError", "host", "tune", "xyzz_to_affine", "xyzz_to_affine_g2"]

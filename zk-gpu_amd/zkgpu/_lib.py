@@ -35,6 +35,8 @@ SIGNATURES = {
     "zkg_msm_default_window": (ctypes.c_uint32, [ctypes.c_size_t]),
     "zkg_msm_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                       c_void_pp]),
+    "zkg_msm_create_tuned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, c_void_pp]),
+    "zkg_msm_item_points": (ctypes.c_uint32, [ctypes.c_void_p]),
     "zkg_msm_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_msm_group": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_msm_window": (ctypes.c_uint32, [ctypes.c_void_p]),
@@ -61,6 +63,15 @@ SIGNATURES = {
     "zkg_batch_add_msm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_batch_run": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_batch_clear": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_tune_msm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, c_u32p, c_u32p, c_u32p,
+                                    ctypes.POINTER(ctypes.c_double)]),
+    "zkg_tune_ntt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.POINTER(ctypes.c_double)]),
+    "zkg_tune_lookup_msm": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
+                                           c_u32p, c_u32p, c_u32p]),
+    "zkg_tune_lookup_ntt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), c_u32p]),
+    "zkg_tune_load": (ctypes.c_int, [ctypes.c_char_p]),
+    "zkg_tune_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "zkg_tune_clear": (ctypes.c_int, []),
     "zkg_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, c_void_pp]),
     "zkg_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_server_add_key": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p * 4, ctypes.c_void_p,

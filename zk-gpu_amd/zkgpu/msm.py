@@ -18,10 +18,13 @@ class Msm:
     """Plan for MSMs of n points in one group with window c (src/msm.rs:95-180). Not re-entrant:
     the plan owns its workspace and the result buffer returned by `run`."""
 
-    def __init__(self, gpu, group: Group, n: int, c: int, item_points: Optional[int] = None):
+    def __init__(self, gpu, group: Group, n: int, c: int, item_points: Optional[int] = None, _tuned: bool = False):
         h = ctypes.c_void_p()
-        check(_lib.load().zkg_msm_create(gpu.handle, group.value, int(n), int(c), int(item_points or 0),
-                                         ctypes.byref(h)))
+        if _tuned:
+            check(_lib.load().zkg_msm_create_tuned(gpu.handle, group.value, int(n), ctypes.byref(h)))
+        else:
+            check(_lib.load().zkg_msm_create(gpu.handle, group.value, int(n), int(c), int(item_points or 0),
+                                             ctypes.byref(h)))
         self._h = h
         self._gpu = gpu
         self._group = group
@@ -42,6 +45,15 @@ class Msm:
     @staticmethod
     def for_group(gpu, group: Group, n: int, c: int, item_points: Optional[int] = None) -> "Msm":
         return Msm(gpu, group, n, c, item_points)
+
+    @staticmethod
+    def tuned(gpu, group: Group, n: int) -> "Msm":
+        """A plan with the tuning table's window, item and sort sizes for this GPU and size
+        (default_window(n) when the size is untuned). No counterpart in the reference."""
+        return Msm(gpu, group, n, 0, None, _tuned=True)
+
+    def item_points(self) -> int:
+        return int(_lib.load().zkg_msm_item_points(self._h))
 
     @staticmethod
     def upload_bases(gpu, plain_xy) -> Buffer:
