@@ -3,7 +3,10 @@
 // Algorithm (same mathematics as the reference's ntt.wgsl:34-74, re-laid out for CDNA4):
 // radix-2 decimation-in-frequency, log_n stages split into P passes of at most TILE_LOG stages.
 // A pass loads a tile of 2^TILE_LOG elements into LDS (up to 128 KB of the 160 KB per CU), runs
-// its stages there with one barrier per stage, and writes the tile back. Element (row, column)
+// its stages there two at a time (radix-4 rounds: each lane keeps four elements in registers for
+// two stages, so one LDS round trip and one barrier per two stages), and writes the tile back.
+// Elements stay in [0, 2r) between the first load and the last store (lazy reduction), which
+// removes the final subtraction from every butterfly's Montgomery product. Element (row, column)
 // of a tile sits at  ((c >> lm) << (lm + s)) + (c & (2^lm - 1)) + (row << lm)  for global column
 // c, so columns with consecutive c are adjacent in memory: loads/stores of a tile are runs of
 // whole 32-byte elements, coalesced across the wavefront.
@@ -38,6 +41,108 @@ __device__ __forceinline__ size_t tile_addr(uint32_t wg, uint32_t row, uint32_t 
     return ((c >> lm) << (lm + s)) + (c & ((1ull << lm) - 1)) + ((size_t)row << lm);
 }
 
+// ---- lazy reduction: between the first load and the last store, elements live in [0, 2r).
+// x + y < 4r and x + 2r - y < 4r both fit 256 bits (r < 2^254). A Montgomery product whose
+// first factor is < 4r and second (a twiddle or n^-1) < r keeps its CIOS running value below
+// a + r < 5r < 2^256 and ends below a b / 2^256 + r < 2r, so it needs no final subtraction.
+
+// word i of 2r
+ZKG_DEV uint32_t twice_r(int i) { return 2u * FrParams::mod(i) + (i ? FrParams::mod(i - 1) >> 31 : 0u); }
+
+// (x + y) reduced to [0, 2r)
+ZKG_DEV Fr add_lazy(const Fr& x, const Fr& y) {
+    Fr s, d;
+    unsigned c = 0, c2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s.w[i] = __builtin_addc(x.w[i], y.w[i], c, &c2);
+        c = c2;
+    }
+    unsigned b = 0, b2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        d.w[i] = __builtin_subc(s.w[i], twice_r(i), b, &b2);
+        b = b2;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s.w[i] = b ? s.w[i] : d.w[i];
+    return s;
+}
+
+// x + 2r - y in (0, 4r)
+ZKG_DEV Fr sub_lazy(const Fr& x, const Fr& y) {
+    Fr s;
+    unsigned c = 0, c2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s.w[i] = __builtin_addc(x.w[i], twice_r(i), c, &c2);
+        c = c2;
+    }
+    unsigned b = 0, b2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s.w[i] = __builtin_subc(s.w[i], y.w[i], b, &b2);
+        b = b2;
+    }
+    return s;
+}
+
+// Montgomery a * b * 2^-256 for a < 4r, b < r: result < 2r (no final subtraction)
+ZKG_DEV Fr mul_lazy(const Fr& a, const Fr& b) {
+    uint32_t t[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint32_t lo[8], hi[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t s = (uint64_t)a.w[j] * b.w[i] + t[j];
+            lo[j] = (uint32_t)s;
+            hi[j] = (uint32_t)(s >> 32);
+        }
+        unsigned c = 0, c2;
+        t[0] = lo[0];
+#pragma unroll
+        for (int j = 1; j < 8; ++j) {
+            t[j] = __builtin_addc(lo[j], hi[j - 1], c, &c2);
+            c = c2;
+        }
+        const uint32_t top = hi[7] + c;
+        const uint32_t m = t[0] * FrParams::INV;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t s = (uint64_t)m * FrParams::mod(j) + t[j];
+            lo[j] = (uint32_t)s;
+            hi[j] = (uint32_t)(s >> 32);
+        }
+        c = 0;
+#pragma unroll
+        for (int j = 1; j < 8; ++j) {
+            t[j - 1] = __builtin_addc(lo[j], hi[j - 1], c, &c2);
+            c = c2;
+        }
+        t[7] = top + hi[7] + c;
+    }
+    Fr r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r.w[j] = t[j];
+    return r;
+}
+
+ZKG_DEV Fr tile_get(const uint4* tile, uint32_t slot) {
+    const uint4 x0 = tile[2 * slot], x1 = tile[2 * slot + 1];
+    Fr x;
+    x.w[0] = x0.x; x.w[1] = x0.y; x.w[2] = x0.z; x.w[3] = x0.w;
+    x.w[4] = x1.x; x.w[5] = x1.y; x.w[6] = x1.z; x.w[7] = x1.w;
+    return x;
+}
+
+ZKG_DEV void tile_put(uint4* tile, uint32_t slot, const Fr& x) {
+    tile[2 * slot] = make_uint4(x.w[0], x.w[1], x.w[2], x.w[3]);
+    tile[2 * slot + 1] = make_uint4(x.w[4], x.w[5], x.w[6], x.w[7]);
+}
+
 // One pass on workgroup `wg`'s tile; `tile` is the workgroup's LDS (2 uint4 per element).
 template <int TILE_LOG, int THREADS>
 __device__ __forceinline__ void ntt_pass_body(const NttPassArgs& a, const uint32_t wg, uint4* __restrict__ tile) {
@@ -69,31 +174,54 @@ __device__ __forceinline__ void ntt_pass_body(const NttPassArgs& a, const uint32
     }
     __syncthreads();
 
-    // ---- s radix-2 DIF stages in LDS
+    // ---- s DIF stages in LDS: pairs of stages as radix-4 rounds (four elements per lane, two
+    // stages in registers, one barrier), then a radix-2 round when s is odd. Stage st pairs rows
+    // that differ in bit lh = s-1-st of the row index; its twiddle exponent is
+    // ((row mod 2^lh) * 2^lm + j0) * 2^(L-1-lh-lm), with j0 the tile column's offset inside its
+    // group of 2^lm columns (0 in the last pass).
     const size_t lmmask = (1ull << lm) - 1;
-    for (uint32_t st = 0; st < s; ++st) {
+    // 1024-lane tiles stay within 128 VGPRs with radix-2 rounds only
+    constexpr bool RADIX4 = THREADS <= 512;
+    uint32_t st = 0;
+    for (; RADIX4 && st + 1 < s; st += 2) {
+        const uint32_t lh1 = s - 1 - st, lh2 = lh1 - 1;
+        for (uint32_t q = threadIdx.x; q < count / 4; q += THREADS) {
+            const uint32_t col = q & gmask, r = q >> lg;
+            const uint32_t lo = r & ((1u << lh2) - 1), hi = r >> lh2;
+            const uint32_t row00 = (hi << (lh1 + 1)) + lo;
+            const uint32_t s00 = (row00 << lg) + col, s01 = s00 + (1u << (lh2 + lg));
+            const uint32_t s10 = s00 + (1u << (lh1 + lg)), s11 = s10 + (1u << (lh2 + lg));
+            const size_t j0 = a.last ? 0 : ((((size_t)wg << lg) + col) & lmmask);
+            const size_t e1a = (((size_t)lo << lm) + j0) << (L - 1 - lh1 - lm);
+            const size_t e1b = ((((size_t)lo + (1u << lh2)) << lm) + j0) << (L - 1 - lh1 - lm);
+            const size_t e2 = (((size_t)lo << lm) + j0) << (L - 1 - lh2 - lm);
+            const Fr t1a = fp_load<FrParams>(a.tw + 8 * e1a), t1b = fp_load<FrParams>(a.tw + 8 * e1b);
+            const Fr t2 = fp_load<FrParams>(a.tw + 8 * e2);
+            const Fr x00 = tile_get(tile, s00), x01 = tile_get(tile, s01);
+            const Fr x10 = tile_get(tile, s10), x11 = tile_get(tile, s11);
+            // stage st: (x0b, x1b) for b = 0, 1
+            const Fr a0 = add_lazy(x00, x10), a1 = mul_lazy(sub_lazy(x00, x10), t1a);
+            const Fr b0 = add_lazy(x01, x11), b1 = mul_lazy(sub_lazy(x01, x11), t1b);
+            // stage st + 1: (a0, b0) -> rows 00, 01 and (a1, b1) -> rows 10, 11
+            tile_put(tile, s00, add_lazy(a0, b0));
+            tile_put(tile, s01, mul_lazy(sub_lazy(a0, b0), t2));
+            tile_put(tile, s10, add_lazy(a1, b1));
+            tile_put(tile, s11, mul_lazy(sub_lazy(a1, b1), t2));
+        }
+        __syncthreads();
+    }
+    for (; st < s; ++st) {  // radix-2 rounds
         const uint32_t lh = s - 1 - st;
         for (uint32_t q = threadIdx.x; q < count / 2; q += THREADS) {
             const uint32_t col = q & gmask, r = q >> lg;
             const uint32_t r_lo = r & ((1u << lh) - 1);
-            const uint32_t row_a = ((r >> lh) << (lh + 1)) + r_lo;
-            const uint32_t sa = (row_a << lg) + col, sb = sa + (1u << (lh + lg));
+            const uint32_t sa = ((((r >> lh) << (lh + 1)) + r_lo) << lg) + col, sb = sa + (1u << (lh + lg));
             const size_t j0 = a.last ? 0 : ((((size_t)wg << lg) + col) & lmmask);
             const size_t e = (((size_t)r_lo << lm) + j0) << (L - 1 - lh - lm);
-            Fr x, y;
-            const uint4 x0 = tile[2 * sa], x1 = tile[2 * sa + 1];
-            const uint4 y0 = tile[2 * sb], y1 = tile[2 * sb + 1];
-            x.w[0] = x0.x; x.w[1] = x0.y; x.w[2] = x0.z; x.w[3] = x0.w;
-            x.w[4] = x1.x; x.w[5] = x1.y; x.w[6] = x1.z; x.w[7] = x1.w;
-            y.w[0] = y0.x; y.w[1] = y0.y; y.w[2] = y0.z; y.w[3] = y0.w;
-            y.w[4] = y1.x; y.w[5] = y1.y; y.w[6] = y1.z; y.w[7] = y1.w;
             const Fr t = fp_load<FrParams>(a.tw + 8 * e);
-            const Fr u = x + y;
-            const Fr v = (x - y) * t;
-            tile[2 * sa] = make_uint4(u.w[0], u.w[1], u.w[2], u.w[3]);
-            tile[2 * sa + 1] = make_uint4(u.w[4], u.w[5], u.w[6], u.w[7]);
-            tile[2 * sb] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
-            tile[2 * sb + 1] = make_uint4(v.w[4], v.w[5], v.w[6], v.w[7]);
+            const Fr x = tile_get(tile, sa), y = tile_get(tile, sb);
+            tile_put(tile, sa, add_lazy(x, y));
+            tile_put(tile, sb, mul_lazy(sub_lazy(x, y), t));
         }
         __syncthreads();
     }
@@ -107,19 +235,13 @@ __device__ __forceinline__ void ntt_pass_body(const NttPassArgs& a, const uint32
             const uint32_t col = e & gmask, orow = e >> lg;
             const uint32_t row = bitrev_bits(orow, s);
             const uint32_t slot = (row << lg) + col;
-            uint4 v0 = tile[2 * slot], v1 = tile[2 * slot + 1];
-            if (a.scale_flag) {
-                Fr v;
-                v.w[0] = v0.x; v.w[1] = v0.y; v.w[2] = v0.z; v.w[3] = v0.w;
-                v.w[4] = v1.x; v.w[5] = v1.y; v.w[6] = v1.z; v.w[7] = v1.w;
-                v = v * scale;
-                v0 = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
-                v1 = make_uint4(v.w[4], v.w[5], v.w[6], v.w[7]);
-            }
+            Fr v = tile_get(tile, slot);  // < 2r
+            if (a.scale_flag) v = mul_lazy(v, scale);  // < 2r
+            fp_reduce_once<FrParams>(v.w);          // canonical output
             const size_t out = ((size_t)orow << (L - s)) + ((size_t)wg << lg) + col;
             uint4* p = reinterpret_cast<uint4*>(a.dst + 8 * out);
-            p[0] = v0;
-            p[1] = v1;
+            p[0] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+            p[1] = make_uint4(v.w[4], v.w[5], v.w[6], v.w[7]);
         }
     } else {
         const uint32_t gj = min(lg, lm);
@@ -170,13 +292,9 @@ __global__ void ntt_twiddle_kernel(const uint32_t* __restrict__ hi, const uint32
 
 // ---------------------------------------------------------------- plan
 
-// Measured on MI355X (tuning runs): 2^10-element tiles are fastest from 2^21 up (more, smaller
-// workgroups at 91 VGPRs), 2^12 at 2^20 (one pass fewer), 2^11 below.
-int ntt_default_tile_log(uint32_t log_n) {
-    if (log_n >= 21) return 10;
-    if (log_n == 20) return 12;
-    return 11;
-}
+// Measured on MI355X (tuning runs at 2^16, 2^20, 2^22, 2^24): 2^10-element tiles are fastest
+// (256-lane workgroups at 100 VGPRs), except at 2^20 where 2^12 saves a pass.
+int ntt_default_tile_log(uint32_t log_n) { return log_n == 20 ? 12 : 10; }
 
 // The tile a plan uses when none is given: the tuning table's, else the default.
 static int ntt_tile_for(const Device* d, uint32_t log_n) {
