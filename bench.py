@@ -32,6 +32,12 @@ METRIC = "G1 MSM 2^20 ms, NTT 2^24 HBM GB/s; batched vs one-call jobs/s, p99 ms"
 R = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 R_WORDS = np.array([(R >> (32 * i)) & 0xFFFFFFFF for i in range(8)], dtype=np.uint32)
 HBM_PEAK_GBPS = 8000.0
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz, full-rate 32-bit integer ops
+# rocprofv3 measurements of ntt_pass_kernel<10,256> on NTT 2^24 (profiles/ntt_pass_2^24_hbm_traffic.md,
+# profiles/valu_rate_msm_accumulate_and_ntt_pass.md): HBM bytes per launch (FETCH_SIZE scaled by the
+# copy-kernel calibration factor 2.0, WRITE_SIZE by 1.0) and VALU lane operations per launch.
+NTT24_TRAFFIC_BYTES_PER_LAUNCH = 1.591e9
+NTT24_VALU_LANE_OPS_PER_LAUNCH = 2.474e10
 
 
 def log(msg):
@@ -440,17 +446,13 @@ def serving_leg(gpu, rng, args):
     return out
 
 
-def copy_calibration():
-    """A plain copy kernel of known size (256 MiB read + 256 MiB write), so that rocprofv3's
-    FETCH_SIZE / WRITE_SIZE for it calibrate the same counters on the NTT kernel."""
-    import torch
-
-    src = torch.zeros(64 << 20, dtype=torch.int32, device="cuda:0")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        torch.add(src, 1, out=dst)  # an elementwise kernel (a plain copy_ would go to the DMA engine)
-    torch.cuda.synchronize()
-    log("copy calibration: 3 x elementwise kernel reading 256 MiB and writing 256 MiB")
+def copy_calibration(gpu):
+    """A plain copy kernel of known size (256 MiB read + 256 MiB write per launch, 3 launches), so
+    that rocprofv3's FETCH_SIZE / WRITE_SIZE for it calibrate the same counters on the NTT kernel."""
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().zkg_bench_copy(gpu.handle, 256 << 20, 3, ctypes.byref(ms)))
+    log(f"copy calibration: copy_kernel 256 MiB -> 256 MiB x 3, {ms.value:.3f} ms per launch "
+        f"({2 * 256 * 2 ** 20 / (ms.value * 1e-3) / 1e9:.0f} GB/s)")
 
 
 def ntt_only(gpu, args):
@@ -479,7 +481,7 @@ def main():
     ap.add_argument("--workload", choices=["all", "ntt24", "msm", "batch", "serve"], default="all",
                     help="ntt24/msm restrict the run to one kernel family (for rocprofv3 counter passes)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 256 MiB device-to-device copy kernel (reference for HBM counters)")
+                    help="also run the library's plain 256 MiB copy kernel (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -490,7 +492,7 @@ def main():
     info = gpu.info()
     log(f"device {info}")
     if args.copy_calibration:
-        copy_calibration()
+        copy_calibration(gpu)
     if args.workload == "ntt24":
         ntt_only(gpu, args)
         return
@@ -637,13 +639,22 @@ def main():
             "device": info,
         },
         "roofline": {
-            "kernel": "ntt_pass_kernel (NTT 2^24, %d passes)" % n24["passes"],
+            "kernel": "ntt_pass_kernel<10,256> (NTT 2^24 forward, %d launches)" % n24["passes"],
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": NTT24_TRAFFIC_BYTES_PER_LAUNCH,
+            "traffic_GBps": round(NTT24_TRAFFIC_BYTES_PER_LAUNCH / (per_pass_ms * 1e-3) / 1e9, 1),
+            "algorithmic_bytes_per_launch": 64 * (1 << 24) // n24["passes"],
+            "launch_ms": round(per_pass_ms, 4),
+            "valu": {"lane_ops_per_launch": NTT24_VALU_LANE_OPS_PER_LAUNCH,
+                     "achieved": round(NTT24_VALU_LANE_OPS_PER_LAUNCH / (per_pass_ms * 1e-3) / 1e12, 2),
+                     "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2), "unit": "T lane-ops/s",
+                     "frac": round(NTT24_VALU_LANE_OPS_PER_LAUNCH / (per_pass_ms * 1e-3) / VALU_PEAK_LANE_OPS, 4)},
+            "note": "the HBM fraction is the metric asked for; the kernel is integer-VALU-bound "
+                    "(Montgomery products), see valu.frac",
         },
         "cpu_baseline": cpu,
     }

@@ -226,6 +226,9 @@ int zkg_test_point_op(zkg_device* dev, int group, int op, const uint32_t* p, con
 /* Field-multiply throughput: `threads` lanes each run `chain` dependent Montgomery products,
  * `ways` independent chains per lane. Reports kernel milliseconds (HIP events). */
 int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t chain, uint32_t ways, double* ms);
+/* A plain device-to-device copy kernel of `bytes` (16 per lane), run `reps` times; reports the
+ * mean milliseconds per launch. The known-size reference for calibrating HBM byte counters. */
+int zkg_bench_copy(zkg_device* dev, size_t bytes, uint32_t reps, double* ms);
 /* The MSM's GLV scalar split on the device: out[0..4n) = k1 of each scalar, out[4n..8n) = k2,
  * each as |k| (127 bits, 4 words) with the sign in bit 127, such that s = k1 + k2 * lambda mod r
  * (lambda = 0xb3c4d79d41a917585bfc41088d8daaa78b17ea66b99c90dd). Synchronous; host arrays. */

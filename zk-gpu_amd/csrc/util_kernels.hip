@@ -4,6 +4,7 @@
 //
 // They stand in for the reference's WGSL point-op harness (tests/curve.rs:8-30) and field
 // benchmark (examples/field_bench.rs:10-97), using the same device code as the MSM and NTT.
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,16 @@ __global__ __launch_bounds__(256) void field_bench_kernel(uint32_t chain, uint32
 
 extern template void launch_point_op<Fq2>(int, const uint32_t*, const uint32_t*, uint32_t*, size_t, hipStream_t);
 extern template void launch_generator_multiples<Fq2>(const uint32_t*, uint32_t*, size_t, hipStream_t);
+
+}  // namespace zkg
+
+namespace zkg {
+
+// dst = src, 16 bytes per lane, grid-stride: the reference copy for HBM counter calibration
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 }  // namespace zkg
 
@@ -177,6 +188,33 @@ int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t c
     float f = 0;
     ZKG_HIP(hipEventElapsedTime(&f, e0, e1));
     *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_bench_copy(zkg_device* dev, size_t bytes, uint32_t reps, double* ms) {
+    if (!dev || !ms) return zkg::fail(zkg::ZKG_ERR_INVALID, "null argument");
+    if (bytes == 0 || bytes % 16 || reps == 0) return zkg::fail(zkg::ZKG_ERR_INVALID, "bytes a positive multiple of 16, reps >= 1");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    hipStream_t s = dev->d->stream;
+    zkg::DeviceArray<uint4> src(bytes / 16), dst(bytes / 16);
+    ZKG_HIP(hipMemsetAsync(src.ptr, 0, bytes, s));
+    hipEvent_t e0, e1;
+    ZKG_HIP(hipEventCreate(&e0));
+    ZKG_HIP(hipEventCreate(&e1));
+    const size_t n = bytes / 16;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    ZKG_HIP(hipEventRecord(e0, s));
+    for (uint32_t r = 0; r < reps; ++r) hipLaunchKernelGGL(zkg::copy_kernel, dim3(grid), dim3(256), 0, s, src.ptr, dst.ptr, n);
+    ZKG_HIP(hipGetLastError());
+    ZKG_HIP(hipEventRecord(e1, s));
+    ZKG_HIP(hipEventSynchronize(e1));
+    float f = 0;
+    ZKG_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return zkg::ZKG_OK;
