@@ -159,6 +159,12 @@ int zkg_batch_add_msm(zkg_batch* batch, zkg_msm* plan, const zkg_buffer* bases, 
 /* Queue every job (asynchronous); the batch keeps its jobs and can be run again. */
 int zkg_batch_run(zkg_batch* batch);
 int zkg_batch_clear(zkg_batch* batch);
+/* Capture the whole batch (every NTT phase launch and every MSM stage, on all of the batch's
+ * streams) into one hipGraph; zkg_batch_run then replays it with a single launch until the batch
+ * is changed (add / clear drop the graph). The graph refers to the jobs' buffers and plans: they
+ * must outlive it, and their contents are read at replay time. */
+int zkg_batch_capture(zkg_batch* batch);
+int zkg_batch_captured(const zkg_batch* batch);  /* 1 if a current graph exists */
 
 /* ------------------------------------------------------------------ autotuning
  * No counterpart in the reference. A process-wide table, keyed by GPU architecture and size

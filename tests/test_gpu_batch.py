@@ -91,3 +91,60 @@ def test_batch_rejects_a_plan_twice(gpu):
     batch.add_msm(plan, bb, sb)
     with pytest.raises(ZkgError, match="only once in a batch"):
         batch.add_msm(plan, bb, sb)
+
+
+def test_batch_graph_replays_match_eager_runs(gpu):
+    """A captured batch replayed several times, with eager work and synchronisation between the
+    replays (fresh inputs written, unrelated NTTs and MSMs run), gives the eager result each time."""
+    from zkgpu import Batch, Group, Msm, Ntt
+
+    rng = np.random.default_rng(520)
+    sizes = [10, 13, 11, 15, 12]
+    inputs = [uniform_fr_words(rng, 1 << k) for k in sizes]
+    bufs = [gpu.buffer(x) for x in inputs]
+    specs = [(1, 3000, 11), (2, 900, 10), (1, 5000, 13)]
+    jobs = []
+    for group, n, c in specs:
+        ks = uniform_fr_words(rng, n)
+        bases = O.multiples(group, O.generator_words(group), ks)
+        sc = witness_scalars(rng, n)
+        jobs.append((group, Msm.for_group(gpu, Group(group), n, c), Msm.upload_bases(gpu, bases), gpu.buffer(sc),
+                     O.msm(group, bases, sc)))
+    batch = Batch(gpu)
+    for j, (k, b) in enumerate(zip(sizes, bufs)):
+        batch.add_ntt(b, k, inverse=bool(j % 2))
+    for group, plan, bb, sb, _ in jobs:
+        batch.add_msm(plan, bb, sb)
+    expect_ntt = [O.ntt(x, k, inverse=bool(j % 2)) for j, (k, x) in enumerate(zip(sizes, inputs))]
+    batch.run()  # eager
+    gpu.wait()
+    for j, (k, b) in enumerate(zip(sizes, bufs)):
+        assert np.array_equal(gpu.read(b, 8 << k), expect_ntt[j]), ("eager", j)
+    batch.capture()
+    assert batch.captured()
+    side = gpu.buffer(uniform_fr_words(rng, 1 << 12))
+    side_plan = Ntt(gpu, 12)
+    other = Msm.new(gpu, 2000, 12)
+    other_bases = Msm.upload_bases(gpu, O.multiples(1, O.generator_words(1), uniform_fr_words(rng, 2000)))
+    other_sc = gpu.buffer(uniform_fr_words(rng, 2000))
+    for rep in range(3):
+        for b, x in zip(bufs, inputs):
+            gpu.write(b, x)
+        side_plan.forward(gpu, side)
+        other.run(gpu, other_bases, other_sc)
+        gpu.wait()
+        batch.run()  # graph replay
+        side_plan.inverse(gpu, side)
+        gpu.wait()
+        for j, (k, b) in enumerate(zip(sizes, bufs)):
+            assert np.array_equal(gpu.read(b, 8 << k), expect_ntt[j]), (rep, j)
+        for group, plan, bb, sb, exp in jobs:
+            assert np.array_equal(_affine(gpu, plan.result_buffer(gpu), group), exp), (rep, group)
+    # changing the batch drops the graph; the next run is eager again
+    extra = gpu.buffer(inputs[0])
+    batch.add_ntt(extra, sizes[0])
+    assert not batch.captured()
+    for b, x in zip(bufs, inputs):
+        gpu.write(b, x)
+    batch.run()
+    assert np.array_equal(gpu.read(extra, 8 << sizes[0]), O.ntt(inputs[0], sizes[0]))

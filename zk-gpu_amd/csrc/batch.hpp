@@ -20,7 +20,12 @@ class Batch {
     size_t add_msm(const MsmPlan* plan, const uint32_t* bases, const uint32_t* scalars);
     void clear();
     // Queue everything; work queued on the device stream afterwards runs after the whole batch.
+    // With a current captured graph, run() replays it with one launch.
     void run();
+    // Capture the whole batch (every kernel on every fork stream) into one hipGraph; later runs
+    // replay it until the batch changes. The NTT tables are prepared and uploaded first.
+    void capture();
+    bool captured() const { return exec_ != nullptr; }
     size_t ntt_jobs() const { return ntt_.size(); }
     size_t msm_jobs() const { return msm_.size(); }
 
@@ -36,7 +41,12 @@ class Batch {
     std::vector<hipStream_t> streams_;  // [0] NTTs, [1..] MSMs
     std::vector<hipEvent_t> done_;
     hipEvent_t fork_ = nullptr;
+    hipStream_t origin_ = nullptr;  // capture origin
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
     void ensure_streams(size_t k);
+    void enqueue(hipStream_t from);  // fork from `from`, queue every job, join back into `from`
+    void drop_graph();
 };
 
 }  // namespace zkg

@@ -470,6 +470,16 @@ int zkg_batch_clear(zkg_batch* batch) {
     return zkg::ZKG_OK;
 }
 
+int zkg_batch_capture(zkg_batch* batch) {
+    ZKG_REQUIRE(batch, "null argument");
+    ZKG_GUARD_BEGIN
+    batch->b->capture();
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_batch_captured(const zkg_batch* batch) { return batch && batch->b->captured() ? 1 : 0; }
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ proof-job server

@@ -454,26 +454,31 @@ const NttPlan& NttBatch::plan(uint32_t log_n, int tile) {
 
 size_t NttBatch::add(uint32_t log_n, uint32_t* data, bool inverse) {
     if (log_n < 10 || log_n > 27) throw Error(ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    prepared_ = false;
     jobs_.push_back(Job{log_n, data, inverse});
     return jobs_.size() - 1;
 }
 
-void NttBatch::clear() { jobs_.clear(); }
+void NttBatch::clear() {
+    jobs_.clear();
+    prepared_ = false;
+}
 
-void NttBatch::run(hipStream_t stream) {
+void NttBatch::prepare(hipStream_t stream) {
+    prepared_ = false;
     if (jobs_.empty()) return;
     device_bind(dev_);
     // one kernel per pass phase serves every job, so one tile for the batch: the best tile of
     // its largest transform (the tuning table's, else the default rule)
     uint32_t top = 0;
     for (const Job& j : jobs_) top = std::max(top, j.log_n);
-    const int tile = tile_for(dev_, top);
+    tile_ = tile_for(dev_, top);
     size_t scratch_elems = 0, max_passes = 0;
     std::vector<size_t> offs(jobs_.size());
     for (size_t j = 0; j < jobs_.size(); ++j) {
         offs[j] = scratch_elems;
         scratch_elems += (size_t)1 << jobs_[j].log_n;
-        max_passes = std::max(max_passes, plan(jobs_[j].log_n, tile).passes.size());
+        max_passes = std::max(max_passes, plan(jobs_[j].log_n, tile_).passes.size());
     }
     // the previous run's upload must be done before the staging memory is rewritten
     ZKG_HIP(hipEventSynchronize(staged_));
@@ -481,8 +486,8 @@ void NttBatch::run(hipStream_t stream) {
     // table layout per phase: njobs NttPassArgs, then (njobs + 1) u32 workgroup starts (padded)
     const size_t nj = jobs_.size();
     const size_t start_slots = (4 * (nj + 1) + sizeof(NttPassArgs) - 1) / sizeof(NttPassArgs);
-    const size_t per_phase = nj + start_slots;
-    const size_t need = per_phase * max_passes;
+    per_phase_ = nj + start_slots;
+    const size_t need = per_phase_ * max_passes;
     if (need > staging_cap_) {
         if (staging_) ZKG_HIP(hipHostFree(staging_));
         staging_ = nullptr;
@@ -490,9 +495,10 @@ void NttBatch::run(hipStream_t stream) {
         staging_cap_ = need;
     }
     table_.ensure(need * sizeof(NttPassArgs));
-    std::vector<uint32_t> phase_jobs(max_passes, 0), phase_wgs(max_passes, 0);
+    phase_jobs_.assign(max_passes, 0);
+    phase_wgs_.assign(max_passes, 0);
     for (size_t k = 0; k < max_passes; ++k) {
-        NttPassArgs* args = staging_ + k * per_phase;
+        NttPassArgs* args = staging_ + k * per_phase_;
         uint32_t* starts = reinterpret_cast<uint32_t*>(args + nj);
         uint32_t cnt = 0, wgs = 0;
         for (size_t j = 0; j < nj; ++j) {
@@ -505,32 +511,44 @@ void NttBatch::run(hipStream_t stream) {
             ++cnt;
         }
         starts[cnt] = wgs;
-        phase_jobs[k] = cnt;
-        phase_wgs[k] = wgs;
+        phase_jobs_[k] = cnt;
+        phase_wgs_[k] = wgs;
     }
     ZKG_HIP(hipMemcpyAsync(table_.ptr, staging_, need * sizeof(NttPassArgs), hipMemcpyHostToDevice, stream));
     ZKG_HIP(hipEventRecord(staged_, stream));
-    for (size_t k = 0; k < max_passes; ++k) {
-        if (!phase_jobs[k]) continue;
-        const NttPassArgs* args = static_cast<const NttPassArgs*>(table_.ptr) + k * per_phase;
+    prepared_ = true;
+}
+
+void NttBatch::launch(hipStream_t stream) const {
+    if (jobs_.empty()) return;
+    if (!prepared_) throw Error(ZKG_ERR_INTERNAL, "NTT batch launched before it was prepared");
+    const size_t nj = jobs_.size();
+    for (size_t k = 0; k < phase_jobs_.size(); ++k) {
+        if (!phase_jobs_[k]) continue;
+        const NttPassArgs* args = static_cast<const NttPassArgs*>(table_.ptr) + k * per_phase_;
         const uint32_t* starts = reinterpret_cast<const uint32_t*>(args + nj);
-        switch (tile) {
+        switch (tile_) {
             case 10:
-                hipLaunchKernelGGL((ntt_pass_batch_kernel<10, 256>), dim3(phase_wgs[k]), dim3(256), 0, stream, args,
-                                   starts, phase_jobs[k]);
+                hipLaunchKernelGGL((ntt_pass_batch_kernel<10, 256>), dim3(phase_wgs_[k]), dim3(256), 0, stream, args,
+                                   starts, phase_jobs_[k]);
                 break;
             case 11:
-                hipLaunchKernelGGL((ntt_pass_batch_kernel<11, 512>), dim3(phase_wgs[k]), dim3(512), 0, stream, args,
-                                   starts, phase_jobs[k]);
+                hipLaunchKernelGGL((ntt_pass_batch_kernel<11, 512>), dim3(phase_wgs_[k]), dim3(512), 0, stream, args,
+                                   starts, phase_jobs_[k]);
                 break;
             case 12:
-                hipLaunchKernelGGL((ntt_pass_batch_kernel<12, 1024>), dim3(phase_wgs[k]), dim3(1024), 0, stream, args,
-                                   starts, phase_jobs[k]);
+                hipLaunchKernelGGL((ntt_pass_batch_kernel<12, 1024>), dim3(phase_wgs_[k]), dim3(1024), 0, stream, args,
+                                   starts, phase_jobs_[k]);
                 break;
             default: throw Error(ZKG_ERR_INTERNAL, "unsupported NTT tile size");
         }
         ZKG_HIP(hipGetLastError());
     }
+}
+
+void NttBatch::run(hipStream_t stream) {
+    prepare(stream);
+    launch(stream);
 }
 
 }  // namespace zkg

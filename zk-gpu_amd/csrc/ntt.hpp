@@ -67,7 +67,12 @@ class NttBatch {
     size_t add(uint32_t log_n, uint32_t* data, bool inverse);  // returns the job index
     void clear();
     size_t size() const { return jobs_.size(); }
-    void run(hipStream_t stream);
+    void run(hipStream_t stream);  // prepare + launch
+    // Build the pass tables, size the scratch and queue the table upload on `stream`
+    // (synchronises with the previous upload; allocates when the batch grows).
+    void prepare(hipStream_t stream);
+    // Only the kernel launches of a prepared batch (no allocation, copy or host wait: capturable).
+    void launch(hipStream_t stream) const;
     // The plan (twiddles and pass structure) of one size at one tile.
     const NttPlan& plan(uint32_t log_n, int tile);
     // Use an existing twiddle plan for its size (rebuilt if a run needs another tile).
@@ -89,6 +94,10 @@ class NttBatch {
     NttPassArgs* staging_ = nullptr;  // pinned host staging of the tables
     size_t staging_cap_ = 0;
     hipEvent_t staged_ = nullptr;     // the last upload from staging has completed
+    bool prepared_ = false;
+    int tile_ = 0;
+    size_t per_phase_ = 0;
+    std::vector<uint32_t> phase_jobs_, phase_wgs_;
 };
 
 int ntt_default_tile_log(uint32_t log_n);

@@ -28,6 +28,13 @@ class Batch:
     def run(self) -> None:
         check(_lib.load().zkg_batch_run(self._h))
 
+    def capture(self) -> None:
+        """Capture the batch into one hipGraph; run() then replays it with one launch."""
+        check(_lib.load().zkg_batch_capture(self._h))
+
+    def captured(self) -> bool:
+        return bool(_lib.load().zkg_batch_captured(self._h))
+
     def clear(self) -> None:
         check(_lib.load().zkg_batch_clear(self._h))
         self._keep = []
