@@ -600,6 +600,22 @@ def main():
             ntt24_in, ntt24_out = x, y
         del buf, plan
 
+    # ---------------- Groth16 quotient polynomial H = (A B - C) / (x^n - 1) at 2^22 (7 NTTs + pointwise)
+    h_res = None
+    if args.workload == "all":
+        from zkgpu import Groth16H
+
+        lh = 22
+        nh = 1 << lh
+        ha, hb = uniform_fr(rng, nh), uniform_fr(rng, nh)
+        hplan = Groth16H(gpu, lh)
+        hbufs = [gpu.buffer(ha), gpu.buffer(hb), gpu.buffer(uniform_fr(rng, nh))]
+        hplan.run(gpu, *hbufs)
+        h_ms, _ = time_calls(gpu, ev, lambda: hplan.run(gpu, *hbufs), max(3, args.steps // 4), 1)
+        h_res = {"log_n": lh, "ms": round(h_ms, 3), "ntts": 7}
+        log(f"Groth16 H 2^{lh}: {h_ms:.3f} ms")
+        del hplan, hbufs
+
     # ---------------- field-multiply throughput (the integer-VALU ceiling of both operators)
     ms = ctypes.c_double()
     threads, chain = 256 * 2048, 1000
@@ -654,6 +670,7 @@ def main():
             "g2_msm_2^18_ms": round(g2_ms, 4),
             "ntt": {f"2^{k}": v for k, v in ntt.items()},
             "ntt_2^24_hbm_GBps": n24["hbm_GBps"],
+            "groth16_h_2^22": h_res,
             "batched_ntt_64": batch_res,
             "groth16_msm_set_2^22": groth,
             "serving": serve,

@@ -166,6 +166,18 @@ int zkg_batch_clear(zkg_batch* batch);
 int zkg_batch_capture(zkg_batch* batch);
 int zkg_batch_captured(const zkg_batch* batch);  /* 1 if a current graph exists */
 
+/* ------------------------------------------------------------------ Groth16 quotient polynomial
+ * No counterpart in the reference (README.md:229-230 names it as the next step around the NTT).
+ * a, b, c hold the evaluations of A, B, C on the domain {w^i} of size n = 2^log_n (natural
+ * order, plain words); run leaves in a the n coefficients of H = (A B - C) / (x^n - 1), computed
+ * on the coset 5 H (3 inverse + 3 forward NTTs with the coset shift fused into the inverse's
+ * scaling, a pointwise step, 1 inverse NTT with the unshift fused); b and c are clobbered.
+ * When A B - C vanishes on the domain, H is the exact quotient (its top coefficient is 0). */
+typedef struct zkg_groth16_h zkg_groth16_h;
+int zkg_groth16_h_create(zkg_device* dev, uint32_t log_n, zkg_groth16_h** out);
+int zkg_groth16_h_destroy(zkg_groth16_h* plan);
+int zkg_groth16_h_run(zkg_groth16_h* plan, zkg_buffer* a, zkg_buffer* b, zkg_buffer* c);
+
 /* ------------------------------------------------------------------ autotuning
  * No counterpart in the reference. A process-wide table, keyed by GPU architecture and size
  * (MSM: group and ceil(log2 n); NTT: log_n), holds the measured best MSM window c, work-item

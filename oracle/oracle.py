@@ -214,6 +214,30 @@ def ntt(words, log_n: int, inverse: bool = False, threads: int | None = None) ->
     return data
 
 
+def groth16_h(a_words, b_words, c_words, log_n: int, g: int = 5) -> np.ndarray:
+    """Coefficients of H = (A B - C) / (x^n - 1) from evaluations of A, B, C on {w^i}, computed
+    on the coset g*{w^i}: inverse transforms, coefficients times g^i, forward transforms,
+    (a b - c) / (g^n - 1) pointwise, inverse transform, coefficients times g^-i. (The reference
+    has no such step; its README.md:229-230 names it as the work around the NTT.)"""
+    n = 1 << log_n
+    coeffs = [words_to_ints(ntt(w, log_n, inverse=True)) for w in (a_words, b_words, c_words)]
+    evals = []
+    for cs in coeffs:
+        gi, out = 1, []
+        for x in cs:
+            out.append(x * gi % R)
+            gi = gi * g % R
+        evals.append(words_to_ints(ntt(ints_to_words(out), log_n)))
+    zinv = pow((pow(g, n, R) - 1) % R, R - 2, R)
+    h = [(x * y - z) * zinv % R for x, y, z in zip(*evals)]
+    hc = words_to_ints(ntt(ints_to_words(h), log_n, inverse=True))
+    ginv, gi, out = pow(g, R - 2, R), 1, []
+    for x in hc:
+        out.append(x * gi % R)
+        gi = gi * ginv % R
+    return ints_to_words(out)
+
+
 def poly_eval(words, x: int) -> int:
     coeffs = _u32(words)
     out = np.zeros(8, dtype=np.uint32)

@@ -216,3 +216,26 @@ def test_golden_rejects_dropped_digit_carry():
     bases = c["bases"].reshape(-1, 16)
     got = O.msm_naive(1, bases, O.ints_to_words(mutated))
     assert not np.array_equal(got, c["result"])
+
+
+def test_groth16_h_is_the_quotient_for_satisfied_systems():
+    """With c = a * b on the domain, A B - C vanishes there, so H is the exact quotient by
+    x^n - 1: H(t) (t^n - 1) = A(t) B(t) - C(t) at random points, and deg H <= n - 2. Any coset
+    gives the same H then; a different coset changes H for an unsatisfied system."""
+    rng = np.random.default_rng(77)
+    log_n = 10
+    n = 1 << log_n
+    a = uniform_fr_words(rng, n)
+    b = uniform_fr_words(rng, n)
+    c = O.ints_to_words([x * y % O.R for x, y in zip(O.words_to_ints(a), O.words_to_ints(b))])
+    h = O.groth16_h(a, b, c, log_n)
+    assert O.words_to_int(h[8 * (n - 1):]) == 0
+    ca, cb, cc = (O.ntt(w, log_n, inverse=True) for w in (a, b, c))
+    for t in (3, 123456789, O.R - 2):
+        lhs = O.poly_eval(h, t) * (pow(t, n, O.R) - 1) % O.R
+        rhs = (O.poly_eval(ca, t) * O.poly_eval(cb, t) - O.poly_eval(cc, t)) % O.R
+        assert lhs == rhs, t
+    assert np.array_equal(O.groth16_h(a, b, c, log_n, g=7), h)
+    c2 = c.copy()
+    c2[:8] = O.int_to_words((O.words_to_int(c[:8]) + 1) % O.R)  # one unsatisfied constraint
+    assert not np.array_equal(O.groth16_h(a, b, c2, log_n, g=7), O.groth16_h(a, b, c2, log_n))

@@ -687,3 +687,52 @@ int zkg_tune_clear(void) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ Groth16 quotient polynomial
+
+#include "h_poly.hpp"
+
+struct zkg_groth16_h {
+    zkg::GrothH* p;
+};
+
+extern "C" {
+
+int zkg_groth16_h_create(zkg_device* dev, uint32_t log_n, zkg_groth16_h** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    if (log_n < 10 || log_n > 27) return zkg::fail(zkg::ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    *out = new zkg_groth16_h{new zkg::GrothH(dev->d, log_n)};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_groth16_h_destroy(zkg_groth16_h* plan) {
+    if (!plan) return zkg::ZKG_OK;
+    (void)hipSetDevice(plan->p->dev->ordinal);
+    (void)hipStreamSynchronize(plan->p->dev->stream);
+    delete plan->p;
+    delete plan;
+    return zkg::ZKG_OK;
+}
+
+int zkg_groth16_h_run(zkg_groth16_h* plan, zkg_buffer* a, zkg_buffer* b, zkg_buffer* c) {
+    ZKG_REQUIRE(plan && a && b && c, "null argument");
+    const size_t need = (size_t)32 << plan->p->log_n;
+    for (zkg_buffer* x : {a, b, c})
+        if (x->b->bytes < need)
+            return zkg::fail(zkg::ZKG_ERR_SIZE, "data buffer holds " + std::to_string(x->b->bytes) + " bytes; a 2^" +
+                                                    std::to_string(plan->p->log_n) + " transform needs " +
+                                                    std::to_string(need));
+    if (a->b->ptr == b->b->ptr || a->b->ptr == c->b->ptr || b->b->ptr == c->b->ptr)
+        return zkg::fail(zkg::ZKG_ERR_INVALID, "a, b and c must be distinct buffers");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(plan->p->dev);
+    plan->p->run(static_cast<uint32_t*>(a->b->ptr), static_cast<uint32_t*>(b->b->ptr),
+                 static_cast<uint32_t*>(c->b->ptr), plan->p->dev->stream);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+}  // extern "C"

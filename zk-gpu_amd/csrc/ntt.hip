@@ -236,9 +236,10 @@ __device__ __forceinline__ void ntt_pass_body(const NttPassArgs& a, const uint32
             const uint32_t row = bitrev_bits(orow, s);
             const uint32_t slot = (row << lg) + col;
             Fr v = tile_get(tile, slot);  // < 2r
-            if (a.scale_flag) v = mul_lazy(v, scale);  // < 2r
-            fp_reduce_once<FrParams>(v.w);          // canonical output
             const size_t out = ((size_t)orow << (L - s)) + ((size_t)wg << lg) + col;
+            if (a.scale_tab) v = mul_lazy(v, fp_load<FrParams>(a.scale_tab + 8 * out));  // per-element scale
+            else if (a.scale_flag) v = mul_lazy(v, scale);                                // < 2r
+            fp_reduce_once<FrParams>(v.w);                                              // canonical output
             uint4* p = reinterpret_cast<uint4*>(a.dst + 8 * out);
             p[0] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
             p[1] = make_uint4(v.w[4], v.w[5], v.w[6], v.w[7]);
@@ -313,21 +314,24 @@ static void launch_pass(int tile_log, const NttPassArgs& a, unsigned wgs, hipStr
     ZKG_HIP(hipGetLastError());
 }
 
-static void build_twiddles(uint32_t* out, uint32_t log_n, const host::U256& root, hipStream_t stream) {
+// out[i] = first * base^i (Montgomery form), i < count: out[i] = hi[i >> lb] * lo[i & (2^lb - 1)]
+// from two ~sqrt(count) seed tables built on the host.
+static void build_geometric(uint32_t* out, size_t count, const host::U256& base, const host::U256& first,
+                            hipStream_t stream) {
     const host::Modulus& F = host::fr();
-    const size_t count = (size_t)1 << (log_n - 1);
-    const uint32_t lb = (log_n - 1 + 1) / 2;  // ceil((log_n - 1) / 2)
+    uint32_t lb = 0;
+    while (((size_t)1 << (2 * lb)) < count) ++lb;  // lo_len^2 >= count
     const size_t lo_len = (size_t)1 << lb;
     const size_t hi_len = (count + lo_len - 1) / lo_len;
     std::vector<uint32_t> lo(8 * lo_len), hi(8 * hi_len);
-    const host::U256 wm = F.to_mont(root);
+    const host::U256 bm = F.to_mont(base);
     host::U256 x = F.one;  // Montgomery 1
     for (size_t i = 0; i < lo_len; ++i) {
         host::to_words32(&lo[8 * i], x);
-        x = F.mont(x, wm);
+        x = F.mont(x, bm);
     }
-    const host::U256 step = x;  // w^lo_len (Montgomery)
-    x = F.one;
+    const host::U256 step = x;  // base^lo_len (Montgomery)
+    x = F.to_mont(first);
     for (size_t i = 0; i < hi_len; ++i) {
         host::to_words32(&hi[8 * i], x);
         x = F.mont(x, step);
@@ -337,13 +341,22 @@ static void build_twiddles(uint32_t* out, uint32_t log_n, const host::U256& root
     ZKG_HIP(hipMalloc(&dhi, hi.size() * 4));
     ZKG_HIP(hipMemcpyAsync(dlo, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, stream));
     ZKG_HIP(hipMemcpyAsync(dhi, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(ntt_twiddle_kernel, dim3(ceil_div(count, 256)), dim3(256), 0, stream, dhi, dlo, out, lb,
-                       count);
+    hipLaunchKernelGGL(ntt_twiddle_kernel, dim3(ceil_div(count, 256)), dim3(256), 0, stream, dhi, dlo, out, lb, count);
     ZKG_HIP(hipGetLastError());
     // The host vectors go out of scope: wait for the copies and the kernel before freeing.
     ZKG_HIP(hipStreamSynchronize(stream));
     ZKG_HIP(hipFree(dlo));
     ZKG_HIP(hipFree(dhi));
+}
+
+static void build_twiddles(uint32_t* out, uint32_t log_n, const host::U256& root, hipStream_t stream) {
+    host::U256 one = {{1, 0, 0, 0}};
+    build_geometric(out, (size_t)1 << (log_n - 1), root, one, stream);
+}
+
+void ntt_geometric_table(uint32_t* out, size_t count, const uint32_t base[8], const uint32_t first[8],
+                         hipStream_t stream) {
+    build_geometric(out, count, host::from_words32(base), host::from_words32(first), stream);
 }
 
 NttPlan::NttPlan(Device* d, uint32_t ln, NttShape shape, bool with_scratch) : dev(d), log_n(ln) {
@@ -392,7 +405,7 @@ NttPlan::~NttPlan() {
 
 size_t NttPlan::workspace_bytes() const { return (size_t)64 << log_n; }
 
-NttPassArgs NttPlan::pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inverse) const {
+NttPassArgs NttPlan::pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inverse, const uint32_t* scale_tab) const {
     const size_t P = passes.size();
     const NttPass& p = passes[i];
     NttPassArgs a;
@@ -414,12 +427,14 @@ NttPassArgs NttPlan::pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inv
     a.log_n = log_n;
     a.last = p.last;
     a.scale_flag = (inverse && p.last) ? 1 : 0;
+    a.scale_tab = p.last ? scale_tab : nullptr;
     for (int k = 0; k < 8; ++k) a.scale[k] = scale[k];
     return a;
 }
 
-void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream) const {
-    for (size_t i = 0; i < passes.size(); ++i) launch_pass(tile_log, pass_args(i, data, scratch, inverse), passes[i].wgs, stream);
+void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream, const uint32_t* scale_tab) const {
+    for (size_t i = 0; i < passes.size(); ++i)
+        launch_pass(tile_log, pass_args(i, data, scratch, inverse, scale_tab), passes[i].wgs, stream);
 }
 
 // ---------------------------------------------------------------- batch

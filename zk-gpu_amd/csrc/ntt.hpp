@@ -20,6 +20,7 @@ struct NttPassArgs {
     uint32_t lm, s, lg, log_n;
     uint32_t last, scale_flag;
     uint32_t scale[8];
+    const uint32_t* scale_tab;  // last pass: multiply output element i by scale_tab[i] (Montgomery), else scale
 };
 
 // One pass = `s` radix-2 decimation-in-frequency stages on LDS-resident tiles.
@@ -51,9 +52,12 @@ struct NttPlan {
     NttPlan(Device* d, uint32_t log_n, NttShape shape = NttShape(), bool with_scratch = true);
     ~NttPlan();
     // Queue a transform of `data` (device pointer holding at least 2^log_n elements) on `stream`.
-    void run(uint32_t* data, bool inverse, hipStream_t stream) const;
+    // scale_tab (optional, 2^log_n Montgomery elements) replaces the inverse's n^-1 by a
+    // per-element factor applied in the last pass's store (either direction).
+    void run(uint32_t* data, bool inverse, hipStream_t stream, const uint32_t* scale_tab = nullptr) const;
     // Pass i of a transform of `data` using `scr` as the out-of-place scratch.
-    NttPassArgs pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inverse) const;
+    NttPassArgs pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inverse,
+                          const uint32_t* scale_tab = nullptr) const;
     size_t workspace_bytes() const;
 };
 
@@ -101,5 +105,8 @@ class NttBatch {
 };
 
 int ntt_default_tile_log(uint32_t log_n);
+// out[i] = first * base^i in Montgomery form (base, first: plain Fr words), i < count.
+void ntt_geometric_table(uint32_t* out, size_t count, const uint32_t base[8], const uint32_t first[8],
+                         hipStream_t stream);
 
 }  // namespace zkg

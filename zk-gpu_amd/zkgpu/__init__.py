@@ -13,5 +13,6 @@ from .msm import Msm, xyzz_to_affine, xyzz_to_affine_g2
 from .ntt import Ntt
 from .batch import Batch
 from .serve import Server
+from .groth16 import Groth16H
 
-__all__ = ["Batch", "Buffer", "Gpu", "Group", "Msm", "Ntt", "Server", "ZkgError", "host", "tune", "xyzz_to_affine", "xyzz_to_affine_g2"]
+__all__ = ["Batch", "Buffer", "Gpu", "Groth16H", "Group", "Msm", "Ntt", "Server", "ZkgError", "host", "tune", "xyzz_to_affine", "xyzz_to_affine_g2"]

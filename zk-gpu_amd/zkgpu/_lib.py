@@ -65,6 +65,9 @@ SIGNATURES = {
     "zkg_batch_clear": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_batch_capture": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_batch_captured": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_groth16_h_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_void_pp]),
+    "zkg_groth16_h_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_groth16_h_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_tune_msm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, c_u32p, c_u32p, c_u32p,
                                     ctypes.POINTER(ctypes.c_double)]),
     "zkg_tune_ntt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.POINTER(ctypes.c_double)]),
