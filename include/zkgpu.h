@@ -70,6 +70,14 @@ int zkg_buffer_empty(zkg_device* dev, size_t n_words, zkg_buffer** out);
 /* Gpu::read (src/gpu.rs:192-199): blocking read of the first n_words u32 (synchronises). */
 int zkg_buffer_read(zkg_buffer* buf, size_t n_words, uint32_t* out);
 /* wgpu Queue::write_buffer as used by callers that refill a buffer: queued host->device copy. */
+/* Gpu::try_read_async / read_async (src/gpu.rs:157-190): queue a read of the first n_words into
+ * `out` and return at once with a ticket. zkg_read_done polls it; zkg_read_wait blocks until the
+ * words are in `out` (which must stay valid until then) and frees the ticket. A failure returns
+ * "GPU read failed: ..." instead of aborting. */
+typedef struct zkg_read zkg_read;
+int zkg_buffer_read_async(zkg_buffer* buf, size_t n_words, uint32_t* out, zkg_read** ticket);
+int zkg_read_done(zkg_read* ticket, int* done);
+int zkg_read_wait(zkg_read* ticket);
 int zkg_buffer_write(zkg_buffer* buf, size_t offset_words, const uint32_t* words, size_t n_words);
 /* wgpu::Buffer::size: capacity in bytes. */
 size_t zkg_buffer_size(const zkg_buffer* buf);
@@ -116,6 +124,10 @@ int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words);
  * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...". */
 int zkg_msm_run(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
 /* The plan-owned result buffer (what zkg_msm_run returns), e.g. after a batch ran the plan. */
+/* Msm::run_yielding (src/msm.rs:300-318): as zkg_msm_run, but waits for the GPU after the
+ * heavy stages (accumulate, finalize, reduce chunks, combine), so a display sharing the GPU gets it
+ * in between. Same result. */
+int zkg_msm_run_yielding(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
 int zkg_msm_result(zkg_msm* plan, zkg_buffer** result);
 /* Msm::profile (src/msm.rs:321-342): run once, synchronising around every stage; writes up to
  * `cap` (stage name, milliseconds) pairs, names as NUL-separated text in `names`. */

@@ -106,6 +106,10 @@ def test_device_open_without_gpu_fails_loudly():
 
     with pytest.raises(zkgpu.ZkgError, match="GPU init failed"):
         zkgpu.Gpu()
+    import asyncio
+
+    with pytest.raises(zkgpu.ZkgError, match="GPU init failed"):
+        asyncio.run(zkgpu.Gpu.new_async())
 
 
 def test_kernels_contain_no_device_calls():

@@ -5,6 +5,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
 #include <string>
 #include <vector>
@@ -122,6 +123,61 @@ int zkg_buffer_read(zkg_buffer* buf, size_t n_words, uint32_t* out) {
     if (e != hipSuccess) return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
     return zkg::ZKG_OK;
     ZKG_GUARD_END
+}
+
+// An asynchronous read: the copy lands in pinned staging; wait() moves it to the caller's array.
+struct zkg_read {
+    zkg::Device* dev;
+    uint32_t* staging;
+    uint32_t* out;
+    size_t words;
+    hipEvent_t done;
+};
+
+int zkg_buffer_read_async(zkg_buffer* buf, size_t n_words, uint32_t* out, zkg_read** ticket) {
+    ZKG_REQUIRE(buf && ticket && (out || n_words == 0), "null argument");
+    if (4 * n_words > buf->b->bytes)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "read of " + std::to_string(4 * n_words) + " bytes from a buffer of " +
+                                                 std::to_string(buf->b->bytes));
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(buf->b->dev);
+    std::unique_ptr<zkg_read> t(new zkg_read{buf->b->dev, nullptr, out, n_words, nullptr});
+    ZKG_HIP(hipHostMalloc(reinterpret_cast<void**>(&t->staging), 4 * std::max<size_t>(n_words, 1)));
+    hipError_t e = hipEventCreateWithFlags(&t->done, hipEventDisableTiming);
+    if (e == hipSuccess && n_words)
+        e = hipMemcpyAsync(t->staging, buf->b->ptr, 4 * n_words, hipMemcpyDeviceToHost, buf->b->dev->stream);
+    if (e == hipSuccess) e = hipEventRecord(t->done, buf->b->dev->stream);
+    if (e != hipSuccess) {
+        (void)hipHostFree(t->staging);
+        if (t->done) (void)hipEventDestroy(t->done);
+        return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+    }
+    *ticket = t.release();
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_read_done(zkg_read* t, int* done) {
+    ZKG_REQUIRE(t && done, "null argument");
+    const hipError_t e = hipEventQuery(t->done);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return zkg::ZKG_OK;
+    }
+    *done = 1;
+    if (e != hipSuccess) return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+    return zkg::ZKG_OK;
+}
+
+int zkg_read_wait(zkg_read* t) {
+    ZKG_REQUIRE(t, "null argument");
+    const hipError_t e = hipEventSynchronize(t->done);
+    if (e == hipSuccess && t->words) memcpy(t->out, t->staging, 4 * t->words);
+    (void)hipHostFree(t->staging);
+    (void)hipEventDestroy(t->done);
+    delete t;
+    if (e != hipSuccess) return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+    return zkg::ZKG_OK;
 }
 
 int zkg_buffer_write(zkg_buffer* buf, size_t offset_words, const uint32_t* words, size_t n_words) {
@@ -372,6 +428,18 @@ int zkg_msm_run(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, 
     zkg::device_bind(m->plan->dev);
     m->plan->run(static_cast<const uint32_t*>(bases->b->ptr), static_cast<const uint32_t*>(scalars->b->ptr),
                  m->plan->dev->stream);
+    if (result) *result = m->result;
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_msm_run_yielding(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result) {
+    int rc = msm_check(m, bases, scalars);
+    if (rc != zkg::ZKG_OK) return rc;
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(m->plan->dev);
+    m->plan->run_yielding(static_cast<const uint32_t*>(bases->b->ptr), static_cast<const uint32_t*>(scalars->b->ptr),
+                          m->plan->dev->stream);
     if (result) *result = m->result;
     return zkg::ZKG_OK;
     ZKG_GUARD_END

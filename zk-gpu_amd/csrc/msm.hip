@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
+#include <cstring>
 
 #include "curve.hpp"
 #include "msm.hpp"
@@ -611,13 +612,19 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
 }
 
 template <class Fe>
+// prof: wait around every stage and record its time. yield_heavy: wait after the heavy stages
+// (accumulate, finalize, reduce, combine) so that other users of the GPU get it in between
+// (the reference's run_yielding, src/msm.rs:300-318).
 static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* scalars, hipStream_t s,
-                       std::vector<std::pair<std::string, double>>* prof) {
+                       std::vector<std::pair<std::string, double>>* prof, bool yield_heavy = false) {
     const uint32_t S = p.W * p.B;
     const uint32_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
     auto t0 = std::chrono::steady_clock::now();
     auto stage = [&](const char* name) {
         ZKG_HIP(hipGetLastError());
+        if (yield_heavy && (!strcmp(name, "accumulate") || !strcmp(name, "finalize") ||
+                            !strcmp(name, "reduce chunks") || !strcmp(name, "combine")))
+            ZKG_HIP(hipStreamSynchronize(s));
         if (!prof) return;
         ZKG_HIP(hipStreamSynchronize(s));
         auto t1 = std::chrono::steady_clock::now();
@@ -704,6 +711,11 @@ void MsmPlan::run(const uint32_t* bases, const uint32_t* scalars, hipStream_t st
                   std::vector<std::pair<std::string, double>>* stage_ms) const {
     if (group == 1) run_stages<Fq>(*this, bases, scalars, stream, stage_ms);
     else run_stages<Fq2>(*this, bases, scalars, stream, stage_ms);
+}
+
+void MsmPlan::run_yielding(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream) const {
+    if (group == 1) run_stages<Fq>(*this, bases, scalars, stream, nullptr, true);
+    else run_stages<Fq2>(*this, bases, scalars, stream, nullptr, true);
 }
 
 }  // namespace zkg

@@ -73,6 +73,9 @@ struct MsmPlan {
     // (stage name, milliseconds) as the reference's Msm::profile does.
     void run(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream,
              std::vector<std::pair<std::string, double>>* stage_ms = nullptr) const;
+    // As run, but waits for the GPU after the heavy stages (accumulate, finalize, reduce,
+    // combine) so that a display sharing the GPU gets it in between.
+    void run_yielding(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream) const;
     size_t workspace_bytes() const;
 };
 

@@ -26,6 +26,9 @@ SIGNATURES = {
     "zkg_buffer_create": (ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_size_t, c_void_pp]),
     "zkg_buffer_empty": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_void_pp]),
     "zkg_buffer_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_u32p]),
+    "zkg_buffer_read_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_u32p, c_void_pp]),
+    "zkg_read_done": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "zkg_read_wait": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_buffer_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_u32p, ctypes.c_size_t]),
     "zkg_buffer_size": (ctypes.c_size_t, [ctypes.c_void_p]),
     "zkg_buffer_wrap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, c_void_pp]),
@@ -45,6 +48,7 @@ SIGNATURES = {
     "zkg_msm_upload_bases_montgomery_256": (ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_size_t, c_void_pp]),
     "zkg_msm_convert_bases": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
     "zkg_msm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_void_pp]),
+    "zkg_msm_run_yielding": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_void_pp]),
     "zkg_msm_result": (ctypes.c_int, [ctypes.c_void_p, c_void_pp]),
     "zkg_msm_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
                                        ctypes.c_size_t, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,

@@ -11,7 +11,10 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import check, u32_ptr
+from ._lib import ZkgError, check, u32_ptr
+
+
+ZKG_ERR_DEVICE = 3
 
 
 class Buffer:
@@ -98,6 +101,46 @@ class Gpu:
 
     def wait(self) -> None:
         check(_lib.load().zkg_wait(self._h))
+
+    # ---- asynchronous forms (src/gpu.rs:22-40, :157-190): awaitable without blocking an event loop
+
+    @staticmethod
+    async def new_async() -> "Gpu":
+        """Gpu::new_async: raises ZkgError("GPU init failed ...") instead of returning Err."""
+        import asyncio
+
+        return await asyncio.to_thread(Gpu)
+
+    async def finished_async(self) -> None:
+        """Gpu::finished_async: resolves when all work queued so far has finished."""
+        import asyncio
+
+        await asyncio.to_thread(self.wait)
+
+    async def try_read_async(self, buf: Buffer, words: int):
+        """Gpu::try_read_async: (True, words) or (False, "GPU read failed: ...")."""
+        import asyncio
+
+        out = np.zeros(int(words), dtype=np.uint32)
+        ticket = ctypes.c_void_p()
+        lib = _lib.load()
+        if lib.zkg_buffer_read_async(buf.handle, int(words), u32_ptr(out), ctypes.byref(ticket)) != 0:
+            return False, lib.zkg_last_error().decode(errors="replace")
+        done = ctypes.c_int()
+        while True:
+            if lib.zkg_read_done(ticket, ctypes.byref(done)) != 0 or done.value:
+                break
+            await asyncio.sleep(0)
+        if lib.zkg_read_wait(ticket) != 0:
+            return False, lib.zkg_last_error().decode(errors="replace")
+        return True, out
+
+    async def read_async(self, buf: Buffer, words: int) -> np.ndarray:
+        """Gpu::read_async: the words, or ZkgError on failure."""
+        ok, val = await self.try_read_async(buf, words)
+        if not ok:
+            raise ZkgError(ZKG_ERR_DEVICE, val)
+        return val
 
     def close(self) -> None:
         if self._h:

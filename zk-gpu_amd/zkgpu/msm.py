@@ -96,6 +96,19 @@ class Msm:
             self._result = Buffer(gpu, h.value, owned=False)
         return self._result
 
+    async def run_yielding(self, gpu, bases: Buffer, scalars: Buffer) -> Buffer:
+        """Msm::run_yielding (src/msm.rs:300-318): waits for the GPU after the heavy stages so a
+        display sharing the GPU gets it in between; same result as run."""
+        import asyncio
+
+        h = ctypes.c_void_p()
+        lib = _lib.load()
+        await asyncio.to_thread(lambda: check(lib.zkg_msm_run_yielding(self._h, bases.handle, scalars.handle,
+                                                                       ctypes.byref(h))))
+        if self._result is None or self._result.handle.value != h.value:
+            self._result = Buffer(gpu, h.value, owned=False)
+        return self._result
+
     def profile(self, gpu, bases: Buffer, scalars: Buffer):
         """Run once with a wait around every stage; [(stage, milliseconds)] (src/msm.rs:321-342)."""
         cap = 64
