@@ -63,13 +63,18 @@ int zkg_device_info(zkg_device* dev, int* cu_count, int* clock_khz, size_t* tota
  * the library queues all work for this device, so callers can record events or order their
  * own work against it. */
 void* zkg_device_stream(zkg_device* dev);
+/* Gpu::report_errors (src/gpu.rs:40-49): `fn` receives "GPU error: <message>" for every error the
+ * HIP runtime returns on this device -- in a call (which also returns the error code) or where no
+ * call waits for it (a proof-job server's scheduler thread). It may run on any thread and must
+ * not call back into the library for the same device. fn = NULL removes the callback. */
+typedef void (*zkg_error_fn)(const char* message, void* user);
+int zkg_device_report_errors(zkg_device* dev, zkg_error_fn fn, void* user);
 /* Gpu::buffer (src/gpu.rs:57-64): device buffer initialised with n_words u32. */
 int zkg_buffer_create(zkg_device* dev, const uint32_t* words, size_t n_words, zkg_buffer** out);
 /* Gpu::empty_buffer (src/gpu.rs:66-69): zero-filled buffer of n_words u32. */
 int zkg_buffer_empty(zkg_device* dev, size_t n_words, zkg_buffer** out);
 /* Gpu::read (src/gpu.rs:192-199): blocking read of the first n_words u32 (synchronises). */
 int zkg_buffer_read(zkg_buffer* buf, size_t n_words, uint32_t* out);
-/* wgpu Queue::write_buffer as used by callers that refill a buffer: queued host->device copy. */
 /* Gpu::try_read_async / read_async (src/gpu.rs:157-190): queue a read of the first n_words into
  * `out` and return at once with a ticket. zkg_read_done polls it; zkg_read_wait blocks until the
  * words are in `out` (which must stay valid until then) and frees the ticket. A failure returns
@@ -78,6 +83,8 @@ typedef struct zkg_read zkg_read;
 int zkg_buffer_read_async(zkg_buffer* buf, size_t n_words, uint32_t* out, zkg_read** ticket);
 int zkg_read_done(zkg_read* ticket, int* done);
 int zkg_read_wait(zkg_read* ticket);
+/* wgpu Queue::write_buffer as used by callers that refill a buffer: host->device copy of n_words
+ * at offset_words (returns once `words` may be reused). */
 int zkg_buffer_write(zkg_buffer* buf, size_t offset_words, const uint32_t* words, size_t n_words);
 /* wgpu::Buffer::size: capacity in bytes. */
 size_t zkg_buffer_size(const zkg_buffer* buf);
@@ -123,11 +130,11 @@ int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words);
  * plan-owned buffer holding one XYZZ point (32 words G1, 64 words G2, Montgomery form), valid
  * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...". */
 int zkg_msm_run(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
-/* The plan-owned result buffer (what zkg_msm_run returns), e.g. after a batch ran the plan. */
 /* Msm::run_yielding (src/msm.rs:300-318): as zkg_msm_run, but waits for the GPU after the
  * heavy stages (accumulate, finalize, reduce chunks, combine), so a display sharing the GPU gets it
  * in between. Same result. */
 int zkg_msm_run_yielding(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
+/* The plan-owned result buffer (what zkg_msm_run returns), e.g. after a batch ran the plan. */
 int zkg_msm_result(zkg_msm* plan, zkg_buffer** result);
 /* Msm::profile (src/msm.rs:321-342): run once, synchronising around every stage; writes up to
  * `cap` (stage name, milliseconds) pairs, names as NUL-separated text in `names`. */

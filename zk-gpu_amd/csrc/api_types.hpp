@@ -16,12 +16,16 @@ struct zkg_buffer {
 namespace zkg {
 // Records `msg` as the calling thread's last error and returns `code`.
 int fail(int code, const std::string& msg);
+// As fail(), for an error the HIP runtime returned: the message also goes to the error callback
+// of `d` (nullptr: of the devices on the calling thread's current HIP device).
+int fail_gpu(Device* d, int code, const std::string& msg);
 }  // namespace zkg
 
 #define ZKG_GUARD_BEGIN try {
 #define ZKG_GUARD_END                                                   \
     }                                                                   \
     catch (const zkg::Error& e) {                                       \
+        if (e.gpu) return zkg::fail_gpu(nullptr, e.code, e.what());     \
         return zkg::fail(e.code, e.what());                             \
     }                                                                   \
     catch (const std::bad_alloc&) {                                     \

@@ -26,6 +26,11 @@ int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+
+int fail_gpu(Device* d, int code, const std::string& msg) {
+    report_gpu_error(d, msg);
+    return fail(code, msg);
+}
 }  // namespace zkg
 
 extern "C" {
@@ -65,6 +70,12 @@ int zkg_device_info(zkg_device* dev, int* cu_count, int* clock_khz, size_t* tota
 
 void* zkg_device_stream(zkg_device* dev) { return dev ? (void*)dev->d->stream : nullptr; }
 
+int zkg_device_report_errors(zkg_device* dev, zkg_error_fn fn, void* user) {
+    ZKG_REQUIRE(dev, "null device");
+    zkg::device_set_error_callback(dev->d, fn, user);
+    return zkg::ZKG_OK;
+}
+
 int zkg_buffer_create(zkg_device* dev, const uint32_t* words, size_t n_words, zkg_buffer** out) {
     ZKG_REQUIRE(dev && out && (words || n_words == 0), "null argument");
     ZKG_GUARD_BEGIN
@@ -74,7 +85,7 @@ int zkg_buffer_create(zkg_device* dev, const uint32_t* words, size_t n_words, zk
         if (e == hipSuccess) e = hipStreamSynchronize(dev->d->stream);  // `words` may be freed on return
         if (e != hipSuccess) {
             zkg::buffer_free(b);
-            return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("upload: ") + hipGetErrorString(e));
+            return zkg::fail_gpu(dev->d, zkg::ZKG_ERR_DEVICE, std::string("upload: ") + hipGetErrorString(e));
         }
     }
     *out = new zkg_buffer{b};
@@ -90,7 +101,7 @@ int zkg_buffer_empty(zkg_device* dev, size_t n_words, zkg_buffer** out) {
         hipError_t e = hipMemsetAsync(b->ptr, 0, 4 * n_words, dev->d->stream);
         if (e != hipSuccess) {
             zkg::buffer_free(b);
-            return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("memset: ") + hipGetErrorString(e));
+            return zkg::fail_gpu(dev->d, zkg::ZKG_ERR_DEVICE, std::string("memset: ") + hipGetErrorString(e));
         }
     }
     *out = new zkg_buffer{b};
@@ -120,7 +131,8 @@ int zkg_buffer_read(zkg_buffer* buf, size_t n_words, uint32_t* out) {
     zkg::device_bind(buf->b->dev);
     ZKG_HIP(hipMemcpyAsync(out, buf->b->ptr, 4 * n_words, hipMemcpyDeviceToHost, buf->b->dev->stream));
     hipError_t e = hipStreamSynchronize(buf->b->dev->stream);
-    if (e != hipSuccess) return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+    if (e != hipSuccess)
+        return zkg::fail_gpu(buf->b->dev, zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
@@ -150,7 +162,7 @@ int zkg_buffer_read_async(zkg_buffer* buf, size_t n_words, uint32_t* out, zkg_re
     if (e != hipSuccess) {
         (void)hipHostFree(t->staging);
         if (t->done) (void)hipEventDestroy(t->done);
-        return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+        return zkg::fail_gpu(buf->b->dev, zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
     }
     *ticket = t.release();
     return zkg::ZKG_OK;
@@ -165,7 +177,8 @@ int zkg_read_done(zkg_read* t, int* done) {
         return zkg::ZKG_OK;
     }
     *done = 1;
-    if (e != hipSuccess) return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+    if (e != hipSuccess)
+        return zkg::fail_gpu(t->dev, zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
     return zkg::ZKG_OK;
 }
 
@@ -173,10 +186,11 @@ int zkg_read_wait(zkg_read* t) {
     ZKG_REQUIRE(t, "null argument");
     const hipError_t e = hipEventSynchronize(t->done);
     if (e == hipSuccess && t->words) memcpy(t->out, t->staging, 4 * t->words);
+    zkg::Device* d = t->dev;
     (void)hipHostFree(t->staging);
     (void)hipEventDestroy(t->done);
     delete t;
-    if (e != hipSuccess) return zkg::fail(zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
+    if (e != hipSuccess) return zkg::fail_gpu(d, zkg::ZKG_ERR_DEVICE, std::string("GPU read failed: ") + hipGetErrorString(e));
     return zkg::ZKG_OK;
 }
 

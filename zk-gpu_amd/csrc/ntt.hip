@@ -69,6 +69,20 @@ ZKG_DEV Fr add_lazy(const Fr& x, const Fr& y) {
     return s;
 }
 
+// x in [0, 4r) -> [0, 2r)
+ZKG_DEV Fr reduce_lazy(const Fr& x) {
+    Fr d, s = x;
+    unsigned b = 0, b2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        d.w[i] = __builtin_subc(s.w[i], twice_r(i), b, &b2);
+        b = b2;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s.w[i] = b ? s.w[i] : d.w[i];
+    return s;
+}
+
 // x + 2r - y in (0, 4r)
 ZKG_DEV Fr sub_lazy(const Fr& x, const Fr& y) {
     Fr s;
@@ -175,16 +189,62 @@ __device__ __forceinline__ void ntt_pass_body(const NttPassArgs& a, const uint32
     __syncthreads();
 
     // ---- s DIF stages in LDS: pairs of stages as radix-4 rounds (four elements per lane, two
-    // stages in registers, one barrier), then a radix-2 round when s is odd. Stage st pairs rows
-    // that differ in bit lh = s-1-st of the row index; its twiddle exponent is
+    // stages in registers, one barrier), after one radix-2 round when s is odd. Stage st pairs
+    // rows that differ in bit lh = s-1-st of the row index; its twiddle exponent is
     // ((row mod 2^lh) * 2^lm + j0) * 2^(L-1-lh-lm), with j0 the tile column's offset inside its
-    // group of 2^lm columns (0 in the last pass).
+    // group of 2^lm columns. In the last pass lm = j0 = 0, so stage lh = 0 has exponent 0 and
+    // stage lh = 1 has exponents 0 and 2^(L-2): the final round multiplies only the rows with
+    // exponent 2^(L-2), and every product by w^0 = 1 becomes a reduction to [0, 2r).
     const size_t lmmask = (1ull << lm) - 1;
     // 1024-lane tiles stay within 128 VGPRs with radix-2 rounds only
     constexpr bool RADIX4 = THREADS <= 512;
-    uint32_t st = 0;
-    for (; RADIX4 && st + 1 < s; st += 2) {
+    const uint32_t r4_from = RADIX4 ? (s & 1) : s;  // stages before it run as radix-2 rounds
+    for (uint32_t st = 0; st < r4_from; ++st) {
+        const uint32_t lh = s - 1 - st;
+        if (a.last && lh == 0) {  // every twiddle is w^0
+            for (uint32_t q = threadIdx.x; q < count / 2; q += THREADS) {
+                const uint32_t sa = ((q >> lg) << (lg + 1)) + (q & gmask), sb = sa + (1u << lg);
+                const Fr x = tile_get(tile, sa), y = tile_get(tile, sb);
+                tile_put(tile, sa, add_lazy(x, y));
+                tile_put(tile, sb, reduce_lazy(sub_lazy(x, y)));
+            }
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t q = threadIdx.x; q < count / 2; q += THREADS) {
+            const uint32_t col = q & gmask, r = q >> lg;
+            const uint32_t r_lo = r & ((1u << lh) - 1);
+            const uint32_t sa = ((((r >> lh) << (lh + 1)) + r_lo) << lg) + col, sb = sa + (1u << (lh + lg));
+            const size_t j0 = a.last ? 0 : ((((size_t)wg << lg) + col) & lmmask);
+            const size_t e = (((size_t)r_lo << lm) + j0) << (L - 1 - lh - lm);
+            const Fr t = fp_load<FrParams>(a.tw + 8 * e);
+            const Fr x = tile_get(tile, sa), y = tile_get(tile, sb);
+            tile_put(tile, sa, add_lazy(x, y));
+            tile_put(tile, sb, mul_lazy(sub_lazy(x, y), t));
+        }
+        __syncthreads();
+    }
+    for (uint32_t st = r4_from; RADIX4 && st + 1 < s; st += 2) {
         const uint32_t lh1 = s - 1 - st, lh2 = lh1 - 1;
+        if (a.last && lh2 == 0) {
+            // final round of the last pass: only stage lh = 1's odd rows carry a twiddle
+            const Fr w4 = fp_load<FrParams>(a.tw + 8 * ((size_t)1 << (L - 2)));
+            for (uint32_t q = threadIdx.x; q < count / 4; q += THREADS) {
+                const uint32_t col = q & gmask, r = q >> lg;
+                const uint32_t s00 = ((r << 2) << lg) + col, s01 = s00 + (1u << lg);
+                const uint32_t s10 = s00 + (2u << lg), s11 = s10 + (1u << lg);
+                const Fr x00 = tile_get(tile, s00), x01 = tile_get(tile, s01);
+                const Fr x10 = tile_get(tile, s10), x11 = tile_get(tile, s11);
+                const Fr a0 = add_lazy(x00, x10), a1 = reduce_lazy(sub_lazy(x00, x10));
+                const Fr b0 = add_lazy(x01, x11), b1 = mul_lazy(sub_lazy(x01, x11), w4);
+                tile_put(tile, s00, add_lazy(a0, b0));
+                tile_put(tile, s01, reduce_lazy(sub_lazy(a0, b0)));
+                tile_put(tile, s10, add_lazy(a1, b1));
+                tile_put(tile, s11, reduce_lazy(sub_lazy(a1, b1)));
+            }
+            __syncthreads();
+            continue;
+        }
         for (uint32_t q = threadIdx.x; q < count / 4; q += THREADS) {
             const uint32_t col = q & gmask, r = q >> lg;
             const uint32_t lo = r & ((1u << lh2) - 1), hi = r >> lh2;
@@ -207,21 +267,6 @@ __device__ __forceinline__ void ntt_pass_body(const NttPassArgs& a, const uint32
             tile_put(tile, s01, mul_lazy(sub_lazy(a0, b0), t2));
             tile_put(tile, s10, add_lazy(a1, b1));
             tile_put(tile, s11, mul_lazy(sub_lazy(a1, b1), t2));
-        }
-        __syncthreads();
-    }
-    for (; st < s; ++st) {  // radix-2 rounds
-        const uint32_t lh = s - 1 - st;
-        for (uint32_t q = threadIdx.x; q < count / 2; q += THREADS) {
-            const uint32_t col = q & gmask, r = q >> lg;
-            const uint32_t r_lo = r & ((1u << lh) - 1);
-            const uint32_t sa = ((((r >> lh) << (lh + 1)) + r_lo) << lg) + col, sb = sa + (1u << (lh + lg));
-            const size_t j0 = a.last ? 0 : ((((size_t)wg << lg) + col) & lmmask);
-            const size_t e = (((size_t)r_lo << lm) + j0) << (L - 1 - lh - lm);
-            const Fr t = fp_load<FrParams>(a.tw + 8 * e);
-            const Fr x = tile_get(tile, sa), y = tile_get(tile, sb);
-            tile_put(tile, sa, add_lazy(x, y));
-            tile_put(tile, sb, mul_lazy(sub_lazy(x, y), t));
         }
         __syncthreads();
     }

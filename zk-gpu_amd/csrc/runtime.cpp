@@ -1,7 +1,37 @@
 // Device runtime implementation (see runtime.hpp).
 #include "runtime.hpp"
 
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 namespace zkg {
+
+namespace {
+// Open devices, so that an error raised by a HIP call can reach the callback of the device the
+// failing call was bound to.
+std::mutex g_devices_mu;
+std::vector<Device*> g_devices;
+}  // namespace
+
+void device_set_error_callback(Device* d, ErrorFn fn, void* user) {
+    std::lock_guard<std::mutex> lk(g_devices_mu);
+    d->on_error = fn;
+    d->on_error_user = fn ? user : nullptr;
+}
+
+void report_gpu_error(Device* d, const std::string& msg) {
+    std::vector<std::pair<ErrorFn, void*>> calls;
+    {
+        std::lock_guard<std::mutex> lk(g_devices_mu);
+        int ordinal = -1;
+        if (!d && hipGetDevice(&ordinal) != hipSuccess) return;
+        for (Device* x : g_devices)
+            if (x->on_error && (d ? x == d : x->ordinal == ordinal)) calls.emplace_back(x->on_error, x->on_error_user);
+    }
+    const std::string text = "GPU error: " + msg;
+    for (auto& c : calls) c.first(text.c_str(), c.second);
+}
 
 Device* device_open(int ordinal) {
     int count = 0;
@@ -23,6 +53,8 @@ Device* device_open(int ordinal) {
         delete d;
         throw Error(ZKG_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(se));
     }
+    std::lock_guard<std::mutex> lk(g_devices_mu);
+    g_devices.push_back(d);
     return d;
 }
 
@@ -30,6 +62,10 @@ void device_bind(Device* d) { ZKG_HIP(hipSetDevice(d->ordinal)); }
 
 void device_close(Device* d) {
     if (!d) return;
+    {
+        std::lock_guard<std::mutex> lk(g_devices_mu);
+        g_devices.erase(std::remove(g_devices.begin(), g_devices.end(), d), g_devices.end());
+    }
     (void)hipSetDevice(d->ordinal);
     (void)hipStreamSynchronize(d->stream);
     (void)hipStreamDestroy(d->stream);
@@ -45,7 +81,8 @@ Buffer* buffer_alloc(Device* d, size_t bytes) {
         hipError_t e = hipMalloc(&b->ptr, bytes);
         if (e != hipSuccess) {
             delete b;
-            throw Error(ZKG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            (void)hipGetLastError();  // leave no stale error for the next launch check to pick up
+            throw Error(ZKG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e), true);
         }
     }
     return b;

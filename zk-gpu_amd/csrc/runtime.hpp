@@ -14,10 +14,12 @@
 
 namespace zkg {
 
-// Raised inside the library and turned into a status code + message at the C ABI.
+// Raised inside the library and turned into a status code + message at the C ABI. `gpu` marks
+// errors returned by the HIP runtime (they also go to the device's error callback).
 struct Error : std::runtime_error {
     int code;
-    Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+    bool gpu;
+    Error(int c, const std::string& msg, bool from_gpu = false) : std::runtime_error(msg), code(c), gpu(from_gpu) {}
 };
 
 enum Status : int {
@@ -34,8 +36,11 @@ enum Status : int {
         hipError_t e_ = (call);                                                                    \
         if (e_ != hipSuccess)                                                                      \
             throw ::zkg::Error(e_ == hipErrorOutOfMemory ? ::zkg::ZKG_ERR_NOMEM : ::zkg::ZKG_ERR_DEVICE, \
-                               std::string(#call) + ": " + hipGetErrorString(e_));                 \
+                               std::string(#call) + ": " + hipGetErrorString(e_), true);           \
     } while (0)
+
+// Callback for GPU errors (Gpu::report_errors, src/gpu.rs:40-49): message and the caller's pointer.
+using ErrorFn = void (*)(const char* message, void* user);
 
 struct Device {
     int ordinal = 0;
@@ -44,6 +49,8 @@ struct Device {
     int clock_khz = 0;
     size_t total_mem = 0;
     std::string name;
+    ErrorFn on_error = nullptr;  // guarded by the device registry's lock
+    void* on_error_user = nullptr;
 };
 
 struct Buffer {
@@ -56,6 +63,11 @@ struct Buffer {
 Device* device_open(int ordinal);
 void device_close(Device* d);
 void device_bind(Device* d);  // make d's ordinal the calling thread's current HIP device
+// Install (fn != nullptr) or remove the device's GPU-error callback.
+void device_set_error_callback(Device* d, ErrorFn fn, void* user);
+// Pass "GPU error: <msg>" to the callback of `d`, or, with d == nullptr, of every open device
+// on the calling thread's current HIP device. Called outside any library lock.
+void report_gpu_error(Device* d, const std::string& msg);
 
 Buffer* buffer_alloc(Device* d, size_t bytes);
 void buffer_free(Buffer* b);

@@ -247,7 +247,7 @@ void Server::loop() {
                     ++i;
                     continue;
                 }
-                if (q != hipSuccess) throw Error(ZKG_ERR_DEVICE, std::string("job failed: ") + hipGetErrorString(q));
+                if (q != hipSuccess) throw Error(ZKG_ERR_DEVICE, std::string("job failed: ") + hipGetErrorString(q), true);
                 std::memcpy(j.results, j.slot->host, sizeof(j.results));
                 j.t_done = now_ms();
                 j.finished = true;
@@ -280,6 +280,14 @@ void Server::loop() {
             running_.clear();
             inflight_ = 0;
             cv_done_.notify_all();
+            // no call waits for this error: hand it to the device's error callback, unlocked
+            const Error* ze = dynamic_cast<const Error*>(&e);
+            if (ze && ze->gpu) {
+                const std::string msg = error_;
+                lk.unlock();
+                report_gpu_error(dev_, msg);
+                lk.lock();
+            }
         }
         if (stop_ && running_.empty()) break;
         if (running_.empty() && (pending_.empty() || !error_.empty())) {

@@ -13,6 +13,8 @@ LIB_PATH = os.environ.get("ZKGPU_LIB", os.path.join(PKG_ROOT, "_build", "libzkgp
 
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 c_void_pp = ctypes.POINTER(ctypes.c_void_p)
+# zkg_error_fn: void (*)(const char* message, void* user)
+ERROR_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
 
 # name: (restype, argtypes)
 SIGNATURES = {
@@ -23,6 +25,7 @@ SIGNATURES = {
     "zkg_device_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p, ctypes.c_size_t]),
     "zkg_device_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "zkg_device_report_errors": (ctypes.c_int, [ctypes.c_void_p, ERROR_FN, ctypes.c_void_p]),
     "zkg_buffer_create": (ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_size_t, c_void_pp]),
     "zkg_buffer_empty": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_void_pp]),
     "zkg_buffer_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_u32p]),

@@ -73,6 +73,27 @@ class Gpu:
         """The hipStream_t (as an integer) on which the library queues this device's work."""
         return int(_lib.load().zkg_device_stream(self._h) or 0)
 
+    def report_errors(self, log) -> None:
+        """Gpu::report_errors (src/gpu.rs:40-49): ``log(message)`` receives "GPU error: ..." for every
+        error the HIP runtime returns on this device, including those no call waits for (a
+        server's scheduler thread). ``log=None`` removes the callback. ``log`` may run on another
+        thread and must not call back into this device."""
+        lib = _lib.load()
+        if log is None:
+            check(lib.zkg_device_report_errors(self._h, _lib.ERROR_FN(), None))
+            self._on_error = None
+            return
+
+        def trampoline(message, _user):
+            try:
+                log(message.decode(errors="replace"))
+            except Exception:  # an exception must not unwind into the library
+                pass
+
+        fn = _lib.ERROR_FN(trampoline)
+        check(lib.zkg_device_report_errors(self._h, fn, None))
+        self._on_error = fn  # the library holds a raw pointer: keep the thunk alive
+
     def buffer(self, data) -> Buffer:
         arr = np.ascontiguousarray(np.asarray(data, dtype=np.uint32).reshape(-1))
         h = ctypes.c_void_p()
@@ -145,4 +166,5 @@ class Gpu:
     def close(self) -> None:
         if self._h:
             _lib.load().zkg_device_close(self._h)
+            self._on_error = None
             self._h = ctypes.c_void_p(0)
