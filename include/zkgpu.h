@@ -34,6 +34,7 @@ typedef struct zkg_device zkg_device;
 typedef struct zkg_buffer zkg_buffer;
 typedef struct zkg_msm zkg_msm;
 typedef struct zkg_ntt zkg_ntt;
+typedef struct zkg_zkey zkg_zkey; /* a mapped snarkjs proving key (see "snarkjs .zkey ingestion") */
 
 enum {
     ZKG_OK = 0,
@@ -222,11 +223,11 @@ int zkg_tune_clear(void);
 /* ------------------------------------------------------------------ proof-job server
  * No counterpart in the reference. A device-resident request loop with continuous batching:
  * proving keys stay in HBM; jobs are admitted in arrival order as workspace slots and the HBM
- * budget allow (footprint per job: zkg_server_slot_bytes), and admitted jobs overlap on the
- * device, each on its own streams. A job = 7 in-place NTTs of the key's size (bit k of
- * inverse_mask: polynomial k inverse) + MSMs A, B1, C (G1, witness scalars), B2 (G2, witness
- * scalars) and H (G1, n - 1 points, scalars = the first n - 1 outputs of polynomial 6).
- * Results: 192 words = XYZZ of A, B1, C, H (32 each), then B2 (64), Montgomery form. */
+ * budget allow (footprint per job: zkg_server_slot_bytes / zkg_server_key_slot_bytes), and
+ * admitted jobs overlap on the device, each on its own streams. A job = 7 in-place NTTs of
+ * 2^log_n elements (bit k of inverse_mask: polynomial k inverse) + MSMs A, B1 and B2 (G2) over
+ * the witness, C over the witness from element c_offset, and H over the first outputs of
+ * polynomial 6. Results: 192 words = XYZZ of A, B1, C, H (32 each), then B2 (64), Montgomery form. */
 typedef struct zkg_server zkg_server;
 typedef struct {
     uint64_t submitted, completed;
@@ -234,12 +235,29 @@ typedef struct {
     size_t bytes_in_use, peak_bytes;
     uint32_t inflight, peak_inflight;
 } zkg_server_stats;
+/* A key's MSM sizes: points[] = A, B1, C, H (G1) and B2 (G2); H holds at most 2^log_n points. */
+typedef struct {
+    uint32_t log_n;
+    size_t points[5];
+    size_t c_offset; /* C's scalars start at this witness element */
+} zkg_key_shape;
 int zkg_server_create(zkg_device* dev, size_t mem_budget_bytes, uint32_t max_inflight, zkg_server** out);
 int zkg_server_destroy(zkg_server* server);
 /* g1[0..3]: A, B1, C (n points each) and H (n - 1 points); g2: B2 (n points); all in the
  * library's form (zkg_msm_upload_bases*). The buffers must outlive the server. */
 int zkg_server_add_key(zkg_server* server, uint32_t log_n, zkg_buffer* const g1[4], zkg_buffer* g2, int* key_id);
+/* As zkg_server_add_key with explicit MSM sizes (bases buffers hold shape->points[] points). */
+int zkg_server_add_key_shaped(zkg_server* server, const zkg_key_shape* shape, zkg_buffer* const g1[4],
+                              zkg_buffer* g2, int* key_id);
+/* Upload a snarkjs key's A, B1, C, H and B2 sections into server-owned HBM (freed with the
+ * server) as a key of shape (log2 domainSize; nVars, nVars, nVars - nPublic - 1, domainSize,
+ * nVars; c_offset = nPublic + 1): the witness is snarkjs's full assignment and polynomial 6's
+ * outputs are the H scalars. Fails unless log2(domainSize) is in 10..=27. */
+int zkg_server_add_zkey(zkg_server* server, const zkg_zkey* zkey, int* key_id);
+int zkg_server_key_shape(zkg_server* server, int key_id, zkg_key_shape* out);
 int zkg_server_slot_bytes(zkg_server* server, uint32_t log_n, size_t* bytes);
+int zkg_server_key_slot_bytes(zkg_server* server, int key_id, size_t* bytes);
+/* witness must hold max(points A, B1, B2, c_offset + points C) scalars. */
 int zkg_server_submit(zkg_server* server, int key_id, zkg_buffer* const polys[7], uint32_t inverse_mask,
                       const zkg_buffer* witness, uint64_t* job_id);
 int zkg_server_done(zkg_server* server, uint64_t job_id, int* done);
@@ -247,6 +265,25 @@ int zkg_server_done(zkg_server* server, uint64_t job_id, int* done);
 int zkg_server_wait(zkg_server* server, uint64_t job_id, uint32_t* results, double* latency_ms);
 int zkg_server_get_stats(zkg_server* server, zkg_server_stats* out);
 int zkg_server_reset_stats(zkg_server* server);
+
+/* ------------------------------------------------------------------ snarkjs .zkey ingestion
+ * No counterpart in the reference (its README.md:224-225 leaves the snarkjs front end to another
+ * project; src/msm.rs:190-197 accepts the Montgomery-2^256 bases such a key holds). Reads the
+ * Groth16 proving-key layout snarkjs writes (version 1): the file is memory-mapped and checked
+ * (magic, version, Groth16, BN254 q and r, section sizes). Point sections: 3 IC (nPublic + 1),
+ * 5 A, 6 B1 (nVars G1), 7 B2 (nVars G2), 8 C (nVars - nPublic - 1), 9 H (domainSize). */
+typedef struct {
+    uint32_t n_vars, n_public, domain_size;
+    /* verification key, plain affine words (infinity = zeros) */
+    uint32_t alpha1[16], beta1[16], beta2[32], gamma2[32], delta1[16], delta2[32];
+} zkg_zkey_info;
+int zkg_zkey_open(const char* path, zkg_zkey** out);
+int zkg_zkey_close(zkg_zkey* zkey);
+int zkg_zkey_get_info(const zkg_zkey* zkey, zkg_zkey_info* out);
+int zkg_zkey_points(const zkg_zkey* zkey, int section, size_t* count);
+/* A point section as MSM bases (the stored words are the library's form: a copy,
+ * as zkg_msm_upload_bases_montgomery_256). */
+int zkg_zkey_upload(zkg_device* dev, const zkg_zkey* zkey, int section, zkg_buffer** out);
 
 /* ------------------------------------------------------------------ test and benchmark hooks
  * No counterpart in the reference's public API. They play the role of the reference's WGSL test

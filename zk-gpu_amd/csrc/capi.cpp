@@ -567,12 +567,46 @@ int zkg_batch_captured(const zkg_batch* batch) { return batch && batch->b->captu
 // ------------------------------------------------------------------ proof-job server
 
 #include "serve.hpp"
+#include "zkey.hpp"
 
 struct zkg_server {
     zkg::Server* s;
     zkg::Device* d;
-    std::vector<uint32_t> key_log_n;
 };
+
+struct zkg_zkey {
+    zkg::Zkey* z;
+};
+
+namespace {
+
+std::string bases_short(const zkg::Buffer* b, size_t pts, size_t words_per_point) {
+    return "bases buffer holds " + std::to_string(b->bytes) + " bytes; " + std::to_string(pts) + " points need " +
+           std::to_string(4 * words_per_point * pts);
+}
+
+int add_key_checked(zkg_server* server, const zkg::KeyShape& shape, zkg_buffer* const g1[4], zkg_buffer* g2,
+                    int* key_id) {
+    ZKG_REQUIRE(server && g1 && g2 && key_id, "null argument");
+    for (int i = 0; i < 4; ++i) ZKG_REQUIRE(g1[i], "null argument");
+    ZKG_GUARD_BEGIN
+    shape.validate();
+    for (int i = 0; i < 4; ++i)
+        if (g1[i]->b->bytes < 64 * shape.points[i])
+            return zkg::fail(zkg::ZKG_ERR_SIZE, bases_short(g1[i]->b, shape.points[i], 16));
+    if (g2->b->bytes < 128 * shape.points[4]) return zkg::fail(zkg::ZKG_ERR_SIZE, bases_short(g2->b, shape.points[4], 32));
+    const uint32_t* p[4];
+    size_t bytes = g2->b->bytes;
+    for (int i = 0; i < 4; ++i) {
+        p[i] = static_cast<const uint32_t*>(g1[i]->b->ptr);
+        bytes += g1[i]->b->bytes;
+    }
+    *key_id = server->s->add_key(shape, p, static_cast<const uint32_t*>(g2->b->ptr), bytes);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -580,7 +614,7 @@ int zkg_server_create(zkg_device* dev, size_t mem_budget_bytes, uint32_t max_inf
     ZKG_REQUIRE(dev && out, "null argument");
     ZKG_GUARD_BEGIN
     zkg::device_bind(dev->d);
-    *out = new zkg_server{new zkg::Server(dev->d, mem_budget_bytes, max_inflight), dev->d, {}};
+    *out = new zkg_server{new zkg::Server(dev->d, mem_budget_bytes, max_inflight), dev->d};
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
@@ -593,29 +627,70 @@ int zkg_server_destroy(zkg_server* server) {
 }
 
 int zkg_server_add_key(zkg_server* server, uint32_t log_n, zkg_buffer* const g1[4], zkg_buffer* g2, int* key_id) {
-    ZKG_REQUIRE(server && g1 && g2 && key_id, "null argument");
     if (log_n < 10 || log_n > 27) return zkg::fail(zkg::ZKG_ERR_INVALID, "log_n must be in 10..=27");
-    const size_t n = (size_t)1 << log_n;
-    for (int i = 0; i < 4; ++i) {
-        ZKG_REQUIRE(g1[i], "null argument");
-        const size_t pts = i == 3 ? n - 1 : n;
-        if (g1[i]->b->bytes < 64 * pts)
-            return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(g1[i]->b->bytes) + " bytes; " +
-                                                    std::to_string(pts) + " points need " + std::to_string(64 * pts));
-    }
-    if (g2->b->bytes < 128 * n)
-        return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(g2->b->bytes) + " bytes; " +
-                                                std::to_string(n) + " points need " + std::to_string(128 * n));
+    return add_key_checked(server, zkg::KeyShape::groth16(log_n), g1, g2, key_id);
+}
+
+int zkg_server_add_key_shaped(zkg_server* server, const zkg_key_shape* shape, zkg_buffer* const g1[4], zkg_buffer* g2,
+                              int* key_id) {
+    ZKG_REQUIRE(shape, "null argument");
+    zkg::KeyShape k;
+    k.log_n = shape->log_n;
+    for (int m = 0; m < 5; ++m) k.points[m] = shape->points[m];
+    k.c_offset = shape->c_offset;
+    return add_key_checked(server, k, g1, g2, key_id);
+}
+
+int zkg_server_add_zkey(zkg_server* server, const zkg_zkey* zkey, int* key_id) {
+    ZKG_REQUIRE(server && zkey && key_id, "null argument");
+    const zkg::Zkey& z = *zkey->z;
+    const uint32_t log_n = (uint32_t)__builtin_ctz(z.domain_size);
+    if (log_n < 10 || log_n > 27)
+        return zkg::fail(zkg::ZKG_ERR_INVALID, "the .zkey domain is 2^" + std::to_string(log_n) +
+                                                   "; a served key needs log_n in 10..=27");
+    zkg::KeyShape shape;
+    shape.log_n = log_n;
+    const int sections[5] = {zkg::ZKEY_A, zkg::ZKEY_B1, zkg::ZKEY_C, zkg::ZKEY_H, zkg::ZKEY_B2};
+    for (int m = 0; m < 5; ++m) shape.points[m] = z.points(sections[m]);
+    shape.c_offset = (size_t)z.n_public + 1;
+    std::vector<zkg::Buffer*> owned;
     ZKG_GUARD_BEGIN
-    const uint32_t* p[4];
-    size_t bytes = g2->b->bytes;
-    for (int i = 0; i < 4; ++i) {
-        p[i] = static_cast<const uint32_t*>(g1[i]->b->ptr);
-        bytes += g1[i]->b->bytes;
+    shape.validate();
+    size_t bytes = 0;
+    for (int m = 0; m < 5; ++m) {
+        const size_t nbytes = 4 * zkg::Zkey::point_words(sections[m]) * shape.points[m];
+        owned.push_back(zkg::buffer_alloc(server->d, nbytes));
+        // the stored words are already the library's Montgomery form (R = 2^256): a plain copy
+        ZKG_HIP(hipMemcpy(owned.back()->ptr, z.words(sections[m]), nbytes, hipMemcpyHostToDevice));
+        bytes += nbytes;
     }
-    *key_id = server->s->add_key(log_n, p, static_cast<const uint32_t*>(g2->b->ptr), bytes);
-    if (server->key_log_n.size() <= (size_t)*key_id) server->key_log_n.resize(*key_id + 1);
-    server->key_log_n[*key_id] = log_n;
+    const uint32_t* g1[4];
+    for (int m = 0; m < 4; ++m) g1[m] = static_cast<const uint32_t*>(owned[m]->ptr);
+    *key_id = server->s->add_key(shape, g1, static_cast<const uint32_t*>(owned[4]->ptr), bytes, owned);
+    return zkg::ZKG_OK;
+    }
+    catch (const zkg::Error& e) {
+        for (zkg::Buffer* b : owned) zkg::buffer_free(b);
+        if (e.gpu) return zkg::fail_gpu(server->d, e.code, e.what());
+        return zkg::fail(e.code, e.what());
+    }
+    catch (const std::bad_alloc&) {
+        for (zkg::Buffer* b : owned) zkg::buffer_free(b);
+        return zkg::fail(zkg::ZKG_ERR_NOMEM, "host allocation failed");
+    }
+    catch (const std::exception& e) {
+        for (zkg::Buffer* b : owned) zkg::buffer_free(b);
+        return zkg::fail(zkg::ZKG_ERR_INTERNAL, e.what());
+    }
+}
+
+int zkg_server_key_shape(zkg_server* server, int key_id, zkg_key_shape* out) {
+    ZKG_REQUIRE(server && out, "null argument");
+    ZKG_GUARD_BEGIN
+    const zkg::KeyShape k = server->s->key_shape(key_id);
+    out->log_n = k.log_n;
+    for (int m = 0; m < 5; ++m) out->points[m] = k.points[m];
+    out->c_offset = k.c_offset;
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
@@ -623,15 +698,26 @@ int zkg_server_add_key(zkg_server* server, uint32_t log_n, zkg_buffer* const g1[
 int zkg_server_slot_bytes(zkg_server* server, uint32_t log_n, size_t* bytes) {
     ZKG_REQUIRE(server && bytes, "null argument");
     if (log_n < 10 || log_n > 27) return zkg::fail(zkg::ZKG_ERR_INVALID, "log_n must be in 10..=27");
-    *bytes = server->s->slot_bytes(log_n);
+    ZKG_GUARD_BEGIN
+    *bytes = server->s->slot_bytes(zkg::KeyShape::groth16(log_n));
     return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_server_key_slot_bytes(zkg_server* server, int key_id, size_t* bytes) {
+    ZKG_REQUIRE(server && bytes, "null argument");
+    ZKG_GUARD_BEGIN
+    *bytes = server->s->slot_bytes(server->s->key_shape(key_id));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
 }
 
 int zkg_server_submit(zkg_server* server, int key_id, zkg_buffer* const polys[7], uint32_t inverse_mask,
                       const zkg_buffer* witness, uint64_t* job_id) {
     ZKG_REQUIRE(server && polys && witness && job_id, "null argument");
-    if (key_id < 0 || (size_t)key_id >= server->key_log_n.size()) return zkg::fail(zkg::ZKG_ERR_INVALID, "unknown proving key");
-    const uint32_t log_n = server->key_log_n[key_id];
+    ZKG_GUARD_BEGIN
+    const zkg::KeyShape shape = server->s->key_shape(key_id);
+    const uint32_t log_n = shape.log_n;
     const size_t need = (size_t)32 << log_n;
     for (int p = 0; p < 7; ++p) {
         ZKG_REQUIRE(polys[p], "null argument");
@@ -640,10 +726,10 @@ int zkg_server_submit(zkg_server* server, int key_id, zkg_buffer* const polys[7]
                                                     " bytes; a 2^" + std::to_string(log_n) + " transform needs " +
                                                     std::to_string(need));
     }
-    if (witness->b->bytes < need)
+    const size_t wlen = shape.witness_len();
+    if (witness->b->bytes < 32 * wlen)
         return zkg::fail(zkg::ZKG_ERR_SIZE, "scalars buffer holds " + std::to_string(witness->b->bytes) + " bytes; " +
-                                                std::to_string((size_t)1 << log_n) + " scalars need " + std::to_string(need));
-    ZKG_GUARD_BEGIN
+                                                std::to_string(wlen) + " scalars need " + std::to_string(32 * wlen));
     uint32_t* p[7];
     for (int k = 0; k < 7; ++k) p[k] = static_cast<uint32_t*>(polys[k]->b->ptr);
     *job_id = server->s->submit(key_id, p, inverse_mask, static_cast<const uint32_t*>(witness->b->ptr));
@@ -687,6 +773,65 @@ int zkg_server_reset_stats(zkg_server* server) {
     ZKG_REQUIRE(server, "null argument");
     server->s->reset_stats();
     return zkg::ZKG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ snarkjs .zkey
+
+extern "C" {
+
+int zkg_zkey_open(const char* path, zkg_zkey** out) {
+    ZKG_REQUIRE(path && out, "null argument");
+    ZKG_GUARD_BEGIN
+    std::unique_ptr<zkg::Zkey> z(new zkg::Zkey(path));
+    *out = new zkg_zkey{z.release()};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_zkey_close(zkg_zkey* zkey) {
+    if (!zkey) return zkg::ZKG_OK;
+    delete zkey->z;
+    delete zkey;
+    return zkg::ZKG_OK;
+}
+
+int zkg_zkey_get_info(const zkg_zkey* zkey, zkg_zkey_info* out) {
+    ZKG_REQUIRE(zkey && out, "null argument");
+    const zkg::Zkey& z = *zkey->z;
+    out->n_vars = z.n_vars;
+    out->n_public = z.n_public;
+    out->domain_size = z.domain_size;
+    // verification-key coordinates: Montgomery (R = 2^256) as stored -> plain words
+    const zkg::host::Modulus& F = zkg::host::fq();
+    auto plain = [&](uint32_t* dst, const uint32_t* src, int coords) {
+        for (int i = 0; i < coords; ++i)
+            zkg::host::to_words32(dst + 8 * i, F.from_mont(zkg::host::from_words32(src + 8 * i)));
+    };
+    plain(out->alpha1, z.alpha1, 2);
+    plain(out->beta1, z.beta1, 2);
+    plain(out->beta2, z.beta2, 4);
+    plain(out->gamma2, z.gamma2, 4);
+    plain(out->delta1, z.delta1, 2);
+    plain(out->delta2, z.delta2, 4);
+    return zkg::ZKG_OK;
+}
+
+int zkg_zkey_points(const zkg_zkey* zkey, int section, size_t* count) {
+    ZKG_REQUIRE(zkey && count, "null argument");
+    ZKG_GUARD_BEGIN
+    *count = zkey->z->points(section);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_zkey_upload(zkg_device* dev, const zkg_zkey* zkey, int section, zkg_buffer** out) {
+    ZKG_REQUIRE(dev && zkey && out, "null argument");
+    ZKG_GUARD_BEGIN
+    const size_t words = zkey->z->points(section) * zkg::Zkey::point_words(section);
+    return zkg_msm_upload_bases_montgomery_256(dev, zkey->z->words(section), words, out);
+    ZKG_GUARD_END
 }
 
 }  // extern "C"

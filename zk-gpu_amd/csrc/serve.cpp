@@ -13,6 +13,28 @@ static double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+size_t KeyShape::witness_len() const {
+    return std::max(std::max(points[0], points[1]), std::max(points[4], c_offset + points[2]));
+}
+
+KeyShape KeyShape::groth16(uint32_t log_n) {
+    KeyShape k;
+    k.log_n = log_n;
+    const size_t n = (size_t)1 << log_n;
+    for (int m = 0; m < SERVE_MSMS; ++m) k.points[m] = m == 3 ? n - 1 : n;
+    return k;
+}
+
+void KeyShape::validate() const {
+    if (log_n < 10 || log_n > 27) throw Error(ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    for (int m = 0; m < SERVE_MSMS; ++m) {
+        if (points[m] < 1) throw Error(ZKG_ERR_INVALID, "an MSM needs at least one point");
+        if (points[m] > 0x7fffffffu) throw Error(ZKG_ERR_INVALID, "an MSM holds at most 2^31 - 1 points");
+    }
+    if (points[3] > ((size_t)1 << log_n))
+        throw Error(ZKG_ERR_INVALID, "the H MSM takes its scalars from one transform: at most 2^log_n points");
+}
+
 Server::Server(Device* d, size_t mem_budget, uint32_t max_inflight)
     : dev_(d), budget_(mem_budget), max_inflight_(max_inflight ? max_inflight : 1) {
     worker_ = std::thread([this] { loop(); });
@@ -32,25 +54,31 @@ Server::~Server() {
     slots_.clear();
     for (auto& j : jobs_)
         if (j.second->ready) (void)hipEventDestroy(j.second->ready);
+    for (auto& k : keys_)
+        for (Buffer* b : k.owned) buffer_free(b);
 }
 
-size_t Server::slot_bytes(uint32_t log_n) {
-    const size_t n = (size_t)1 << log_n;
-    const MsmTune g1 = msm_config(dev_, 1, n), g2 = msm_config(dev_, 2, n);
-    return 3 * msm_workspace_bytes(1, n, g1.c, g1.item_points, g1.sort_points) +
-           msm_workspace_bytes(1, n - 1, g1.c, g1.item_points, g1.sort_points) +
-           msm_workspace_bytes(2, n, g2.c, g2.item_points, g2.sort_points) + (size_t)SERVE_POLYS * 32 * n;
+size_t Server::slot_bytes(const KeyShape& shape) {
+    size_t bytes = (size_t)SERVE_POLYS * 32 * ((size_t)1 << shape.log_n);
+    for (int m = 0; m < SERVE_MSMS; ++m) {
+        const int group = m == 4 ? 2 : 1;
+        const MsmTune t = msm_config(dev_, group, shape.points[m]);
+        bytes += msm_workspace_bytes(group, shape.points[m], t.c, t.item_points, t.sort_points);
+    }
+    return bytes;
 }
 
-int Server::add_key(uint32_t log_n, const uint32_t* const g1[4], const uint32_t* g2, size_t key_bytes) {
-    if (log_n < 10 || log_n > 27) throw Error(ZKG_ERR_INVALID, "log_n must be in 10..=27");
+int Server::add_key(const KeyShape& shape, const uint32_t* const g1[4], const uint32_t* g2, size_t key_bytes,
+                    std::vector<Buffer*> owned) {
+    shape.validate();
     std::lock_guard<std::mutex> g(mu_);
     if (key_bytes_ + key_bytes > budget_) throw Error(ZKG_ERR_NOMEM, "proving key does not fit the server's HBM budget");
     Key k;
-    k.log_n = log_n;
+    k.shape = shape;
     for (int i = 0; i < 4; ++i) k.g1[i] = g1[i];
     k.g2 = g2;
-    keys_.push_back(k);
+    k.owned = std::move(owned);
+    keys_.push_back(std::move(k));
     key_bytes_ += key_bytes;
     peak_bytes_ = std::max(peak_bytes_, key_bytes_ + slot_bytes_total_);
     return (int)keys_.size() - 1;
@@ -62,7 +90,7 @@ uint64_t Server::submit(int key, uint32_t* const polys[SERVE_POLYS], uint32_t in
         std::lock_guard<std::mutex> g(mu_);
         if (key < 0 || (size_t)key >= keys_.size()) throw Error(ZKG_ERR_INVALID, "unknown proving key");
         if (!error_.empty()) throw Error(ZKG_ERR_DEVICE, "server stopped after an error: " + error_);
-        const size_t need = slot_bytes(keys_[key].log_n);
+        const size_t need = slot_bytes(keys_[key].shape);
         if (key_bytes_ + need > budget_)
             throw Error(ZKG_ERR_NOMEM, "a job for this key needs " + std::to_string(need) +
                                            " bytes of workspace, more than the server's HBM budget leaves");
@@ -137,21 +165,28 @@ void Server::reset_stats() {
     peak_inflight_ = inflight_;
 }
 
-std::unique_ptr<Server::Slot> Server::make_slot(uint32_t log_n) {
+KeyShape Server::key_shape(int key) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (key < 0 || (size_t)key >= keys_.size()) throw Error(ZKG_ERR_INVALID, "unknown proving key");
+    return keys_[key].shape;
+}
+
+std::unique_ptr<Server::Slot> Server::make_slot(const KeyShape& shape) {
     std::unique_ptr<Slot> s(new Slot());
-    s->log_n = log_n;
-    const size_t n = (size_t)1 << log_n;
+    s->shape = shape;
+    const uint32_t log_n = shape.log_n;
     // window, item and sort sizes from the tuning table (default_window when untuned)
-    const MsmTune g1 = msm_config(dev_, 1, n), g2 = msm_config(dev_, 2, n);
-    for (int k = 0; k < 3; ++k) s->plans[k].reset(new MsmPlan(dev_, 1, n, g1.c, g1.item_points, g1.sort_points));
-    s->plans[3].reset(new MsmPlan(dev_, 1, n - 1, g1.c, g1.item_points, g1.sort_points));
-    s->plans[4].reset(new MsmPlan(dev_, 2, n, g2.c, g2.item_points, g2.sort_points));
+    for (int m = 0; m < SERVE_MSMS; ++m) {
+        const int group = m == 4 ? 2 : 1;
+        const MsmTune t = msm_config(dev_, group, shape.points[m]);
+        s->plans[m].reset(new MsmPlan(dev_, group, shape.points[m], t.c, t.item_points, t.sort_points));
+    }
     s->ntt.reset(new NttBatch(dev_));
     auto tw = twiddles_.find(log_n);
     if (tw == twiddles_.end()) {
-        NttShape shape;
-        shape.tile_log = NttBatch::tile_for(dev_, log_n);
-        tw = twiddles_.emplace(log_n, std::make_shared<NttPlan>(dev_, log_n, shape, false)).first;
+        NttShape ns;
+        ns.tile_log = NttBatch::tile_for(dev_, log_n);
+        tw = twiddles_.emplace(log_n, std::make_shared<NttPlan>(dev_, log_n, ns, false)).first;
     }
     s->ntt->share_plan(tw->second);
     for (int k = 0; k < SERVE_MSMS; ++k) {
@@ -160,7 +195,7 @@ std::unique_ptr<Server::Slot> Server::make_slot(uint32_t log_n) {
     }
     ZKG_HIP(hipEventCreateWithFlags(&s->ev_ntt, hipEventDisableTiming));
     ZKG_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->host), 4 * SERVE_RESULT_WORDS));
-    s->bytes = slot_bytes(log_n);
+    s->bytes = slot_bytes(shape);
     return s;
 }
 
@@ -177,10 +212,10 @@ void Server::destroy_slot(Slot* s) {
     s->host = nullptr;
 }
 
-Server::Slot* Server::acquire_slot(uint32_t log_n) {
+Server::Slot* Server::acquire_slot(const KeyShape& shape) {
     for (auto& s : slots_)
-        if (!s->busy && s->log_n == log_n) return s.get();
-    const size_t need = slot_bytes(log_n);
+        if (!s->busy && s->shape == shape) return s.get();
+    const size_t need = slot_bytes(shape);
     // evict idle slots of other sizes, least recently used first, until the new one fits
     while (key_bytes_ + slot_bytes_total_ + need > budget_) {
         Slot* victim = nullptr;
@@ -191,7 +226,7 @@ Server::Slot* Server::acquire_slot(uint32_t log_n) {
         slot_bytes_total_ -= victim->bytes;
         slots_.erase(std::find_if(slots_.begin(), slots_.end(), [&](const std::unique_ptr<Slot>& p) { return p.get() == victim; }));
     }
-    slots_.push_back(make_slot(log_n));
+    slots_.push_back(make_slot(shape));
     slot_bytes_total_ += slots_.back()->bytes;
     peak_bytes_ = std::max(peak_bytes_, key_bytes_ + slot_bytes_total_);
     return slots_.back().get();
@@ -199,7 +234,7 @@ Server::Slot* Server::acquire_slot(uint32_t log_n) {
 
 bool Server::try_admit(Job& j) {
     if (inflight_ >= max_inflight_) return false;
-    Slot* s = acquire_slot(keys_[j.key].log_n);
+    Slot* s = acquire_slot(keys_[j.key].shape);
     if (!s) return false;
     s->busy = true;
     s->last_use = ++use_clock_;
@@ -211,10 +246,10 @@ void Server::launch(Job& j) {
     Slot* s = j.slot;
     const Key& k = keys_[j.key];
     s->ntt->clear();
-    for (int p = 0; p < SERVE_POLYS; ++p) s->ntt->add(k.log_n, j.polys[p], (j.inverse_mask >> p) & 1);
+    for (int p = 0; p < SERVE_POLYS; ++p) s->ntt->add(k.shape.log_n, j.polys[p], (j.inverse_mask >> p) & 1);
     hipStream_t s0 = s->streams[0];
     for (int t = 0; t < SERVE_MSMS; ++t) ZKG_HIP(hipStreamWaitEvent(s->streams[t], j.ready, 0));
-    // NTT bundle, then H (its scalars are the first n - 1 outputs of the seventh transform)
+    // NTT bundle, then H (its scalars are the first outputs of the seventh transform)
     s->ntt->run(s0);
     s->plans[3]->run(k.g1[3], j.polys[SERVE_POLYS - 1], s0);
     ZKG_HIP(hipMemcpyAsync(s->host + 3 * 32, s->plans[3]->result, 4 * 32, hipMemcpyDeviceToHost, s0));
@@ -225,7 +260,7 @@ void Server::launch(Job& j) {
         const int m = order[t];
         hipStream_t st = s->streams[t + 1];
         const uint32_t* bases = m == 4 ? k.g2 : k.g1[m];
-        s->plans[m]->run(bases, j.witness, st);
+        s->plans[m]->run(bases, j.witness + (m == 2 ? 8 * k.shape.c_offset : 0), st);
         ZKG_HIP(hipMemcpyAsync(s->host + offset[t], s->plans[m]->result, 4 * (size_t)s->plans[m]->xw,
                                hipMemcpyDeviceToHost, st));
         ZKG_HIP(hipEventRecord(s->ev_done[t + 1], st));

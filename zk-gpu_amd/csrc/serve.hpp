@@ -2,9 +2,11 @@
 //
 // No counterpart in the reference (one call at a time). Proving keys (four G1 base sets A, B1,
 // C, H and one G2 set B2) stay resident in HBM. A job names a key and brings seven polynomials
-// (transformed in place, each in its own direction) and a witness (scalars of the A, B1, C and B2
-// MSMs); the H MSM takes the first n-1 outputs of the seventh transform as its scalars, as in a
-// Groth16 prover. A scheduler thread admits jobs in arrival order as soon as a workspace slot and
+// of 2^log_n elements (transformed in place, each in its own direction) and a witness (scalars of
+// the A, B1 and B2 MSMs; C's start at an offset into it); the H MSM takes the first outputs of the
+// seventh transform as its scalars, as in a Groth16 prover. A key's shape gives each MSM's point
+// count: n, n, n, n - 1, n by default, or a snarkjs key's nVars, nVars, nVars - nPublic - 1,
+// domainSize, nVars with C's scalars from witness element nPublic + 1. A scheduler thread admits jobs in arrival order as soon as a workspace slot and
 // the HBM budget allow, so jobs that arrive while others run overlap on the device: every job
 // runs on its own streams (NTTs grouped into one launch per pass phase, then H; A, B1, C and B2
 // concurrently). Results (five XYZZ points) land in pinned host memory.
@@ -27,6 +29,25 @@ static constexpr int SERVE_POLYS = 7;
 static constexpr int SERVE_MSMS = 5;                  // A, B1, C, H (G1) and B2 (G2)
 static constexpr int SERVE_RESULT_WORDS = 4 * 32 + 64;  // XYZZ words: four G1 points, then the G2 point
 
+struct Buffer;
+
+struct KeyShape {
+    uint32_t log_n = 0;               // polynomial size 2^log_n
+    size_t points[SERVE_MSMS] = {};   // A, B1, C, H (G1) and B2 (G2)
+    size_t c_offset = 0;              // C's scalars start at this witness element
+    bool operator==(const KeyShape& o) const {
+        for (int k = 0; k < SERVE_MSMS; ++k)
+            if (points[k] != o.points[k]) return false;
+        return log_n == o.log_n && c_offset == o.c_offset;
+    }
+    // scalars a job's witness must hold
+    size_t witness_len() const;
+    // n, n, n, n - 1, n points with n = 2^log_n
+    static KeyShape groth16(uint32_t log_n);
+    // checks the ranges; throws on violation
+    void validate() const;
+};
+
 struct ServerStats {
     uint64_t submitted = 0, completed = 0;
     double p50_ms = 0, p99_ms = 0, mean_ms = 0, jobs_per_s = 0;
@@ -41,9 +62,11 @@ class Server {
     Server(Device* d, size_t mem_budget, uint32_t max_inflight);
     ~Server();
 
-    // bases: device buffers in the library's Montgomery form; g1[0..3] = A, B1, C (n points),
-    // H (n - 1 points); g2 = B2 (n points). Returns the key id.
-    int add_key(uint32_t log_n, const uint32_t* const g1[4], const uint32_t* g2, size_t key_bytes);
+    // bases: device buffers in the library's Montgomery form holding shape.points[] points;
+    // g1[0..3] = A, B1, C, H; g2 = B2. `owned` buffers are freed with the server. Returns the key id.
+    int add_key(const KeyShape& shape, const uint32_t* const g1[4], const uint32_t* g2, size_t key_bytes,
+                std::vector<Buffer*> owned = {});
+    KeyShape key_shape(int key);
     // Queue a job (returns its id); the buffers must stay valid until the job completes.
     uint64_t submit(int key, uint32_t* const polys[SERVE_POLYS], uint32_t inverse_mask, const uint32_t* witness);
     // Block until the job is done; copies its SERVE_RESULT_WORDS result words and latency.
@@ -51,17 +74,18 @@ class Server {
     bool done(uint64_t job);
     ServerStats stats();
     void reset_stats();
-    // Per-job device footprint for a key of 2^log_n points (workspace slot).
-    size_t slot_bytes(uint32_t log_n);
+    // Per-job device footprint (workspace slot) for a key of this shape.
+    size_t slot_bytes(const KeyShape& shape);
 
   private:
     struct Key {
-        uint32_t log_n;
+        KeyShape shape;
         const uint32_t* g1[4];
         const uint32_t* g2;
+        std::vector<Buffer*> owned;
     };
     struct Slot {
-        uint32_t log_n = 0;
+        KeyShape shape;
         std::unique_ptr<MsmPlan> plans[SERVE_MSMS];
         std::unique_ptr<NttBatch> ntt;
         hipStream_t streams[SERVE_MSMS] = {};
@@ -87,8 +111,8 @@ class Server {
     void loop();
     bool try_admit(Job& j);           // called with mu_ held
     void launch(Job& j);              // called with mu_ held
-    Slot* acquire_slot(uint32_t log_n);
-    std::unique_ptr<Slot> make_slot(uint32_t log_n);
+    Slot* acquire_slot(const KeyShape& shape);
+    std::unique_ptr<Slot> make_slot(const KeyShape& shape);
     void destroy_slot(Slot* s);
 
     Device* dev_;

@@ -12,7 +12,9 @@ from .group import Group
 from .msm import Msm, xyzz_to_affine, xyzz_to_affine_g2
 from .ntt import Ntt
 from .batch import Batch
-from .serve import Server
+from .serve import KeyShape, Server
 from .groth16 import Groth16H
+from .zkey import Zkey
 
-__all__ = ["Batch", "Buffer", "Gpu", "Groth16H", "Group", "Msm", "Ntt", "Server", "ZkgError", "host", "tune", "xyzz_to_affine", "xyzz_to_affine_g2"]
+__all__ = ["Batch", "Buffer", "Gpu", "Groth16H", "Group", "KeyShape", "Msm", "Ntt", "Server", "Zkey", "ZkgError", "host",
+           "tune", "xyzz_to_affine", "xyzz_to_affine_g2"]

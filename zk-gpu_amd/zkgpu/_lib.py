@@ -89,6 +89,16 @@ SIGNATURES = {
     "zkg_server_add_key": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p * 4, ctypes.c_void_p,
                                           ctypes.POINTER(ctypes.c_int)]),
     "zkg_server_slot_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]),
+    "zkg_server_add_key_shaped": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p * 4,
+                                                 ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "zkg_server_add_zkey": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "zkg_server_key_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "zkg_server_key_slot_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
+    "zkg_zkey_open": (ctypes.c_int, [ctypes.c_char_p, c_void_pp]),
+    "zkg_zkey_close": (ctypes.c_int, [ctypes.c_void_p]),
+    "zkg_zkey_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "zkg_zkey_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
+    "zkg_zkey_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, c_void_pp]),
     "zkg_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p * 7, ctypes.c_uint32,
                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "zkg_server_done": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),

@@ -1,8 +1,10 @@
 """Proof-job server: resident proving keys, continuous batching (no counterpart in the reference).
 
-A job = seven in-place NTTs of the key's size + the five Groth16 MSMs (A, B1, C and B2 over the
-witness, H over the first n - 1 outputs of the seventh transform). Jobs are admitted in arrival
-order while the HBM budget and the in-flight limit allow, and admitted jobs overlap on the device.
+A job = seven in-place NTTs of the key's size + the five Groth16 MSMs (A, B1 and B2 over the
+witness, C over the witness from the key's C offset, H over the first outputs of the seventh
+transform). Jobs are admitted in arrival order while the HBM budget and the in-flight limit allow,
+and admitted jobs overlap on the device. Keys come from device buffers (`add_key`,
+`add_key_shaped`) or from a snarkjs proving key (`add_zkey`).
 """
 from __future__ import annotations
 
@@ -34,6 +36,19 @@ class ServerStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class KeyShape(ctypes.Structure):
+    """MSM sizes of a key: points = (A, B1, C, H, B2); C's scalars start at witness[c_offset]."""
+
+    _fields_ = [("log_n", ctypes.c_uint32), ("points", ctypes.c_size_t * 5), ("c_offset", ctypes.c_size_t)]
+
+    @classmethod
+    def make(cls, log_n: int, points, c_offset: int = 0) -> "KeyShape":
+        return cls(int(log_n), (ctypes.c_size_t * 5)(*[int(p) for p in points]), int(c_offset))
+
+    def as_dict(self):
+        return {"log_n": self.log_n, "points": tuple(self.points), "c_offset": self.c_offset}
+
+
 class Server:
     def __init__(self, gpu, mem_budget_bytes: int, max_inflight: int = 4):
         h = ctypes.c_void_p()
@@ -49,6 +64,30 @@ class Server:
         check(_lib.load().zkg_server_add_key(self._h, int(log_n), arr, g2.handle, ctypes.byref(kid)))
         self._keep[("key", kid.value)] = (tuple(g1), g2)
         return kid.value
+
+    def add_key_shaped(self, shape: KeyShape, g1, g2) -> int:
+        """As add_key with explicit MSM sizes (`KeyShape.make(log_n, points, c_offset)`)."""
+        arr = (ctypes.c_void_p * 4)(*[b.handle for b in g1])
+        kid = ctypes.c_int()
+        check(_lib.load().zkg_server_add_key_shaped(self._h, ctypes.byref(shape), arr, g2.handle, ctypes.byref(kid)))
+        self._keep[("key", kid.value)] = (tuple(g1), g2)
+        return kid.value
+
+    def add_zkey(self, zkey) -> int:
+        """Upload a snarkjs proving key (`zkgpu.Zkey`) into server-owned HBM; returns the key id."""
+        kid = ctypes.c_int()
+        check(_lib.load().zkg_server_add_zkey(self._h, zkey.handle, ctypes.byref(kid)))
+        return kid.value
+
+    def key_shape(self, key: int) -> dict:
+        out = KeyShape()
+        check(_lib.load().zkg_server_key_shape(self._h, int(key), ctypes.byref(out)))
+        return out.as_dict()
+
+    def key_slot_bytes(self, key: int) -> int:
+        out = ctypes.c_size_t()
+        check(_lib.load().zkg_server_key_slot_bytes(self._h, int(key), ctypes.byref(out)))
+        return out.value
 
     def slot_bytes(self, log_n: int) -> int:
         out = ctypes.c_size_t()
