@@ -1,0 +1,224 @@
+"""The 29-bit-limb field arithmetic of the G1 bucket accumulation (zk-gpu_amd/csrc/field29.hpp,
+msm_accumulate29_kernel), restated limb for limb in Python: the constants are re-derived from p,
+every intermediate is checked against the ranges the kernel relies on (limbs in [0, 2^32),
+column sums below 2^64) on worst-case and random inputs, the value bounds the kernel states
+(X < 9p, Y < 5p, ZZ, ZZZ < 2p) are shown to be a fixed point, and the point law it computes equals
+the textbook affine law. CPU only."""
+import os
+import random
+import re
+
+import pytest
+
+import oracle as O
+
+P = O.P
+MASK = (1 << 29) - 1
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = open(os.path.join(ROOT, "zk-gpu_amd", "csrc", "field29.hpp")).read()
+
+
+def limbs(v):
+    return [(v >> (29 * i)) & MASK for i in range(8)] + [v >> 232]
+
+
+def value(l):
+    return sum(x << (29 * i) for i, x in enumerate(l))
+
+
+def header_table(name):
+    """The nine (or twelve) constants of one `name(int i)` function in field29.hpp."""
+    m = re.search(r"constexpr uint32_t " + re.escape(name) + r"\(int [ik]\) \{(.*?)\}", HDR, re.S)
+    return [int(x, 16) for x in re.findall(r"0x([0-9a-f]{8})u", m.group(1))] if m else None
+
+
+def header_k(c):
+    m = re.search(r"K<%d>\(int i\) \{(.*?)\}" % c, HDR, re.S)
+    return [int(x, 16) for x in re.findall(r"0x([0-9a-f]{8})u", m.group(1))]
+
+
+def test_constants_follow_from_p():
+    assert header_table("P") == limbs(P)
+    assert int(re.search(r"PINV = 0x([0-9a-f]+)u", HDR).group(1), 16) == (-pow(P, -1, 1 << 29)) % (1 << 29)
+    assert header_table("ONE") == limbs(pow(2, 261, P))
+    assert header_table("C256") == limbs(pow(2, 256, P))
+    kp0 = header_table("KP0")
+    assert [0] + kp0 == [limbs(k * P)[0] for k in range(12)]  # the k = 0 entry is written as 0u
+    for c in (3, 6, 7, 10):
+        k = header_k(c)
+        assert value(k) == c * P
+        assert k[0] >= 1 << 31 and all(x >= (1 << 31) - 4 for x in k[1:8]) and all(x < 1 << 32 for x in k)
+
+
+# ---- limb-exact restatement with range checks
+
+def mul(a, b):
+    assert all(0 <= x < 1 << 30 for x in a + b), "multiplication input limb out of range"
+    pinv = (-pow(P, -1, 1 << 29)) % (1 << 29)
+    pl = limbs(P)
+    acc, m, r = 0, [0] * 9, [0] * 9
+    for k in range(9):
+        for i in range(k + 1):
+            acc += a[i] * b[k - i]
+        for j in range(k):
+            acc += m[j] * pl[k - j]
+        assert acc < 1 << 64
+        m[k] = ((acc & 0xFFFFFFFF) * pinv) & MASK
+        acc += m[k] * pl[0]
+        assert acc < 1 << 64 and acc & MASK == 0
+        acc >>= 29
+    for k in range(9, 17):
+        for i in range(k - 8, 9):
+            acc += a[i] * b[k - i]
+        for j in range(k - 8, 9):
+            acc += m[j] * pl[k - j]
+        assert acc < 1 << 64
+        r[k - 9] = acc & MASK
+        acc >>= 29
+    r[8] = acc
+    assert r[8] < 1 << 32
+    assert value(r) % P == value(a) * value(b) * pow(2, -261, P) % P
+    return r
+
+
+def add(a, b):
+    return [x + y for x, y in zip(a, b)]
+
+
+def sub(c, a, b):
+    k = header_k(c)
+    r = [x + y - z for x, y, z in zip(a, k, b)]
+    assert all(0 <= x < 1 << 32 for x in r), "subtraction limb out of range"
+    for i in range(8):
+        r[i + 1] += r[i] >> 29
+        r[i] &= MASK
+    assert value(r) == value(a) + c * P - value(b)
+    return r
+
+
+def worst(bound_p):
+    """A normalised value just below bound_p * p with every low limb at its maximum."""
+    v = bound_p * P - 1
+    l = limbs(v)
+    return [MASK] * 8 + [l[8] - 1] if l[8] > 0 else l
+
+
+def test_column_sums_fit_for_worst_case_limbs():
+    # all limbs at 2^30 - 1: the largest inputs the multiplication accepts
+    a = [(1 << 30) - 1] * 9
+    pinv = (-pow(P, -1, 1 << 29)) % (1 << 29)
+    pl = limbs(P)
+    acc, m = 0, [MASK] * 9  # m at its maximum too
+    worst_col = 0
+    for k in range(17):
+        col = sum(a[i] * a[k - i] for i in range(max(0, k - 8), min(k, 8) + 1))
+        col += sum(m[j] * pl[k - j] for j in range(max(0, k - 8), min(k, 8) + 1))
+        worst_col = max(worst_col, col + ((1 << 64) >> 29))
+    assert worst_col < 1 << 64
+
+
+def test_value_bounds_are_a_fixed_point():
+    """Interval check of the kernel's stated bounds (units of p): products need a * b <= 169."""
+    lim = (1 << 261) / P
+    assert 169 < lim < 170
+    X, Y, ZZ, ZZZ, xs = 9, 5, 2, 2, 32
+
+    def prod(a, b):
+        assert a * b < lim, (a, b)
+        return a * b / lim + 1  # output < a b p / 2^261 + p
+
+    u2 = prod(xs, ZZ)
+    s2 = 3  # neg: 3p - s2
+    assert prod(xs, ZZZ) <= 2
+    pp = u2 + 10
+    r = s2 + 6
+    assert X < 10 - 1 + 1e-9 and Y < 6 - 1 + 1e-9 and s2 <= 3
+    ppsq, ppp, q, rr = prod(pp, pp), prod(pp, 2), prod(X, 2), prod(r, r)
+    assert max(ppsq, ppp, q, rr) <= 2
+    t = 2 + 2 * 2
+    assert t <= 7 - 1
+    x3 = rr + 7
+    d = q + 10
+    assert x3 <= 9 and x3 <= 10 - 1
+    y3 = prod(r, d) + 3
+    assert prod(Y, 2) <= 2 and y3 <= 5
+    assert prod(ZZ, 2) <= 2 and prod(ZZZ, 2) <= 2
+    assert prod(X, 1) <= 2  # the final conversion by 2^256 mod p < p
+
+
+def test_kernel_point_law_matches_textbook_on_random_and_edge_chains():
+    """The kernel's bucket loop (madd with lazy bounds), restated, against affine addition."""
+    rng = random.Random(29)
+    one = limbs(pow(2, 261, P))
+    c256 = limbs(pow(2, 256, P))
+
+    def load(x256):  # R = 2^256 words -> 32x (R = 2^261 form, unreduced)
+        return limbs(x256 << 5)
+
+    def to_affine(Xl, Yl, ZZl, ZZZl):
+        x, y, zz, zzz = (value(mul(v, c256)) % P * pow(2, -256, P) % P for v in (Xl, Yl, ZZl, ZZZl))
+        return (x * pow(zz, -1, P) % P, y * pow(zzz, -1, P) % P)
+
+    hits = set()
+    for trial in range(6):
+        n = 12
+        ks = [rng.randrange(1, O.R) for _ in range(n)]
+        negs = [rng.random() < 0.5 for _ in range(n)]
+        if trial in (1, 2):
+            ks[1] = ks[0]  # the accumulator equals the next point: doubling (1) or P + (-P) (2)
+            negs[1] = negs[0] if trial == 1 else not negs[0]
+        if trial == 3:
+            ks[5] = O.R - ks[4]  # a point and its negative inside a chain
+        pts = [O.py_g1_mul(O.G1, k) for k in ks]
+        exp = None
+        acc = None
+        for (x, y), neg, k in zip(pts, negs, ks):
+            xs, ys = load(x * 2 ** 256 % P), load(y * 2 ** 256 % P)
+            yy = (P - y) % P if neg else y
+            exp = O.py_g1_add(exp, (x, yy)) if exp is not None else (x, yy)
+            if acc is None:
+                Y = mul(ys, one)
+                acc = [mul(xs, one), sub(3, [0] * 9, Y) if neg else Y, one, one]
+                continue
+            X, Y, ZZ, ZZZ = acc
+            u2 = mul(xs, ZZ)
+            s2 = mul(ys, ZZZ)
+            if neg:
+                s2 = sub(3, [0] * 9, s2)
+            pp = sub(10, u2, X)
+            r = sub(6, s2, Y)
+            if value(pp) % P == 0:
+                assert value(pp) < 12 * P
+                hits.add("doubling" if value(r) % P == 0 else "cancel")
+                if value(r) % P == 0:
+                    # doubling in the 32-bit field, then back (a fresh, reduced representation)
+                    ax, ay = to_affine(X, Y, ZZ, ZZZ)
+                    dx, dy = O.py_g1_add((ax, ay), (ax, ay))
+                    acc = [mul(load(dx * 2 ** 256 % P), one), mul(load(dy * 2 ** 256 % P), one), one, one]
+                else:
+                    acc = None
+                continue
+            ppsq = mul(pp, pp)
+            ppp, q, rr = mul(pp, ppsq), mul(X, ppsq), mul(r, r)
+            t = add(ppp, add(q, q))
+            x3 = sub(7, rr, t)
+            d = sub(10, q, x3)
+            y3 = sub(3, mul(r, d), mul(Y, ppp))
+            acc = [x3, y3, mul(ZZ, ppsq), mul(ZZZ, ppp)]
+            assert value(x3) < 9 * P and value(y3) < 5 * P
+            assert value(acc[2]) < 2 * P and value(acc[3]) < 2 * P
+        if exp is None or acc is None:
+            assert acc is None and exp is None
+            continue
+        assert to_affine(*acc) == exp, trial
+    assert hits == {"doubling", "cancel"}
+
+
+@pytest.mark.parametrize("bound", [2, 9, 12])
+def test_worst_case_normalised_inputs_multiply_in_range(bound):
+    a = worst(bound)
+    assert value(a) < bound * P
+    b = worst(12 if bound <= 12 else 2)
+    if bound * 12 < (1 << 261) / P:
+        r = mul(a, b)
+        assert value(r) < 2 * P

@@ -1,0 +1,217 @@
+// BN254 base-field arithmetic in nine 29-bit limbs (Montgomery R = 2^261), for the MSM's
+// bucket accumulation.
+//
+// Why a second representation: with 32-bit limbs every partial product of a Montgomery multiply
+// needs an add-with-carry and a register move besides its v_mad_u64_u32 (the hardware adds one
+// 64-bit operand and the compiler does not use the multiply-add's carry-out), about 390
+// instructions per product. With 29-bit limbs a product of two limbs is below 2^60 and a whole
+// column of the schoolbook product plus the reduction terms (at most 18 such products) stays
+// below 2^64, so product scanning ("finely integrated product scanning", Koc, Acar and Kaliski,
+// IEEE Micro 1996) accumulates every partial product with one v_mad_u64_u32 and no carry
+// handling; a column costs one shift and mask. About 235 instructions per product.
+//
+// Values are kept lazily reduced. A value v represents v * 2^-261 mod p; limbs 0..7 are below
+// 2^29 after a multiplication or a subtraction ("normalised") and limb 8 holds the rest. The
+// multiplication needs limbs below 2^30 and v(a) * v(b) < p * 2^261 (about 169 p^2), and then
+// returns a normalised value below 2p. Callers track value bounds in units of p (the bucket
+// addition in msm_kernels29.hpp states its bounds step by step).
+#pragma once
+#include <stdint.h>
+
+#include "field.hpp"
+
+namespace zkg {
+
+struct F29 {
+    uint32_t l[9];
+};
+
+namespace f29 {
+
+static constexpr uint32_t MASK = (1u << 29) - 1;
+static constexpr uint32_t PINV = 0x04866389u;  // -p^-1 mod 2^29
+
+ZKG_DEV constexpr uint32_t P(int i) {  // p
+    return i == 0 ? 0x187cfd47u : i == 1 ? 0x010460b6u : i == 2 ? 0x1c72a34fu : i == 3 ? 0x02d522d0u
+         : i == 4 ? 0x1585d978u : i == 5 ? 0x02db40c0u : i == 6 ? 0x00a6e141u : i == 7 ? 0x0e5c2634u : 0x0030644eu;
+}
+
+// c * p in "borrowed" limbs: limb 0 plus 2^31, limbs 1..7 plus 2^31 - 4, limb 8 minus 4, so that
+// subtracting any limb below 2^31 - 4 leaves every limb non-negative (same value c * p).
+template <int C>
+ZKG_DEV constexpr uint32_t K(int i);
+template <>
+ZKG_DEV constexpr uint32_t K<3>(int i) {
+    return i == 0 ? 0x8976f7d5u : i == 1 ? 0x830d2220u : i == 2 ? 0x9557e9e9u : i == 3 ? 0x887f686eu
+         : i == 4 ? 0x80918c64u : i == 5 ? 0x8891c23eu : i == 6 ? 0x81f4a3bfu : i == 7 ? 0x8b147298u : 0x00912ce7u;
+}
+template <>
+ZKG_DEV constexpr uint32_t K<6>(int i) {
+    return i == 0 ? 0x92edefaau : i == 1 ? 0x861a4444u : i == 2 ? 0x8aafd3d6u : i == 3 ? 0x90fed0e1u
+         : i == 4 ? 0x812318ccu : i == 5 ? 0x91238480u : i == 6 ? 0x83e94782u : i == 7 ? 0x9628e534u : 0x012259d2u;
+}
+template <>
+ZKG_DEV constexpr uint32_t K<7>(int i) {
+    return i == 0 ? 0x8b6aecf1u : i == 1 ? 0x871ea4fbu : i == 2 ? 0x87227725u : i == 3 ? 0x93d3f3b2u
+         : i == 4 ? 0x96a8f244u : i == 5 ? 0x93fec540u : i == 6 ? 0x849028c3u : i == 7 ? 0x84850b68u : 0x0152be21u;
+}
+template <>
+ZKG_DEV constexpr uint32_t K<10>(int i) {
+    return i == 0 ? 0x94e1e4c6u : i == 1 ? 0x8a2bc71fu : i == 2 ? 0x9c7a6112u : i == 3 ? 0x9c535c24u
+         : i == 4 ? 0x973a7eacu : i == 5 ? 0x9c908782u : i == 6 ? 0x8684cc86u : i == 7 ? 0x8f997e04u : 0x01e3eb0cu;
+}
+
+ZKG_DEV constexpr uint32_t ONE(int i) {  // 2^261 mod p: the Montgomery form of 1
+    return i == 0 ? 0x157ccc21u : i == 1 ? 0x141c2758u : i == 2 ? 0x185230d3u : i == 3 ? 0x014c0419u
+         : i == 4 ? 0x0aa36fb9u : i == 5 ? 0x1d4240ceu : i == 6 ? 0x11d54c07u : i == 7 ? 0x052ac7a8u : 0x000dc836u;
+}
+
+ZKG_DEV constexpr uint32_t C256(int i) {  // 2^256 mod p: multiplying by it maps R = 2^261 form to R = 2^256 form
+    return i == 0 ? 0x058f0d9du : i == 1 ? 0x1aea1c6eu : i == 2 ? 0x11c2cf74u : i == 3 ? 0x11d651ebu
+         : i == 4 ? 0x1462c0a7u : i == 5 ? 0x11b7bc3cu : i == 6 ? 0x1cbd99bau : i == 7 ? 0x183340fbu : 0x000e0a77u;
+}
+
+// Limb 0 of k * p for k = 0..11 (a value below 12p is 0 mod p only if its limb 0 is one of these).
+ZKG_DEV constexpr uint32_t KP0(int k) {
+    return k == 0 ? 0u : k == 1 ? 0x187cfd47u : k == 2 ? 0x10f9fa8eu : k == 3 ? 0x0976f7d5u : k == 4 ? 0x01f3f51cu
+         : k == 5 ? 0x1a70f263u : k == 6 ? 0x12edefaau : k == 7 ? 0x0b6aecf1u : k == 8 ? 0x03e7ea38u
+         : k == 9 ? 0x1c64e77fu : k == 10 ? 0x14e1e4c6u : 0x0d5ee20du;
+}
+
+}  // namespace f29
+
+ZKG_DEV F29 f29_const_one() {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = f29::ONE(i);
+    return r;
+}
+
+// Montgomery product a * b * 2^-261 (FIPS). Needs limbs < 2^30 and v(a) v(b) < p 2^261; the
+// result is normalised and below a b / 2^261 + p < 2p. Column sums: at most nine products below
+// 2^60, nine below 2^58 and a carry below 2^35, i.e. below 1.3 * 2^63.
+ZKG_DEV F29 f29_mul(const F29& a, const F29& b) {
+    uint64_t acc = 0;
+    uint32_t m[9];
+    F29 r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int j = 0; j < k; ++j) acc += (uint64_t)m[j] * f29::P(k - j);
+        m[k] = ((uint32_t)acc * f29::PINV) & f29::MASK;
+        acc += (uint64_t)m[k] * f29::P(0);  // clears the low 29 bits
+        acc >>= 29;
+    }
+#pragma unroll
+    for (int k = 9; k < 17; ++k) {
+#pragma unroll
+        for (int i = k - 8; i <= 8; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int j = k - 8; j <= 8; ++j) acc += (uint64_t)m[j] * f29::P(k - j);
+        r.l[k - 9] = (uint32_t)acc & f29::MASK;
+        acc >>= 29;
+    }
+    r.l[8] = (uint32_t)acc;
+    return r;
+}
+
+// Limb-wise sum, not normalised: limbs add, value adds.
+ZKG_DEV F29 f29_add(const F29& a, const F29& b) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = a.l[i] + b.l[i];
+    return r;
+}
+
+// a + C p - b, normalised. Needs v(b) < (C - 1) p, b's limbs < 2^31 - 4 and a's limbs < 2^30, so
+// that no limb leaves [0, 2^32); the value is below v(a) + C p.
+template <int C>
+ZKG_DEV F29 f29_sub(const F29& a, const F29& b) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = a.l[i] + f29::K<C>(i) - b.l[i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        r.l[i + 1] += r.l[i] >> 29;
+        r.l[i] &= f29::MASK;
+    }
+    return r;
+}
+
+template <int C>
+ZKG_DEV F29 f29_neg(const F29& b) {
+    F29 z;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) z.l[i] = 0;
+    return f29_sub<C>(z, b);
+}
+
+// For a normalised value below 12p: is it 0 mod p? (k p is normalised-unique, so equality of the
+// limbs decides; limb 0 filters first and the full comparison runs only on a match.)
+ZKG_DEV bool f29_is_zero_below12p(const F29& a) {
+    bool maybe = false;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) maybe |= a.l[0] == f29::KP0(k);
+    if (!maybe) return false;
+    // rare: compare every limb with k p, computed here
+    bool zero = false;
+    for (uint32_t k = 0; k < 12; ++k) {
+        uint64_t carry = 0;
+        bool eq = true;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            carry += (uint64_t)k * f29::P(i);
+            const uint32_t limb = i < 8 ? (uint32_t)carry & f29::MASK : (uint32_t)carry;
+            carry >>= 29;
+            eq &= a.l[i] == limb;
+        }
+        zero |= eq;
+    }
+    return zero;
+}
+
+// Eight 32-bit words holding x (< 2^256, e.g. an Fq in R = 2^256 form) -> limbs of 32 x: the same
+// field element in R = 2^261 form, not reduced (value below 32p for x < p).
+ZKG_DEV F29 f29_from_r256_words(const uint32_t w[8]) {
+    F29 r;
+    r.l[0] = (w[0] << 5) & f29::MASK;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        const int o = 29 * i - 5, q = o >> 5, s = o & 31;
+        const uint64_t two = ((uint64_t)w[q + 1] << 32) | w[q];
+        r.l[i] = (uint32_t)(two >> s) & f29::MASK;
+    }
+    r.l[8] = w[7] >> 3;
+    return r;
+}
+
+// A normalised value below 2^256 -> its eight 32-bit words.
+ZKG_DEV void f29_to_words(const F29& a, uint32_t w[8]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int b = 32 * j, i = b / 29, s = b - 29 * i;
+        uint64_t v = (uint64_t)a.l[i] >> s;
+        v |= (uint64_t)a.l[i + 1] << (29 - s);
+        if (i + 2 <= 8 && 58 - s < 32) v |= (uint64_t)a.l[i + 2] << (58 - s);
+        w[j] = (uint32_t)v;
+    }
+}
+
+// Fq (R = 2^256, canonical) -> F29 below 2p.
+ZKG_DEV F29 f29_from_fq(const Fq& x) { return f29_mul(f29_from_r256_words(x.w), f29_const_one()); }
+
+// F29 (value below 12p) -> canonical Fq in R = 2^256 form.
+ZKG_DEV Fq f29_to_fq(const F29& a) {
+    F29 c;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) c.l[i] = f29::C256(i);
+    const F29 t = f29_mul(a, c);  // a 2^256 2^-261 = the R = 2^256 form, below 2p
+    Fq r;
+    f29_to_words(t, r.w);
+    fp_reduce_once<FqParams>(r.w);
+    return r;
+}
+
+}  // namespace zkg

@@ -519,6 +519,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     n = (uint32_t)npts;
     c = win;
     glv = msm_glv_enabled(npts);
+    f29 = grp == 1 && msm_f29_enabled();
     nv = glv ? 2 * n : n;
     W = glv ? (128 + c - 1) / c : (255 + c - 1) / c;
     wb = glv ? 128 / W : c;
@@ -592,6 +593,11 @@ bool msm_glv_enabled(size_t n) {
     // virtual point indices (2n of them) must fit the 31-bit index field of a sorted entry
     if (n > 0x3fffffffu) return false;
     const char* e = std::getenv("ZKGPU_MSM_GLV");
+    return !(e && e[0] == '0');
+}
+
+bool msm_f29_enabled() {
+    const char* e = std::getenv("ZKGPU_MSM_F29");
     return !(e && e[0] == '0');
 }
 

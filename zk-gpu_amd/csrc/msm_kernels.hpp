@@ -3,8 +3,10 @@
 // by the translation units that instantiate them (msm_g1.hip, msm_g2_*.hip).
 #pragma once
 #include <algorithm>
+#include <type_traits>
 
 #include "curve.hpp"
+#include "field29.hpp"
 #include "msm.hpp"
 #include "msm_stages.hpp"
 
@@ -51,6 +53,95 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     }
     store_xyzz(item_pts + (size_t)(4 * Tr::WORDS) * item, acc);
     // (position in the bucket, items in the bucket) for the merge rounds
+    item_info[item] = make_uint2(item - istarts[slot], istarts[slot + 1] - istarts[slot]);
+}
+
+// G1 accumulation with the field in nine 29-bit limbs (field29.hpp): the same work split, the same
+// point law (madd-2008-s with every special case) and the same output (canonical R = 2^256 XYZZ
+// words) as msm_accumulate_kernel<Fq>. Bases are read in R = 2^256 form and taken as 32x their
+// value, which is the R = 2^261 form unreduced (below 32p). Value bounds in units of p, kept
+// between points: X < 9, Y < 5, ZZ < 2, ZZZ < 2; every product's operand bounds multiply to at
+// most 144 (the multiplication allows 169). A template so that the header defines it once per
+// instantiation (only Fe = Fq).
+template <class Fe>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void msm_accumulate29_kernel(
+                                                            const uint32_t* __restrict__ bases,
+                                                            const uint32_t* __restrict__ endo_x, uint32_t n_real,
+                                                            const uint32_t* __restrict__ sorted,
+                                                            const uint32_t* __restrict__ starts,
+                                                            const uint32_t* __restrict__ istarts,
+                                                            const uint32_t* __restrict__ counts, uint32_t S,
+                                                            uint32_t ch, uint2* __restrict__ item_info,
+                                                            uint32_t* __restrict__ item_pts) {
+    static_assert(std::is_same<Fe, Fq>::value, "29-bit limbs are for the base field of G1");
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= istarts[S]) return;
+    uint32_t lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (istarts[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t slot = lo;
+    const uint32_t skip = (item - istarts[slot]) * ch;
+    const uint32_t len = min(ch, counts[slot] - skip);
+    const uint32_t* entries = sorted + starts[slot] + skip;
+    const F29 one = f29_const_one();
+    F29 X = one, Y = one, ZZ = one, ZZZ = one;
+    bool inf = true;
+    for (uint32_t j = 0; j < len; ++j) {
+        const uint32_t e = entries[j], v = e & 0x7fffffffu;
+        const bool endo = v >= n_real;
+        const uint32_t idx = endo ? v - n_real : v;
+        const uint32_t* yp = bases + (size_t)16 * idx + 8;
+        const uint32_t* xp = endo ? endo_x + (size_t)8 * idx : yp - 8;
+        const Fq qx = fp_load<FqParams>(xp), qy = fp_load<FqParams>(yp);
+        if (fp_is_zero(qx) && fp_is_zero(qy)) continue;  // the point at infinity
+        const F29 xs = f29_from_r256_words(qx.w), ys = f29_from_r256_words(qy.w);  // < 32
+        const bool neg = e >> 31;
+        if (inf) {
+            X = f29_mul(xs, one);  // < 2
+            Y = f29_mul(ys, one);  // < 2
+            if (neg) Y = f29_neg<3>(Y);  // < 3
+            ZZ = one;
+            ZZZ = one;
+            inf = false;
+            continue;
+        }
+        const F29 u2 = f29_mul(xs, ZZ);    // 32 * 2 -> < 2
+        F29 s2 = f29_mul(ys, ZZZ);         // < 2
+        if (neg) s2 = f29_neg<3>(s2);      // < 3
+        const F29 pp = f29_sub<10>(u2, X);  // < 12
+        const F29 r = f29_sub<6>(s2, Y);    // < 9
+        if (f29_is_zero_below12p(pp)) {
+            if (f29_is_zero_below12p(r)) {  // the same point: double (rare; in the 32-bit field)
+                Xyzz<Fq> a{f29_to_fq(X), f29_to_fq(Y), f29_to_fq(ZZ), f29_to_fq(ZZZ)};
+                a = xyzz_dbl(a);
+                X = f29_from_fq(a.x);
+                Y = f29_from_fq(a.y);
+                ZZ = f29_from_fq(a.zz);
+                ZZZ = f29_from_fq(a.zzz);
+            } else {
+                inf = true;  // P + (-P)
+            }
+            continue;
+        }
+        const F29 ppsq = f29_mul(pp, pp);              // 12 * 12 -> < 2
+        const F29 ppp = f29_mul(pp, ppsq);             // 12 * 2 -> < 2
+        const F29 q = f29_mul(X, ppsq);                // 9 * 2 -> < 2
+        const F29 rr = f29_mul(r, r);                  // 9 * 9 -> < 2
+        const F29 t = f29_add(ppp, f29_add(q, q));     // < 6, limbs < 3 * 2^29
+        const F29 x3 = f29_sub<7>(rr, t);              // < 9
+        const F29 d = f29_sub<10>(q, x3);              // < 12
+        const F29 y3 = f29_sub<3>(f29_mul(r, d), f29_mul(Y, ppp));  // 9 * 12, 5 * 2 -> < 5
+        ZZ = f29_mul(ZZ, ppsq);                        // < 2
+        ZZZ = f29_mul(ZZZ, ppp);                       // < 2
+        X = x3;
+        Y = y3;
+    }
+    Xyzz<Fq> out = xyzz_inf<Fq>();
+    if (!inf) out = Xyzz<Fq>{f29_to_fq(X), f29_to_fq(Y), f29_to_fq(ZZ), f29_to_fq(ZZZ)};
+    store_xyzz(item_pts + (size_t)32 * item, out);
     item_info[item] = make_uint2(item - istarts[slot], istarts[slot + 1] - istarts[slot]);
 }
 
@@ -283,6 +374,14 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
 
 template <class Fe>
 void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
+    if constexpr (std::is_same<Fe, Fq>::value) {
+        if (p.f29) {
+            hipLaunchKernelGGL(msm_accumulate29_kernel<Fq>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
+                               (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
+                               p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
+            return;
+        }
+    }
     hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
                        (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts, p.counts,
                        p.W * p.B, p.CH, p.item_info, p.item_pts);
