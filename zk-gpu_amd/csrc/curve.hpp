@@ -38,6 +38,7 @@ struct FeTraits<Fq> {
     static ZKG_DEV Fq sqr(const Fq& x) { return fp_sqr(x); }
     static ZKG_DEV Fq load(const uint32_t* p) { return fp_load<FqParams>(p); }
     static ZKG_DEV void store(uint32_t* p, const Fq& x) { fp_store(p, x); }
+    static ZKG_DEV void store_result(uint32_t* p, const Fq& x) { fp_store(p, x); }
     static ZKG_DEV Fq to_mont(const Fq& x) { return fp_to_mont(x); }
     static ZKG_DEV Fq from_mont(const Fq& x) { return fp_from_mont(x); }
 };
@@ -57,6 +58,7 @@ struct FeTraits<Fq2> {
         fp_store(p, x.a);
         fp_store(p + 8, x.b);
     }
+    static ZKG_DEV void store_result(uint32_t* p, const Fq2& x) { store(p, x); }
     static ZKG_DEV Fq2 to_mont(const Fq2& x) { return Fq2{fp_to_mont(x.a), fp_to_mont(x.b)}; }
     static ZKG_DEV Fq2 from_mont(const Fq2& x) { return Fq2{fp_from_mont(x.a), fp_from_mont(x.b)}; }
 };
@@ -208,6 +210,17 @@ ZKG_DEV void store_xyzz(uint32_t* p, const Xyzz<Fe>& v) {
     T::store(p + T::WORDS, v.y);
     T::store(p + 2 * T::WORDS, v.zz);
     T::store(p + 3 * T::WORDS, v.zzz);
+}
+
+// The MSM result in the library's API form (coordinate types with another internal form
+// convert here).
+template <class Fe>
+ZKG_DEV void store_xyzz_result(uint32_t* p, const Xyzz<Fe>& v) {
+    typedef FeTraits<Fe> T;
+    T::store_result(p, v.x);
+    T::store_result(p + T::WORDS, v.y);
+    T::store_result(p + 2 * T::WORDS, v.zz);
+    T::store_result(p + 3 * T::WORDS, v.zzz);
 }
 
 }  // namespace zkg

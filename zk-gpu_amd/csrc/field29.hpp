@@ -18,6 +18,7 @@
 #pragma once
 #include <stdint.h>
 
+#include "curve.hpp"
 #include "field.hpp"
 
 namespace zkg {
@@ -41,6 +42,11 @@ ZKG_DEV constexpr uint32_t P(int i) {  // p
 template <int C>
 ZKG_DEV constexpr uint32_t K(int i);
 template <>
+ZKG_DEV constexpr uint32_t K<2>(int i) {
+    return i == 0 ? 0x90f9fa8eu : i == 1 ? 0x8208c169u : i == 2 ? 0x98e5469au : i == 3 ? 0x85aa459du
+         : i == 4 ? 0x8b0bb2ecu : i == 5 ? 0x85b6817du : i == 6 ? 0x814dc27eu : i == 7 ? 0x9cb84c64u : 0x0060c898u;
+}
+template <>
 ZKG_DEV constexpr uint32_t K<3>(int i) {
     return i == 0 ? 0x8976f7d5u : i == 1 ? 0x830d2220u : i == 2 ? 0x9557e9e9u : i == 3 ? 0x887f686eu
          : i == 4 ? 0x80918c64u : i == 5 ? 0x8891c23eu : i == 6 ? 0x81f4a3bfu : i == 7 ? 0x8b147298u : 0x00912ce7u;
@@ -59,6 +65,11 @@ template <>
 ZKG_DEV constexpr uint32_t K<10>(int i) {
     return i == 0 ? 0x94e1e4c6u : i == 1 ? 0x8a2bc71fu : i == 2 ? 0x9c7a6112u : i == 3 ? 0x9c535c24u
          : i == 4 ? 0x973a7eacu : i == 5 ? 0x9c908782u : i == 6 ? 0x8684cc86u : i == 7 ? 0x8f997e04u : 0x01e3eb0cu;
+}
+
+ZKG_DEV constexpr uint32_t TWO_P(int i) {  // 2p, normalised
+    return i == 0 ? 0x10f9fa8eu : i == 1 ? 0x0208c16du : i == 2 ? 0x18e5469eu : i == 3 ? 0x05aa45a1u
+         : i == 4 ? 0x0b0bb2f0u : i == 5 ? 0x05b68181u : i == 6 ? 0x014dc282u : i == 7 ? 0x1cb84c68u : 0x0060c89cu;
 }
 
 ZKG_DEV constexpr uint32_t ONE(int i) {  // 2^261 mod p: the Montgomery form of 1
@@ -180,8 +191,9 @@ ZKG_DEV F29 f29_from_r256_words(const uint32_t w[8]) {
 #pragma unroll
     for (int i = 1; i < 8; ++i) {
         const int o = 29 * i - 5, q = o >> 5, s = o & 31;
-        const uint64_t two = ((uint64_t)w[q + 1] << 32) | w[q];
-        r.l[i] = (uint32_t)(two >> s) & f29::MASK;
+        // bits [o, o + 29) of w: a funnel shift of two 32-bit words (v_alignbit_b32), which keeps w
+        // in registers (a 64-bit read at a 4-byte offset would put it in scratch memory)
+        r.l[i] = __builtin_amdgcn_alignbit(w[q + 1], w[q], s) & f29::MASK;
     }
     r.l[8] = w[7] >> 3;
     return r;
@@ -192,10 +204,9 @@ ZKG_DEV void f29_to_words(const F29& a, uint32_t w[8]) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int b = 32 * j, i = b / 29, s = b - 29 * i;
-        uint64_t v = (uint64_t)a.l[i] >> s;
-        v |= (uint64_t)a.l[i + 1] << (29 - s);
-        if (i + 2 <= 8 && 58 - s < 32) v |= (uint64_t)a.l[i + 2] << (58 - s);
-        w[j] = (uint32_t)v;
+        uint32_t v = (a.l[i] >> s) | (a.l[i + 1] << (29 - s));  // 29 - s <= 29: no undefined shift
+        if (i + 2 <= 8 && 58 - s < 32) v |= a.l[i + 2] << (58 - s);
+        w[j] = v;
     }
 }
 
@@ -211,6 +222,137 @@ ZKG_DEV Fq f29_to_fq(const F29& a) {
     Fq r;
     f29_to_words(t, r.w);
     fp_reduce_once<FqParams>(r.w);
+    return r;
+}
+
+// Eight 32-bit words of a value below 2^256 -> its nine limbs (no scaling).
+ZKG_DEV F29 f29_unpack(const uint32_t w[8]) {
+    F29 r;
+    r.l[0] = w[0] & f29::MASK;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        const int o = 29 * i, q = o >> 5, s = o & 31;
+        r.l[i] = __builtin_amdgcn_alignbit(w[q + 1], w[q], s) & f29::MASK;
+    }
+    r.l[8] = w[7] >> 8;
+    return r;
+}
+
+// ---------------------------------------------------------------- Fq29: reduced values
+// The same limbs with every operation returning a normalised value below 2p, so generic point
+// formulas (msm_kernels.hpp's merge, finalize, reduce, window-sum and combine kernels) can use
+// it like Fq. In memory an element is the R = 2^261 value (below 2p) in eight 32-bit words.
+struct Fq29 {
+    uint32_t l[9];
+};
+
+// limbs in [0, 2^32) of a value below 4p -> normalised, below 2p
+ZKG_DEV Fq29 fq29_reduce(F29 s) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s.l[i + 1] += s.l[i] >> 29;
+        s.l[i] &= f29::MASK;
+    }
+    Fq29 t;
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const uint32_t d = s.l[i] - f29::TWO_P(i) - br;  // in (-2^30, 2^29): the sign bit is the borrow
+        br = d >> 31;
+        t.l[i] = i < 8 ? d & f29::MASK : d;
+    }
+    Fq29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = br ? s.l[i] : t.l[i];  // borrow out: s < 2p already
+    return r;
+}
+
+ZKG_DEV Fq29 operator*(const Fq29& a, const Fq29& b) {  // 2p * 2p < 169 p^2: below 2p
+    F29 x, y;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        x.l[i] = a.l[i];
+        y.l[i] = b.l[i];
+    }
+    const F29 m = f29_mul(x, y);
+    Fq29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = m.l[i];
+    return r;
+}
+
+ZKG_DEV Fq29 operator+(const Fq29& a, const Fq29& b) {
+    F29 s;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s.l[i] = a.l[i] + b.l[i];
+    return fq29_reduce(s);
+}
+
+// a + 2p - b in (0, 4p): limbs of 2p borrowed so none goes negative (the top limb may wrap by at
+// most 4 and is made whole again by the carries of the borrowed low limbs)
+ZKG_DEV Fq29 operator-(const Fq29& a, const Fq29& b) {
+    F29 s;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s.l[i] = a.l[i] + f29::K<2>(i) - b.l[i];
+    return fq29_reduce(s);
+}
+
+ZKG_DEV bool fq29_is_zero(const Fq29& a) {  // 0 or p
+    uint32_t z = 0, q = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        z |= a.l[i];
+        q |= a.l[i] ^ f29::P(i);
+    }
+    return z == 0 || q == 0;
+}
+
+template <>
+struct FeTraits<Fq29> {
+    static constexpr int WORDS = 8;
+    static ZKG_DEV Fq29 zero() {
+        Fq29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = 0;
+        return r;
+    }
+    static ZKG_DEV Fq29 one() {
+        Fq29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = f29::ONE(i);
+        return r;
+    }
+    static ZKG_DEV bool is_zero(const Fq29& x) { return fq29_is_zero(x); }
+    static ZKG_DEV Fq29 sqr(const Fq29& x) { return x * x; }
+    static ZKG_DEV Fq29 load(const uint32_t* p) {
+        const Fq w = fp_load<FqParams>(p);
+        const F29 u = f29_unpack(w.w);
+        Fq29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = u.l[i];
+        return r;
+    }
+    static ZKG_DEV void store(uint32_t* p, const Fq29& x) {
+        F29 u;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) u.l[i] = x.l[i];
+        Fq w;
+        f29_to_words(u, w.w);
+        fp_store(p, w);
+    }
+    // the library's result form: canonical R = 2^256 words
+    static ZKG_DEV void store_result(uint32_t* p, const Fq29& x) {
+        F29 u;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) u.l[i] = x.l[i];
+        fp_store(p, f29_to_fq(u));
+    }
+};
+
+ZKG_DEV Fq29 fe_select(bool c, const Fq29& a, const Fq29& b) {
+    Fq29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = c ? a.l[i] : b.l[i];
     return r;
 }
 

@@ -26,12 +26,14 @@
 #include <cstring>
 
 #include "curve.hpp"
+#include "field29.hpp"
 #include "msm.hpp"
 #include "msm_stages.hpp"
 
 namespace zkg {
 
 ZKG_MSM_EXTERN(Fq)
+ZKG_MSM_EXTERN(Fq29)
 ZKG_MSM_EXTERN(Fq2)
 
 uint32_t msm_default_window(size_t n) {
@@ -715,13 +717,15 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
 
 void MsmPlan::run(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream,
                   std::vector<std::pair<std::string, double>>* stage_ms) const {
-    if (group == 1) run_stages<Fq>(*this, bases, scalars, stream, stage_ms);
-    else run_stages<Fq2>(*this, bases, scalars, stream, stage_ms);
+    if (group == 2) run_stages<Fq2>(*this, bases, scalars, stream, stage_ms);
+    else if (f29) run_stages<Fq29>(*this, bases, scalars, stream, stage_ms);
+    else run_stages<Fq>(*this, bases, scalars, stream, stage_ms);
 }
 
 void MsmPlan::run_yielding(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream) const {
-    if (group == 1) run_stages<Fq>(*this, bases, scalars, stream, nullptr, true);
-    else run_stages<Fq2>(*this, bases, scalars, stream, nullptr, true);
+    if (group == 2) run_stages<Fq2>(*this, bases, scalars, stream, nullptr, true);
+    else if (f29) run_stages<Fq29>(*this, bases, scalars, stream, nullptr, true);
+    else run_stages<Fq>(*this, bases, scalars, stream, nullptr, true);
 }
 
 }  // namespace zkg

@@ -56,13 +56,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     item_info[item] = make_uint2(item - istarts[slot], istarts[slot + 1] - istarts[slot]);
 }
 
-// G1 accumulation with the field in nine 29-bit limbs (field29.hpp): the same work split, the same
-// point law (madd-2008-s with every special case) and the same output (canonical R = 2^256 XYZZ
-// words) as msm_accumulate_kernel<Fq>. Bases are read in R = 2^256 form and taken as 32x their
+// G1 accumulation with the field in nine 29-bit limbs (field29.hpp): the same work split and the
+// same point law (madd-2008-s with every special case) as msm_accumulate_kernel<Fq>; the item sums
+// are written in Fq29's memory form for the Fq29 instantiation of the later stages. Bases are read in R = 2^256 form and taken as 32x their
 // value, which is the R = 2^261 form unreduced (below 32p). Value bounds in units of p, kept
 // between points: X < 9, Y < 5, ZZ < 2, ZZZ < 2; every product's operand bounds multiply to at
 // most 144 (the multiplication allows 169). A template so that the header defines it once per
-// instantiation (only Fe = Fq).
+// instantiation (only Fe = Fq29, the G1 pipeline in 29-bit limbs).
 template <class Fe>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void msm_accumulate29_kernel(
                                                             const uint32_t* __restrict__ bases,
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
                                                             const uint32_t* __restrict__ counts, uint32_t S,
                                                             uint32_t ch, uint2* __restrict__ item_info,
                                                             uint32_t* __restrict__ item_pts) {
-    static_assert(std::is_same<Fe, Fq>::value, "29-bit limbs are for the base field of G1");
+    static_assert(std::is_same<Fe, Fq29>::value, "29-bit limbs are for the base field of G1");
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= istarts[S]) return;
     uint32_t lo = 0, hi = S;
@@ -139,8 +139,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
         X = x3;
         Y = y3;
     }
-    Xyzz<Fq> out = xyzz_inf<Fq>();
-    if (!inf) out = Xyzz<Fq>{f29_to_fq(X), f29_to_fq(Y), f29_to_fq(ZZ), f29_to_fq(ZZZ)};
+    // item sums in the Fq29 memory form (R = 2^261 value below 2p), read by the Fq29 stages
+    Xyzz<Fq29> out = xyzz_inf<Fq29>();
+    if (!inf) {
+        const F29 x = f29_mul(X, one), y = f29_mul(Y, one);  // value-preserving, below 2p
+        Fq29 o[4];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            o[0].l[i] = x.l[i];
+            o[1].l[i] = y.l[i];
+            o[2].l[i] = ZZ.l[i];
+            o[3].l[i] = ZZZ.l[i];
+        }
+        out = Xyzz<Fq29>{o[0], o[1], o[2], o[3]};
+    }
     store_xyzz(item_pts + (size_t)32 * item, out);
     item_info[item] = make_uint2(item - istarts[slot], istarts[slot + 1] - istarts[slot]);
 }
@@ -275,6 +287,16 @@ struct LaneOps<Fq> {
 };
 
 template <>
+struct LaneOps<Fq29> {
+    static ZKG_DEV Fq29 shfl(const Fq29& x, int src) {
+        Fq29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = __shfl(x.l[i], src);
+        return r;
+    }
+};
+
+template <>
 struct LaneOps<Fq2> {
     static ZKG_DEV Fq2 shfl(const Fq2& x, int src) { return Fq2{LaneOps<Fq>::shfl(x.a, src), LaneOps<Fq>::shfl(x.b, src)}; }
 };
@@ -365,7 +387,7 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
             if (!xyzz_is_inf(acc)) acc = xyzz_dbl_lanes(acc, lane);
         acc = xyzz_add_lanes(acc, load_xyzz<Fe>(wsums + (size_t)XW * (w + zero)), lane);
     }
-    if (lane == 0) store_xyzz(result, acc);
+    if (lane == 0) store_xyzz_result(result, acc);
 }
 
 // ---------------------------------------------------------------- stage launchers
@@ -374,17 +396,15 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
 
 template <class Fe>
 void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
-    if constexpr (std::is_same<Fe, Fq>::value) {
-        if (p.f29) {
-            hipLaunchKernelGGL(msm_accumulate29_kernel<Fq>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
-                               (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
-                               p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
-            return;
-        }
+    if constexpr (std::is_same<Fe, Fq29>::value) {
+        hipLaunchKernelGGL(msm_accumulate29_kernel<Fq29>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
+                           (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
+                           p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
+    } else {
+        hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
+                           (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
+                           p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
     }
-    hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
-                       (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts, p.counts,
-                       p.W * p.B, p.CH, p.item_info, p.item_pts);
 }
 
 template <class Fe>
