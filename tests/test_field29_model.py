@@ -222,3 +222,79 @@ def test_worst_case_normalised_inputs_multiply_in_range(bound):
     if bound * 12 < (1 << 261) / P:
         r = mul(a, b)
         assert value(r) < 2 * P
+
+
+# ---- Fq29: the reduced variant used by the generic G1 stages (every result normalised, below 2p)
+
+def two_p_limbs():
+    return header_table("TWO_P")
+
+
+def fq29_reduce(s):
+    """field29.hpp fq29_reduce: limbs in [0, 2^32), value < 4p -> normalised, value < 2p."""
+    assert all(0 <= x < 1 << 32 for x in s) and value(s) < 4 * P
+    s = list(s)
+    for i in range(8):
+        s[i + 1] = (s[i + 1] + (s[i] >> 29)) & 0xFFFFFFFF
+        s[i] &= MASK
+    assert value(s) < 4 * P
+    t, br = [0] * 9, 0
+    for i, q in enumerate(two_p_limbs()):
+        d = s[i] - q - br
+        assert -(1 << 30) < d < 1 << 29 if i < 8 else -(1 << 30) < d < 1 << 30
+        br = 1 if d < 0 else 0
+        t[i] = d & MASK if i < 8 else d & 0xFFFFFFFF
+    r = s if br else t
+    assert all(x < 1 << 29 for x in r[:8]) and value(r) < 2 * P
+    return r
+
+
+def fq29_add(a, b):
+    return fq29_reduce([x + y for x, y in zip(a, b)])
+
+
+def fq29_sub(a, b):
+    """a + 2p - b with 2p in borrowed limbs; the top limb may wrap (mod 2^32) and is restored."""
+    k = header_k(2)
+    s = [(x + y - z) & 0xFFFFFFFF for x, y, z in zip(a, k, b)]
+    assert all(0 <= x + y - z < 1 << 32 for x, y, z in zip(a[:8], k[:8], b[:8]))
+    assert k[8] - b[8] >= -4
+    r = fq29_reduce_wrapped(s)
+    assert value(r) % P == (value(a) - value(b)) % P
+    return r
+
+
+def fq29_reduce_wrapped(s):
+    # as fq29_reduce, but the top limb arrives modulo 2^32 (it may be a small negative number)
+    s = list(s)
+    for i in range(8):
+        s[i + 1] = (s[i + 1] + (s[i] >> 29)) & 0xFFFFFFFF
+        s[i] &= MASK
+    return fq29_reduce(s)
+
+
+def test_fq29_add_sub_at_the_edges():
+    edge = [0, 1, P - 1, P, P + 1, 2 * P - 1]
+    rng = random.Random(7)
+    vals = edge + [rng.randrange(2 * P) for _ in range(40)]
+    for x in vals:
+        for y in vals[:12] + vals[-3:]:
+            a, b = limbs(x), limbs(y)
+            r = fq29_add(a, b)
+            assert value(r) % P == (x + y) % P
+            d = fq29_sub(a, b)
+            assert value(d) < 2 * P
+    # the product of two reduced values stays below 2p
+    for x in edge:
+        assert value(mul(limbs(x), limbs(2 * P - 1))) < 2 * P
+
+
+def test_fq29_zero_test_and_storage_round_trip():
+    for x in (0, P, 1, P - 1, 2 * P - 1):
+        l = limbs(x)
+        assert (all(v == 0 for v in l) or l == limbs(P)) == (x % P == 0)
+        words = [(x >> (32 * j)) & 0xFFFFFFFF for j in range(8)]
+        # f29_unpack: limb i = bits [29 i, 29 i + 29) of the eight words
+        unpacked = [(sum(w << (32 * j) for j, w in enumerate(words)) >> (29 * i)) & MASK for i in range(8)]
+        unpacked.append(words[7] >> 8)
+        assert unpacked == l
