@@ -1,4 +1,5 @@
-"""The G1 bucket accumulation in 29-bit limbs (msm_accumulate29_kernel) and the 32-bit-limb kernel
+"""The MSM pipelines in 29-bit limbs (G1: msm_accumulate29_kernel and the Fq29 stages; G2: the Fq2_29
+stages) and the 32-bit-limb pipelines
 both against the oracle, on inputs that drive every special case of the accumulation: equal
 points in one bucket (doubling), a point and its negative (cancellation), points at infinity,
 zero and r - 1 scalars, and work items long enough to run the lazy bounds through many steps."""
@@ -69,6 +70,31 @@ def test_f29_closed_form_at_2_20(gpu):
             os.environ["ZKGPU_MSM_F29"] = flag
             m = Msm.for_group(gpu, Group.G1, n, 15)
             assert np.array_equal(affine(gpu, m.run(gpu, bb, sb), 1), exp), flag
+    finally:
+        if old is None:
+            os.environ.pop("ZKGPU_MSM_F29", None)
+        else:
+            os.environ["ZKGPU_MSM_F29"] = old
+
+
+@pytest.mark.parametrize("n,c", [(1500, 11), (2500, 13)])
+def test_g2_f29_and_32bit_stages_match_oracle(gpu, n, c):
+    """G2: the 32-bit accumulation feeding the Fq2-over-29-bit stages, and the all-32-bit path."""
+    from test_gpu_msm import affine
+    from zkgpu import Group, Msm
+
+    rng = np.random.default_rng(1420 + n)
+    ks, sc = _inputs(rng, n)
+    bases = O.multiples(2, O.generator_words(2), ks)
+    exp = O.msm(2, bases, sc)
+    bb = Msm.upload_bases(gpu, bases)
+    sb = gpu.buffer(sc)
+    old = os.environ.get("ZKGPU_MSM_F29")
+    try:
+        for flag in ("1", "0"):
+            os.environ["ZKGPU_MSM_F29"] = flag
+            m = Msm.for_group(gpu, Group.G2, n, c)
+            assert np.array_equal(affine(gpu, m.run(gpu, bb, sb), 2), exp), (flag, n, c)
     finally:
         if old is None:
             os.environ.pop("ZKGPU_MSM_F29", None)

@@ -356,4 +356,59 @@ ZKG_DEV Fq29 fe_select(bool c, const Fq29& a, const Fq29& b) {
     return r;
 }
 
+// Canonical Fq (R = 2^256) -> Fq29 (R = 2^261, below 2p).
+ZKG_DEV Fq29 fq29_from_fq(const Fq& x) {
+    const F29 t = f29_from_fq(x);
+    Fq29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = t.l[i];
+    return r;
+}
+
+// ---------------------------------------------------------------- Fq2 over Fq29
+// Fq2 = Fq[u]/(u^2 + 1) as in the 32-bit Fq2 (Karatsuba product, (a+b)(a-b) and 2ab square),
+// every component a reduced Fq29: the G2 stages after the bucket accumulation.
+struct Fq2_29 {
+    Fq29 a, b;
+};
+
+ZKG_DEV Fq2_29 operator+(const Fq2_29& x, const Fq2_29& y) { return Fq2_29{x.a + y.a, x.b + y.b}; }
+ZKG_DEV Fq2_29 operator-(const Fq2_29& x, const Fq2_29& y) { return Fq2_29{x.a - y.a, x.b - y.b}; }
+ZKG_DEV Fq2_29 operator*(const Fq2_29& x, const Fq2_29& y) {
+    const Fq29 ac = x.a * y.a, bd = x.b * y.b;
+    const Fq29 m = (x.a + x.b) * (y.a + y.b);
+    return Fq2_29{ac - bd, (m - ac) - bd};
+}
+
+template <>
+struct FeTraits<Fq2_29> {
+    static constexpr int WORDS = 16;
+    static ZKG_DEV Fq2_29 zero() { return Fq2_29{FeTraits<Fq29>::zero(), FeTraits<Fq29>::zero()}; }
+    static ZKG_DEV Fq2_29 one() { return Fq2_29{FeTraits<Fq29>::one(), FeTraits<Fq29>::zero()}; }
+    static ZKG_DEV bool is_zero(const Fq2_29& x) { return fq29_is_zero(x.a) && fq29_is_zero(x.b); }
+    static ZKG_DEV Fq2_29 sqr(const Fq2_29& x) {
+        const Fq29 ab = x.a * x.b;
+        return Fq2_29{(x.a + x.b) * (x.a - x.b), ab + ab};
+    }
+    static ZKG_DEV Fq2_29 load(const uint32_t* p) {
+        return Fq2_29{FeTraits<Fq29>::load(p), FeTraits<Fq29>::load(p + 8)};
+    }
+    static ZKG_DEV void store(uint32_t* p, const Fq2_29& x) {
+        FeTraits<Fq29>::store(p, x.a);
+        FeTraits<Fq29>::store(p + 8, x.b);
+    }
+    static ZKG_DEV void store_result(uint32_t* p, const Fq2_29& x) {
+        FeTraits<Fq29>::store_result(p, x.a);
+        FeTraits<Fq29>::store_result(p + 8, x.b);
+    }
+};
+
+ZKG_DEV Fq2_29 fe_select(bool c, const Fq2_29& x, const Fq2_29& y) {
+    return Fq2_29{fe_select(c, x.a, y.a), fe_select(c, x.b, y.b)};
+}
+
+// The storage form of the Fq29 / Fq2_29 stages, from canonical 32-bit coordinates.
+ZKG_DEV Fq29 to_storage_field(const Fq& x, const Fq29*) { return fq29_from_fq(x); }
+ZKG_DEV Fq2_29 to_storage_field(const Fq2& x, const Fq2_29*) { return Fq2_29{fq29_from_fq(x.a), fq29_from_fq(x.b)}; }
+
 }  // namespace zkg

@@ -35,6 +35,7 @@ namespace zkg {
 ZKG_MSM_EXTERN(Fq)
 ZKG_MSM_EXTERN(Fq29)
 ZKG_MSM_EXTERN(Fq2)
+ZKG_MSM_EXTERN(Fq2_29)
 
 uint32_t msm_default_window(size_t n) {
     size_t m = n < 1 ? 1 : n;
@@ -521,7 +522,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     n = (uint32_t)npts;
     c = win;
     glv = msm_glv_enabled(npts);
-    f29 = grp == 1 && msm_f29_enabled();
+    f29 = msm_f29_enabled();
     nv = glv ? 2 * n : n;
     W = glv ? (128 + c - 1) / c : (255 + c - 1) / c;
     wb = glv ? 128 / W : c;
@@ -717,13 +718,15 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
 
 void MsmPlan::run(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream,
                   std::vector<std::pair<std::string, double>>* stage_ms) const {
-    if (group == 2) run_stages<Fq2>(*this, bases, scalars, stream, stage_ms);
+    if (group == 2 && f29) run_stages<Fq2_29>(*this, bases, scalars, stream, stage_ms);
+    else if (group == 2) run_stages<Fq2>(*this, bases, scalars, stream, stage_ms);
     else if (f29) run_stages<Fq29>(*this, bases, scalars, stream, stage_ms);
     else run_stages<Fq>(*this, bases, scalars, stream, stage_ms);
 }
 
 void MsmPlan::run_yielding(const uint32_t* bases, const uint32_t* scalars, hipStream_t stream) const {
-    if (group == 2) run_stages<Fq2>(*this, bases, scalars, stream, nullptr, true);
+    if (group == 2 && f29) run_stages<Fq2_29>(*this, bases, scalars, stream, nullptr, true);
+    else if (group == 2) run_stages<Fq2>(*this, bases, scalars, stream, nullptr, true);
     else if (f29) run_stages<Fq29>(*this, bases, scalars, stream, nullptr, true);
     else run_stages<Fq>(*this, bases, scalars, stream, nullptr, true);
 }

@@ -20,8 +20,9 @@ uint32_t msm_plan_item_points(size_t n, uint32_t c);
 size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0);
 // Whether plans use the GLV split (default on; ZKGPU_MSM_GLV=0 turns it off for comparisons).
 bool msm_glv_enabled(size_t n);
-// Whether G1 plans accumulate buckets in the 29-bit-limb representation (default on;
-// ZKGPU_MSM_F29=0 selects the 32-bit-limb kernel, for comparisons). Same results either way.
+// Whether plans use the 29-bit-limb field (field29.hpp): the whole G1 pipeline, and the G2 stages
+// after the bucket accumulation (default on; ZKGPU_MSM_F29=0 selects the 32-bit-limb kernels,
+// for comparisons). Same results either way.
 bool msm_f29_enabled();
 
 struct MsmPlan {
@@ -32,7 +33,7 @@ struct MsmPlan {
     // so the MSM runs over nv = 2n virtual points (P_i with k1_i, phi(P_i) with k2_i) with
     // 128-bit scalars: half the windows, half the bucket reduction and half the doubling chain.
     bool glv = false;
-    bool f29 = false;        // G1 accumulation in 29-bit limbs (field29.hpp)
+    bool f29 = false;        // 29-bit-limb field (field29.hpp)
     uint32_t nv = 0;         // virtual points: 2n with GLV, else n
     uint32_t W = 0;          // windows, ceil(128 / c) with GLV, else ceil(255 / c)
     // Window w covers bits [w*wb + min(w, nw), +wb + (w < nw)): without GLV wb = c, nw = 0; with

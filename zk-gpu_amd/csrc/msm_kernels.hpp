@@ -16,7 +16,9 @@ namespace zkg {
 
 // Sorted entries name virtual points v | sign << 31: v < n_real is base v, v >= n_real (GLV) is
 // phi(base v - n_real) = (endo_x[v - n_real], y). Without GLV n_real is 2^32 - 1.
-template <class Fe>
+// Out: the coordinate type of the later stages; item sums are stored in its memory form (the
+// 32-bit G2 accumulation feeding the Fq2_29 stages converts each coordinate once per item).
+template <class Fe, class Out = Fe>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void msm_accumulate_kernel(
                                                             const uint32_t* __restrict__ bases,
                                                             const uint32_t* __restrict__ endo_x, uint32_t n_real,
@@ -51,7 +53,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         if (e >> 31) p.y = Tr::zero() - p.y;
         acc = xyzz_madd(acc, p);
     }
-    store_xyzz(item_pts + (size_t)(4 * Tr::WORDS) * item, acc);
+    if constexpr (std::is_same<Fe, Out>::value) {
+        store_xyzz(item_pts + (size_t)(4 * Tr::WORDS) * item, acc);
+    } else {
+        static_assert(4 * FeTraits<Out>::WORDS == 4 * Tr::WORDS, "same memory layout");
+        const Out* tag = nullptr;
+        Xyzz<Out> o = xyzz_inf<Out>();
+        if (!xyzz_is_inf(acc))
+            o = Xyzz<Out>{to_storage_field(acc.x, tag), to_storage_field(acc.y, tag), to_storage_field(acc.zz, tag),
+                          to_storage_field(acc.zzz, tag)};
+        store_xyzz(item_pts + (size_t)(4 * Tr::WORDS) * item, o);
+    }
     // (position in the bucket, items in the bucket) for the merge rounds
     item_info[item] = make_uint2(item - istarts[slot], istarts[slot + 1] - istarts[slot]);
 }
@@ -297,6 +309,13 @@ struct LaneOps<Fq29> {
 };
 
 template <>
+struct LaneOps<Fq2_29> {
+    static ZKG_DEV Fq2_29 shfl(const Fq2_29& x, int src) {
+        return Fq2_29{LaneOps<Fq29>::shfl(x.a, src), LaneOps<Fq29>::shfl(x.b, src)};
+    }
+};
+
+template <>
 struct LaneOps<Fq2> {
     static ZKG_DEV Fq2 shfl(const Fq2& x, int src) { return Fq2{LaneOps<Fq>::shfl(x.a, src), LaneOps<Fq>::shfl(x.b, src)}; }
 };
@@ -396,7 +415,11 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
 
 template <class Fe>
 void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
-    if constexpr (std::is_same<Fe, Fq29>::value) {
+    if constexpr (std::is_same<Fe, Fq2_29>::value) {
+        hipLaunchKernelGGL((msm_accumulate_kernel<Fq2, Fq2_29>), dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s,
+                           bases, (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
+                           p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
+    } else if constexpr (std::is_same<Fe, Fq29>::value) {
         hipLaunchKernelGGL(msm_accumulate29_kernel<Fq29>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
                            (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
                            p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
