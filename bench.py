@@ -658,7 +658,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "u32 (exact BN254 modular arithmetic; 8x32-bit limbs, G1 bucket accumulation in 9x29-bit limbs)",
+        "dtype": "u32 (exact BN254 modular arithmetic; MSM in 9x29-bit limbs except the G2 bucket accumulation, NTT in 8x32-bit limbs)",
         "data": "synthetic, seeded (numpy PCG64 seed %d); bases k_i*G generated on the device" % args.seed,
         "config": {
             "workload": "plain single-call path: G1 MSM n=2^20 c=15 (value); G2 MSM 2^18 c=13; NTT 2^20/2^22/2^24",
