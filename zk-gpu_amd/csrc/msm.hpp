@@ -24,6 +24,8 @@ bool msm_glv_enabled(size_t n);
 // after the bucket accumulation (default on; ZKGPU_MSM_F29=0 selects the 32-bit-limb kernels,
 // for comparisons). Same results either way.
 bool msm_f29_enabled();
+// Counting-sort block size (points) a plan over nv virtual points with window c uses by default.
+uint32_t msm_default_sort_points(size_t nv, uint32_t c);
 
 struct MsmPlan {
     Device* dev = nullptr;

@@ -235,9 +235,11 @@ MsmTune tune_msm(Device* d, int group, size_t n) {
         cand.ms = measure(cand);
         if (cand.ms < best.ms) best = cand;
     }
-    // 3. points per counting-sort block
-    const uint32_t auto_sp = std::max<uint32_t>(1u << (best.c - 1), 8192);
-    for (uint32_t sp : {auto_sp * 2, auto_sp * 4}) {
+    // 3. points per counting-sort block around the default one
+    const uint32_t B = 1u << (best.c - 1);
+    const uint32_t auto_sp = msm_default_sort_points(msm_glv_enabled(n) ? 2 * n : n, best.c);
+    for (uint32_t sp : {auto_sp / 2, auto_sp * 2, auto_sp * 4}) {
+        if (sp < B) continue;
         MsmTune cand = best;
         cand.sort_points = sp;
         cand.ms = measure(cand);
