@@ -420,13 +420,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_block_kernel(const uint32_t
                                                                  uint32_t* __restrict__ istarts,
                                                                  uint32_t* __restrict__ block_sums) {
     __shared__ uint32_t sp[SCAN_THREADS], si[SCAN_THREADS];
+    __shared__ uint32_t bmax;
     const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
-    uint32_t cp[SCAN_PER_THREAD], tp = 0, ti = 0;
+    uint32_t cp[SCAN_PER_THREAD], tp = 0, ti = 0, mi = 0;
+    if (threadIdx.x == 0) bmax = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         cp[k] = base + k < S ? counts[base + k] : 0;
         tp += cp[k];
         ti += items_of(cp[k], ch);
+        mi = max(mi, items_of(cp[k], ch));
     }
     sp[threadIdx.x] = tp;
     si[threadIdx.x] = ti;
@@ -456,6 +459,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_block_kernel(const uint32_t
         block_sums[2 * blockIdx.x] = sp[threadIdx.x];
         block_sums[2 * blockIdx.x + 1] = si[threadIdx.x];
     }
+    // the most work items of any slot in this block (block_sums[2 nb + block])
+    atomicMax(&bmax, mi);
+    __syncthreads();
+    if (threadIdx.x == 0) block_sums[2 * gridDim.x + blockIdx.x] = bmax;
 }
 
 // Exclusive scan of `nb` (points, items) pairs in place; grand totals go to starts[S], istarts[S].
@@ -463,12 +470,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_top_kernel(uint32_t* __rest
                                                                uint32_t S, uint32_t* __restrict__ starts,
                                                                uint32_t* __restrict__ istarts) {
     __shared__ uint32_t sp[SCAN_THREADS], si[SCAN_THREADS];
+    __shared__ uint32_t smax;
     const uint32_t per = (nb + SCAN_THREADS - 1) / SCAN_THREADS;
     const uint32_t lo = threadIdx.x * per, hi = min(lo + per, nb);
-    uint32_t tp = 0, ti = 0;
+    uint32_t tp = 0, ti = 0, mi = 0;
+    if (threadIdx.x == 0) smax = 0;
     for (uint32_t k = lo; k < hi; ++k) {
         tp += block_sums[2 * k];
         ti += block_sums[2 * k + 1];
+        mi = max(mi, block_sums[2 * nb + k]);
     }
     sp[threadIdx.x] = tp;
     si[threadIdx.x] = ti;
@@ -496,6 +506,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_top_kernel(uint32_t* __rest
         starts[S] = sp[threadIdx.x];
         istarts[S] = si[threadIdx.x];
     }
+    // block_sums[3 nb] = the most work items of any slot: merge rounds that no bucket needs exit
+    atomicMax(&smax, mi);
+    __syncthreads();
+    if (threadIdx.x == 0) block_sums[3 * nb] = smax;
 }
 
 __global__ __launch_bounds__(256) void scan_add_kernel(const uint32_t* __restrict__ block_sums, uint32_t S,
@@ -557,7 +571,8 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
         ZKG_HIP(hipMalloc(&cursors, 4 * S));
         ZKG_HIP(hipMalloc(&starts, 4 * (S + 1)));
         ZKG_HIP(hipMalloc(&istarts, 4 * (S + 1)));
-        ZKG_HIP(hipMalloc(&scan_tmp, 8 * nb));
+        ZKG_HIP(hipMalloc(&scan_tmp, 4 * (3 * (size_t)nb + 1)));
+        max_items_dev = scan_tmp + 3 * nb;
         if (glv) {
             ZKG_HIP(hipMalloc(&split, 16 * (size_t)nv));
             ZKG_HIP(hipMalloc(&endo_x, 2 * (size_t)n * aw));
@@ -587,6 +602,7 @@ MsmPlan::~MsmPlan() {
     if (item_info) (void)hipFree(item_info);
     split = endo_x = counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_pts = buckets =
         partials = wsums = result = nullptr;
+    max_items_dev = nullptr;  // points into scan_tmp
     item_info = nullptr;
 }
 
@@ -623,7 +639,7 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
     const size_t P = sort_points ? std::max<uint32_t>(sort_points, B) : msm_default_sort_points(nv, c);
     const size_t nblk = (nv + P - 1) / P;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    return 4 * ((glv ? 4 * nv + n * xw / 2 : 0) + 4 * S + 2 + 2 * nb + (size_t)W * nv * (lds ? 2 : 1) +
+    return 4 * ((glv ? 4 * nv + n * xw / 2 : 0) + 4 * S + 2 + 3 * nb + 1 + (size_t)W * nv * (lds ? 2 : 1) +
                 (lds ? (size_t)W * nblk * B : 0) + max_items * (2 + (size_t)xw) + S * xw + (size_t)W * T * xw +
                 (size_t)W * xw + xw);
 }

@@ -61,7 +61,8 @@ struct MsmPlan {
     uint32_t* cursors = nullptr;     // W*B  scatter cursors
     uint32_t* starts = nullptr;      // W*B + 1  exclusive scan of counts (global over windows)
     uint32_t* istarts = nullptr;     // W*B + 1  exclusive scan of items per slot
-    uint32_t* scan_tmp = nullptr;    // per-block partial sums of the scan
+    uint32_t* scan_tmp = nullptr;    // per-block partial sums of the scan, per-block item maxima, the maximum
+    uint32_t* max_items_dev = nullptr;  // scan_tmp's last word: the most work items of any bucket
     uint32_t* sorted = nullptr;      // W*nv  point index | sign << 31, grouped by slot
     uint32_t* digits = nullptr;      // W*nv  bucket | sign << 31 per (window, point)  (LDS sort only)
     uint32_t* table = nullptr;       // W*nblk*B  per-block bucket counts -> offsets    (LDS sort only)

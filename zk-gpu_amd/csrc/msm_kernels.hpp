@@ -231,8 +231,11 @@ template <class Fe>
 __global__ __launch_bounds__(256) void msm_merge_kernel(uint32_t* __restrict__ item_pts,
                                                        const uint2* __restrict__ item_info,
                                                        const uint32_t* __restrict__ istarts, uint32_t S,
-                                                       uint32_t stride) {
+                                                       uint32_t stride, const uint32_t* __restrict__ max_items) {
     typedef FeTraits<Fe> Tr;
+    // no bucket needs this round (the scan recorded the most items of any bucket): exit before
+    // reading any item
+    if ((max_items[0] + stride - 1) / stride <= FINAL_GROUP) return;
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= istarts[S]) return;
     const uint2 info = item_info[item];
@@ -488,7 +491,7 @@ void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
 template <class Fe>
 void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
     hipLaunchKernelGGL(msm_merge_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, p.item_pts,
-                       p.item_info, p.istarts, p.W * p.B, stride);
+                       p.item_info, p.istarts, p.W * p.B, stride, (const uint32_t*)p.max_items_dev);
 }
 
 template <class Fe>
