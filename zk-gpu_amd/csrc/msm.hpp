@@ -27,7 +27,7 @@ bool msm_f29_enabled();
 // Counting-sort block size (points) a plan over nv virtual points with window c uses by default.
 uint32_t msm_default_sort_points(size_t nv, uint32_t c);
 // Buckets per lane of the bucket-reduction stage.
-uint32_t msm_reduce_chunk(int group);
+uint32_t msm_reduce_chunk(int group, uint32_t c);
 
 struct MsmPlan {
     Device* dev = nullptr;
