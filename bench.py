@@ -36,7 +36,7 @@ VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz, full-rate
 # rocprofv3 measurements of ntt_pass_kernel<10,256> on NTT 2^24 (profiles/ntt_pass_2^24_hbm_traffic.md,
 # profiles/valu_rate_msm_accumulate_and_ntt_pass.md): HBM bytes per launch (FETCH_SIZE scaled by the
 # copy-kernel calibration factor 2.0, WRITE_SIZE by 1.0) and VALU lane operations per launch.
-NTT24_TRAFFIC_BYTES_PER_LAUNCH = 1.616e9
+NTT24_TRAFFIC_BYTES_PER_LAUNCH = 1.611e9
 NTT24_VALU_LANE_OPS_PER_LAUNCH = 2.353e10
 
 
