@@ -195,6 +195,11 @@ int zkg_batch_captured(const zkg_batch* batch);  /* 1 if a current graph exists 
  * When A B - C vanishes on the domain, H is the exact quotient (its top coefficient is 0). */
 typedef struct zkg_groth16_h zkg_groth16_h;
 int zkg_groth16_h_create(zkg_device* dev, uint32_t log_n, zkg_groth16_h** out);
+/* The quotient in the form a proving key's H bases expect: ZKG_H_COEFFICIENTS (the above: H's
+ * coefficients, for bases tau^i Z(tau)/delta) or ZKG_H_ODD_COSET (snarkjs keys: A B - C on the odd
+ * coset w_2n H, a primitive 2n-th root times the domain, with no division; six transforms). */
+enum { ZKG_H_COEFFICIENTS = 0, ZKG_H_ODD_COSET = 1 };
+int zkg_groth16_h_create_form(zkg_device* dev, uint32_t log_n, int form, zkg_groth16_h** out);
 int zkg_groth16_h_destroy(zkg_groth16_h* plan);
 int zkg_groth16_h_run(zkg_groth16_h* plan, zkg_buffer* a, zkg_buffer* b, zkg_buffer* c);
 
@@ -240,6 +245,7 @@ typedef struct {
     uint32_t log_n;
     size_t points[5];
     size_t c_offset; /* C's scalars start at this witness element */
+    int h_form;      /* quotient jobs: ZKG_H_COEFFICIENTS or ZKG_H_ODD_COSET (H bases' convention) */
 } zkg_key_shape;
 int zkg_server_create(zkg_device* dev, size_t mem_budget_bytes, uint32_t max_inflight, zkg_server** out);
 int zkg_server_destroy(zkg_server* server);
@@ -257,6 +263,12 @@ int zkg_server_add_zkey(zkg_server* server, const zkg_zkey* zkey, int* key_id);
 int zkg_server_key_shape(zkg_server* server, int key_id, zkg_key_shape* out);
 int zkg_server_slot_bytes(zkg_server* server, uint32_t log_n, size_t* bytes);
 int zkg_server_key_slot_bytes(zkg_server* server, int key_id, size_t* bytes);
+/* A quotient job: a, b, c hold A, B, C's evaluations on the domain (2^log_n elements each,
+ * clobbered; a receives the H MSM's scalars in the key's H form, see zkg_groth16_h_create_form);
+ * then the five MSMs as for zkg_server_submit. Keys from zkg_server_add_zkey use the odd-coset
+ * form, zkg_server_add_key the coefficient form. */
+int zkg_server_submit_groth16(zkg_server* server, int key_id, zkg_buffer* a, zkg_buffer* b, zkg_buffer* c,
+                              const zkg_buffer* witness, uint64_t* job_id);
 /* witness must hold max(points A, B1, B2, c_offset + points C) scalars. */
 int zkg_server_submit(zkg_server* server, int key_id, zkg_buffer* const polys[7], uint32_t inverse_mask,
                       const zkg_buffer* witness, uint64_t* job_id);

@@ -238,6 +238,23 @@ def groth16_h(a_words, b_words, c_words, log_n: int, g: int = 5) -> np.ndarray:
     return ints_to_words(out)
 
 
+def groth16_h_odd_coset(a_words, b_words, c_words, log_n: int) -> np.ndarray:
+    """A B - C on the odd coset w_2n*{w^i} (w_2n a primitive 2n-th root of unity, generator 5), from
+    evaluations of A, B, C on {w^i}: inverse transforms, coefficients times w_2n^i, forward
+    transforms, a b - c pointwise. These are the H scalars of a prover whose H bases carry the
+    division by Z (snarkjs's groth16 prover; the reference has no such step, README.md:224-225)."""
+    g = py_root_of_unity(log_n + 1)
+    coeffs = [words_to_ints(ntt(w, log_n, inverse=True)) for w in (a_words, b_words, c_words)]
+    evals = []
+    for cs in coeffs:
+        gi, out = 1, []
+        for x in cs:
+            out.append(x * gi % R)
+            gi = gi * g % R
+        evals.append(words_to_ints(ntt(ints_to_words(out), log_n)))
+    return ints_to_words([(x * y - z) % R for x, y, z in zip(*evals)])
+
+
 def poly_eval(words, x: int) -> int:
     coeffs = _u32(words)
     out = np.zeros(8, dtype=np.uint32)

@@ -71,3 +71,19 @@ def test_h_rejects_aliased_and_short_buffers(gpu):
         plan.run(gpu, x, x, y)
     with pytest.raises(ZkgError, match="data buffer holds"):
         plan.run(gpu, x, y, short)
+
+
+@pytest.mark.parametrize("log_n", [10, 12])
+def test_h_odd_coset_matches_oracle(gpu, log_n):
+    """The snarkjs form: A B - C on the odd coset, word for word, for a random and a satisfied
+    system (where it is -2 H(t) of the coefficient form, checked on the oracle side)."""
+    from zkgpu import Groth16H, groth16
+
+    rng = np.random.default_rng(1330 + log_n)
+    n = 1 << log_n
+    plan = Groth16H(gpu, log_n, groth16.ODD_COSET)
+    for satisfied in (False, True):
+        a, b, c = _satisfied(rng, n) if satisfied else (uniform_fr_words(rng, n) for _ in range(3))
+        bufs = [gpu.buffer(x) for x in (a, b, c)]
+        plan.run(gpu, *bufs)
+        assert np.array_equal(gpu.read(bufs[0], 8 * n), O.groth16_h_odd_coset(a, b, c, log_n)), (log_n, satisfied)

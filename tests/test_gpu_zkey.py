@@ -52,7 +52,7 @@ def test_server_serves_a_zkey_key_next_to_a_default_key(gpu, zkey_path):
     zid = server.add_zkey(z)
     shape = server.key_shape(zid)
     assert shape == {"log_n": LOG_N, "points": (N_VARS, N_VARS, N_VARS - N_PUBLIC - 1, 1 << LOG_N, N_VARS),
-                     "c_offset": N_PUBLIC + 1}
+                     "c_offset": N_PUBLIC + 1, "h_form": 1}  # snarkjs's H: the odd-coset form
     assert server.key_slot_bytes(zid) > server.slot_bytes(LOG_N)  # nVars > 2^log_n
     z.close()  # the server holds its own copy of the bases
     plain = _Key(gpu, rng, LOG_N)

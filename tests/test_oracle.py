@@ -239,3 +239,31 @@ def test_groth16_h_is_the_quotient_for_satisfied_systems():
     c2 = c.copy()
     c2[:8] = O.int_to_words((O.words_to_int(c[:8]) + 1) % O.R)  # one unsatisfied constraint
     assert not np.array_equal(O.groth16_h(a, b, c2, log_n, g=7), O.groth16_h(a, b, c2, log_n))
+
+
+def test_groth16_h_odd_coset_by_horner_and_against_the_coefficient_form():
+    """The odd-coset values are A B - C at t_j = w_2n w^j (w_2n^2 = w; Horner on the coefficients,
+    independent of the transform), and for a satisfied system t_j^n = -1 makes them -2 H(t_j)
+    with H the coefficient-form quotient."""
+    rng = np.random.default_rng(78)
+    log_n = 10
+    n = 1 << log_n
+    a = uniform_fr_words(rng, n)
+    b = uniform_fr_words(rng, n)
+    c = uniform_fr_words(rng, n)
+    w2n = O.py_root_of_unity(log_n + 1)
+    w = O.py_root_of_unity(log_n)
+    assert pow(w2n, 2, O.R) == w and pow(w2n, n, O.R) == O.R - 1
+    odd = O.groth16_h_odd_coset(a, b, c, log_n)
+    ca, cb, cc = (O.ntt(x, log_n, inverse=True) for x in (a, b, c))
+    for j in (0, 1, 5, n - 1):
+        t = w2n * pow(w, j, O.R) % O.R
+        exp = (O.poly_eval(ca, t) * O.poly_eval(cb, t) - O.poly_eval(cc, t)) % O.R
+        assert O.words_to_int(odd[8 * j: 8 * j + 8]) == exp, j
+    # satisfied system: c = a b on the domain
+    c = O.ints_to_words([x * y % O.R for x, y in zip(O.words_to_ints(a), O.words_to_ints(b))])
+    odd = O.groth16_h_odd_coset(a, b, c, log_n)
+    h = O.groth16_h(a, b, c, log_n)
+    for j in (0, 7, n - 2):
+        t = w2n * pow(w, j, O.R) % O.R
+        assert O.words_to_int(odd[8 * j: 8 * j + 8]) == (-2 * O.poly_eval(h, t)) % O.R, j

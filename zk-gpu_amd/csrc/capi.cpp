@@ -634,10 +634,13 @@ int zkg_server_add_key(zkg_server* server, uint32_t log_n, zkg_buffer* const g1[
 int zkg_server_add_key_shaped(zkg_server* server, const zkg_key_shape* shape, zkg_buffer* const g1[4], zkg_buffer* g2,
                               int* key_id) {
     ZKG_REQUIRE(shape, "null argument");
+    if (shape->h_form != ZKG_H_COEFFICIENTS && shape->h_form != ZKG_H_ODD_COSET)
+        return zkg::fail(zkg::ZKG_ERR_INVALID, "unknown H form");
     zkg::KeyShape k;
     k.log_n = shape->log_n;
     for (int m = 0; m < 5; ++m) k.points[m] = shape->points[m];
     k.c_offset = shape->c_offset;
+    k.h_form = static_cast<zkg::HForm>(shape->h_form);
     return add_key_checked(server, k, g1, g2, key_id);
 }
 
@@ -653,6 +656,7 @@ int zkg_server_add_zkey(zkg_server* server, const zkg_zkey* zkey, int* key_id) {
     const int sections[5] = {zkg::ZKEY_A, zkg::ZKEY_B1, zkg::ZKEY_C, zkg::ZKEY_H, zkg::ZKEY_B2};
     for (int m = 0; m < 5; ++m) shape.points[m] = z.points(sections[m]);
     shape.c_offset = (size_t)z.n_public + 1;
+    shape.h_form = zkg::HForm::odd_coset;  // snarkjs's H bases
     std::vector<zkg::Buffer*> owned;
     ZKG_GUARD_BEGIN
     shape.validate();
@@ -691,6 +695,7 @@ int zkg_server_key_shape(zkg_server* server, int key_id, zkg_key_shape* out) {
     out->log_n = k.log_n;
     for (int m = 0; m < 5; ++m) out->points[m] = k.points[m];
     out->c_offset = k.c_offset;
+    out->h_form = (int)k.h_form;
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
@@ -733,6 +738,30 @@ int zkg_server_submit(zkg_server* server, int key_id, zkg_buffer* const polys[7]
     uint32_t* p[7];
     for (int k = 0; k < 7; ++k) p[k] = static_cast<uint32_t*>(polys[k]->b->ptr);
     *job_id = server->s->submit(key_id, p, inverse_mask, static_cast<const uint32_t*>(witness->b->ptr));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_server_submit_groth16(zkg_server* server, int key_id, zkg_buffer* a, zkg_buffer* b, zkg_buffer* c,
+                              const zkg_buffer* witness, uint64_t* job_id) {
+    ZKG_REQUIRE(server && a && b && c && witness && job_id, "null argument");
+    if (a->b->ptr == b->b->ptr || a->b->ptr == c->b->ptr || b->b->ptr == c->b->ptr)
+        return zkg::fail(zkg::ZKG_ERR_INVALID, "a, b and c must be distinct buffers");
+    ZKG_GUARD_BEGIN
+    const zkg::KeyShape shape = server->s->key_shape(key_id);
+    const size_t need = (size_t)32 << shape.log_n;
+    for (zkg_buffer* x : {a, b, c})
+        if (x->b->bytes < need)
+            return zkg::fail(zkg::ZKG_ERR_SIZE, "data buffer holds " + std::to_string(x->b->bytes) + " bytes; a 2^" +
+                                                    std::to_string(shape.log_n) + " transform needs " +
+                                                    std::to_string(need));
+    const size_t wlen = shape.witness_len();
+    if (witness->b->bytes < 32 * wlen)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "scalars buffer holds " + std::to_string(witness->b->bytes) + " bytes; " +
+                                                std::to_string(wlen) + " scalars need " + std::to_string(32 * wlen));
+    *job_id = server->s->submit_groth16(key_id, static_cast<uint32_t*>(a->b->ptr), static_cast<uint32_t*>(b->b->ptr),
+                                        static_cast<uint32_t*>(c->b->ptr),
+                                        static_cast<const uint32_t*>(witness->b->ptr));
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
@@ -931,6 +960,17 @@ int zkg_groth16_h_create(zkg_device* dev, uint32_t log_n, zkg_groth16_h** out) {
     ZKG_GUARD_BEGIN
     zkg::device_bind(dev->d);
     *out = new zkg_groth16_h{new zkg::GrothH(dev->d, log_n)};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_groth16_h_create_form(zkg_device* dev, uint32_t log_n, int form, zkg_groth16_h** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    if (log_n < 10 || log_n > 27) return zkg::fail(zkg::ZKG_ERR_INVALID, "log_n must be in 10..=27");
+    if (form != ZKG_H_COEFFICIENTS && form != ZKG_H_ODD_COSET) return zkg::fail(zkg::ZKG_ERR_INVALID, "unknown H form");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    *out = new zkg_groth16_h{new zkg::GrothH(dev->d, log_n, static_cast<zkg::HForm>(form))};
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }

@@ -478,8 +478,13 @@ NttPassArgs NttPlan::pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inv
 }
 
 void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream, const uint32_t* scale_tab) const {
+    run_with(data, inverse, stream, scale_tab, scratch);
+}
+
+void NttPlan::run_with(uint32_t* data, bool inverse, hipStream_t stream, const uint32_t* scale_tab, uint32_t* scr) const {
+    if (!scr && passes.size() > 1) throw Error(ZKG_ERR_INTERNAL, "NTT plan run without scratch");
     for (size_t i = 0; i < passes.size(); ++i)
-        launch_pass(tile_log, pass_args(i, data, scratch, inverse, scale_tab), passes[i].wgs, stream);
+        launch_pass(tile_log, pass_args(i, data, scr, inverse, scale_tab), passes[i].wgs, stream);
 }
 
 // ---------------------------------------------------------------- batch

@@ -55,6 +55,8 @@ struct NttPlan {
     // scale_tab (optional, 2^log_n Montgomery elements) replaces the inverse's n^-1 by a
     // per-element factor applied in the last pass's store (either direction).
     void run(uint32_t* data, bool inverse, hipStream_t stream, const uint32_t* scale_tab = nullptr) const;
+    // The same with the caller's n-element scratch (plans built without one share their twiddles).
+    void run_with(uint32_t* data, bool inverse, hipStream_t stream, const uint32_t* scale_tab, uint32_t* scr) const;
     // Pass i of a transform of `data` using `scr` as the out-of-place scratch.
     NttPassArgs pass_args(size_t i, uint32_t* data, uint32_t* scr, bool inverse,
                           const uint32_t* scale_tab = nullptr) const;

@@ -59,7 +59,8 @@ Server::~Server() {
 }
 
 size_t Server::slot_bytes(const KeyShape& shape) {
-    size_t bytes = (size_t)SERVE_POLYS * 32 * ((size_t)1 << shape.log_n);
+    // NTT batch scratch (7 n elements) and the quotient's scratch (n)
+    size_t bytes = (size_t)(SERVE_POLYS + 1) * 32 * ((size_t)1 << shape.log_n);
     for (int m = 0; m < SERVE_MSMS; ++m) {
         const int group = m == 4 ? 2 : 1;
         const MsmTune t = msm_config(dev_, group, shape.points[m]);
@@ -86,19 +87,50 @@ int Server::add_key(const KeyShape& shape, const uint32_t* const g1[4], const ui
 
 uint64_t Server::submit(int key, uint32_t* const polys[SERVE_POLYS], uint32_t inverse_mask, const uint32_t* witness) {
     std::unique_ptr<Job> j(new Job());
+    for (int p = 0; p < SERVE_POLYS; ++p) j->polys[p] = polys[p];
+    j->key = key;
+    j->inverse_mask = inverse_mask;
+    j->witness = witness;
+    std::lock_guard<std::mutex> g(mu_);
+    return enqueue(std::move(j));
+}
+
+uint64_t Server::submit_groth16(int key, uint32_t* a, uint32_t* b, uint32_t* c, const uint32_t* witness) {
+    std::unique_ptr<Job> j(new Job());
+    for (int p = 0; p < SERVE_POLYS; ++p) j->polys[p] = nullptr;
+    j->polys[0] = a;
+    j->polys[1] = b;
+    j->polys[2] = c;
+    j->groth16 = true;
+    j->key = key;
+    j->inverse_mask = 0;
+    j->witness = witness;
+    std::lock_guard<std::mutex> g(mu_);
+    return enqueue(std::move(j));
+}
+
+uint64_t Server::enqueue(std::unique_ptr<Job> j) {
     {
-        std::lock_guard<std::mutex> g(mu_);
+        const int key = j->key;
         if (key < 0 || (size_t)key >= keys_.size()) throw Error(ZKG_ERR_INVALID, "unknown proving key");
         if (!error_.empty()) throw Error(ZKG_ERR_DEVICE, "server stopped after an error: " + error_);
-        const size_t need = slot_bytes(keys_[key].shape);
-        if (key_bytes_ + need > budget_)
-            throw Error(ZKG_ERR_NOMEM, "a job for this key needs " + std::to_string(need) +
+        const KeyShape& shape = keys_[key].shape;
+        const size_t need = slot_bytes(shape);
+        // a quotient job needs the scale tables of its size and form, made at the first such job
+        // and shared by every key of that size and form (counted with the keys)
+        const auto tkey = std::make_pair(shape.log_n, (int)shape.h_form);
+        const size_t tbytes = j->groth16 && !h_tables_.count(tkey)
+                                  ? (size_t)(shape.h_form == HForm::coefficients ? 64 : 32) << shape.log_n
+                                  : 0;
+        if (key_bytes_ + tbytes + need > budget_)
+            throw Error(ZKG_ERR_NOMEM, "a job for this key needs " + std::to_string(need + tbytes) +
                                            " bytes of workspace, more than the server's HBM budget leaves");
+        if (tbytes) {
+            h_tables_[tkey].reset(new HTables(dev_, shape.log_n, shape.h_form));
+            key_bytes_ += tbytes;
+            peak_bytes_ = std::max(peak_bytes_, key_bytes_ + slot_bytes_total_);
+        }
         j->id = next_id_++;
-        j->key = key;
-        for (int p = 0; p < SERVE_POLYS; ++p) j->polys[p] = polys[p];
-        j->inverse_mask = inverse_mask;
-        j->witness = witness;
         j->t_submit = now_ms();
         // the job's inputs may still be written by work queued on the device stream
         ZKG_HIP(hipEventCreateWithFlags(&j->ready, hipEventDisableTiming));
@@ -182,6 +214,7 @@ std::unique_ptr<Server::Slot> Server::make_slot(const KeyShape& shape) {
         s->plans[m].reset(new MsmPlan(dev_, group, shape.points[m], t.c, t.item_points, t.sort_points));
     }
     s->ntt.reset(new NttBatch(dev_));
+    s->h_scratch.ensure((size_t)32 << log_n);
     auto tw = twiddles_.find(log_n);
     if (tw == twiddles_.end()) {
         NttShape ns;
@@ -208,6 +241,7 @@ void Server::destroy_slot(Slot* s) {
         if (s->ev_done[k]) (void)hipEventDestroy(s->ev_done[k]);
     }
     if (s->ev_ntt) (void)hipEventDestroy(s->ev_ntt);
+    s->h_scratch.release();
     if (s->host) (void)hipHostFree(s->host);
     s->host = nullptr;
 }
@@ -245,13 +279,23 @@ bool Server::try_admit(Job& j) {
 void Server::launch(Job& j) {
     Slot* s = j.slot;
     const Key& k = keys_[j.key];
-    s->ntt->clear();
-    for (int p = 0; p < SERVE_POLYS; ++p) s->ntt->add(k.shape.log_n, j.polys[p], (j.inverse_mask >> p) & 1);
     hipStream_t s0 = s->streams[0];
     for (int t = 0; t < SERVE_MSMS; ++t) ZKG_HIP(hipStreamWaitEvent(s->streams[t], j.ready, 0));
-    // NTT bundle, then H (its scalars are the first outputs of the seventh transform)
-    s->ntt->run(s0);
-    s->plans[3]->run(k.g1[3], j.polys[SERVE_POLYS - 1], s0);
+    const uint32_t* h_scalars;
+    if (j.groth16) {
+        // the quotient (six or seven transforms and the pointwise step), then H over it
+        const HTables& t = *h_tables_.at(std::make_pair(k.shape.log_n, (int)k.shape.h_form));
+        h_compute(*twiddles_.at(k.shape.log_n), static_cast<uint32_t*>(s->h_scratch.ptr), t, j.polys[0], j.polys[1],
+                  j.polys[2], s0);
+        h_scalars = j.polys[0];
+    } else {
+        // the NTT bundle, then H (its scalars are the first outputs of the seventh transform)
+        s->ntt->clear();
+        for (int p = 0; p < SERVE_POLYS; ++p) s->ntt->add(k.shape.log_n, j.polys[p], (j.inverse_mask >> p) & 1);
+        s->ntt->run(s0);
+        h_scalars = j.polys[SERVE_POLYS - 1];
+    }
+    s->plans[3]->run(k.g1[3], h_scalars, s0);
     ZKG_HIP(hipMemcpyAsync(s->host + 3 * 32, s->plans[3]->result, 4 * 32, hipMemcpyDeviceToHost, s0));
     // A, B1, C and B2 on their own streams
     const int order[4] = {0, 1, 2, 4};

@@ -20,6 +20,7 @@
 #include <thread>
 #include <vector>
 
+#include "h_poly.hpp"
 #include "msm.hpp"
 #include "ntt.hpp"
 
@@ -35,6 +36,8 @@ struct KeyShape {
     uint32_t log_n = 0;               // polynomial size 2^log_n
     size_t points[SERVE_MSMS] = {};   // A, B1, C, H (G1) and B2 (G2)
     size_t c_offset = 0;              // C's scalars start at this witness element
+    HForm h_form = HForm::coefficients;  // what quotient jobs (submit_groth16) feed the H MSM
+    // slots (workspaces) depend on the sizes only
     bool operator==(const KeyShape& o) const {
         for (int k = 0; k < SERVE_MSMS; ++k)
             if (points[k] != o.points[k]) return false;
@@ -69,6 +72,10 @@ class Server {
     KeyShape key_shape(int key);
     // Queue a job (returns its id); the buffers must stay valid until the job completes.
     uint64_t submit(int key, uint32_t* const polys[SERVE_POLYS], uint32_t inverse_mask, const uint32_t* witness);
+    // A quotient job: a, b, c are A, B, C's evaluations on the domain (2^log_n each, clobbered);
+    // the job computes the key's H form into a (h_poly.hpp), runs the H MSM on it and A, B1, C, B2
+    // on the witness.
+    uint64_t submit_groth16(int key, uint32_t* a, uint32_t* b, uint32_t* c, const uint32_t* witness);
     // Block until the job is done; copies its SERVE_RESULT_WORDS result words and latency.
     void wait(uint64_t job, uint32_t* results, double* latency_ms);
     bool done(uint64_t job);
@@ -88,6 +95,7 @@ class Server {
         KeyShape shape;
         std::unique_ptr<MsmPlan> plans[SERVE_MSMS];
         std::unique_ptr<NttBatch> ntt;
+        Scratch h_scratch;  // n elements for the quotient's transforms
         hipStream_t streams[SERVE_MSMS] = {};
         hipEvent_t ev_ntt = nullptr, ev_done[SERVE_MSMS] = {};
         uint32_t* host = nullptr;  // pinned results
@@ -100,6 +108,7 @@ class Server {
         int key;
         uint32_t* polys[SERVE_POLYS];
         uint32_t inverse_mask;
+        bool groth16 = false;  // polys[0..2] = a, b, c of a quotient job
         const uint32_t* witness;
         hipEvent_t ready = nullptr;  // recorded on the device stream at submit
         double t_submit = 0, t_done = 0;
@@ -121,6 +130,8 @@ class Server {
     std::vector<Key> keys_;
     std::vector<std::unique_ptr<Slot>> slots_;
     std::map<uint32_t, std::shared_ptr<NttPlan>> twiddles_;
+    std::map<std::pair<uint32_t, int>, std::unique_ptr<HTables>> h_tables_;  // per (log_n, form)
+    uint64_t enqueue(std::unique_ptr<Job> j);  // called with mu_ held
     std::map<uint64_t, std::unique_ptr<Job>> jobs_;
     std::deque<uint64_t> pending_;
     std::vector<uint64_t> running_;

@@ -73,6 +73,7 @@ SIGNATURES = {
     "zkg_batch_capture": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_batch_captured": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_groth16_h_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_void_pp]),
+    "zkg_groth16_h_create_form": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, c_void_pp]),
     "zkg_groth16_h_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_groth16_h_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_tune_msm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, c_u32p, c_u32p, c_u32p,
@@ -91,6 +92,8 @@ SIGNATURES = {
     "zkg_server_slot_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]),
     "zkg_server_add_key_shaped": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p * 4,
                                                  ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "zkg_server_submit_groth16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "zkg_server_add_zkey": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "zkg_server_key_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "zkg_server_key_slot_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),

@@ -39,14 +39,16 @@ class ServerStats(ctypes.Structure):
 class KeyShape(ctypes.Structure):
     """MSM sizes of a key: points = (A, B1, C, H, B2); C's scalars start at witness[c_offset]."""
 
-    _fields_ = [("log_n", ctypes.c_uint32), ("points", ctypes.c_size_t * 5), ("c_offset", ctypes.c_size_t)]
+    _fields_ = [("log_n", ctypes.c_uint32), ("points", ctypes.c_size_t * 5), ("c_offset", ctypes.c_size_t),
+                ("h_form", ctypes.c_int)]
 
     @classmethod
-    def make(cls, log_n: int, points, c_offset: int = 0) -> "KeyShape":
-        return cls(int(log_n), (ctypes.c_size_t * 5)(*[int(p) for p in points]), int(c_offset))
+    def make(cls, log_n: int, points, c_offset: int = 0, h_form: int = 0) -> "KeyShape":
+        """h_form: the H bases' convention for quotient jobs (groth16.COEFFICIENTS or ODD_COSET)."""
+        return cls(int(log_n), (ctypes.c_size_t * 5)(*[int(p) for p in points]), int(c_offset), int(h_form))
 
     def as_dict(self):
-        return {"log_n": self.log_n, "points": tuple(self.points), "c_offset": self.c_offset}
+        return {"log_n": self.log_n, "points": tuple(self.points), "c_offset": self.c_offset, "h_form": self.h_form}
 
 
 class Server:
@@ -101,6 +103,15 @@ class Server:
         check(_lib.load().zkg_server_submit(self._h, int(key), arr, int(inverse_mask), witness.handle,
                                             ctypes.byref(jid)))
         self._keep[("job", jid.value)] = (tuple(polys), witness)
+        return jid.value
+
+    def submit_groth16(self, key: int, a, b, c, witness) -> int:
+        """A quotient job: a, b, c hold A, B, C's domain evaluations (clobbered; a receives the H
+        scalars in the key's H form); then the five MSMs."""
+        jid = ctypes.c_uint64()
+        check(_lib.load().zkg_server_submit_groth16(self._h, int(key), a.handle, b.handle, c.handle, witness.handle,
+                                                    ctypes.byref(jid)))
+        self._keep[("job", jid.value)] = ((a, b, c), witness)
         return jid.value
 
     def done(self, job: int) -> bool:
