@@ -33,11 +33,12 @@ R = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 R_WORDS = np.array([(R >> (32 * i)) & 0xFFFFFFFF for i in range(8)], dtype=np.uint32)
 HBM_PEAK_GBPS = 8000.0
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz, full-rate 32-bit integer ops
-# rocprofv3 measurements of ntt_pass_kernel<10,256> on NTT 2^24 (profiles/ntt_pass_2^24_hbm_traffic.md,
+# rocprofv3 measurements of ntt_pass_kernel<11,512> on NTT 2^24 (profiles/ntt_pass_2^24_hbm_traffic.md,
 # profiles/valu_rate_msm_accumulate_and_ntt_pass.md): HBM bytes per launch (FETCH_SIZE scaled by the
 # copy-kernel calibration factor 2.0, WRITE_SIZE by 1.0) and VALU lane operations per launch.
-NTT24_TRAFFIC_BYTES_PER_LAUNCH = 1.611e9
-NTT24_VALU_LANE_OPS_PER_LAUNCH = 2.353e10
+NTT24_KERNEL = "ntt_pass_kernel<11,512>"  # the default tile at 2^24: 2^11 elements, 512 lanes
+NTT24_TRAFFIC_BYTES_PER_LAUNCH = 1.536e9
+NTT24_VALU_LANE_OPS_PER_LAUNCH = 2.044e10
 
 
 def log(msg):
@@ -658,7 +659,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "u32 (exact BN254 modular arithmetic; MSM in 9x29-bit limbs except the G2 bucket accumulation, NTT in 8x32-bit limbs)",
+        "dtype": "u32 (exact BN254 modular arithmetic; MSM in 9x29-bit limbs except the G2 bucket accumulation; NTT data in 8x32-bit words, twiddle products in 9x29-bit limbs)",
         "data": "synthetic, seeded (numpy PCG64 seed %d); bases k_i*G generated on the device" % args.seed,
         "config": {
             "workload": "plain single-call path: G1 MSM n=2^20 c=15 (value); G2 MSM 2^18 c=13; NTT 2^20/2^22/2^24",
@@ -679,7 +680,7 @@ def main():
             "device": info,
         },
         "roofline": {
-            "kernel": "ntt_pass_kernel<10,256> (NTT 2^24 forward, %d launches)" % n24["passes"],
+            "kernel": "%s (NTT 2^24 forward, %d launches)" % (NTT24_KERNEL, n24["passes"]),
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,

@@ -52,10 +52,10 @@ def test_constants_follow_from_p():
 
 # ---- limb-exact restatement with range checks
 
-def mul(a, b):
+def mul(a, b, mod=P):
     assert all(0 <= x < 1 << 30 for x in a + b), "multiplication input limb out of range"
-    pinv = (-pow(P, -1, 1 << 29)) % (1 << 29)
-    pl = limbs(P)
+    pinv = (-pow(mod, -1, 1 << 29)) % (1 << 29)
+    pl = limbs(mod)
     acc, m, r = 0, [0] * 9, [0] * 9
     for k in range(9):
         for i in range(k + 1):
@@ -77,7 +77,7 @@ def mul(a, b):
         acc >>= 29
     r[8] = acc
     assert r[8] < 1 << 32
-    assert value(r) % P == value(a) * value(b) * pow(2, -261, P) % P
+    assert value(r) % mod == value(a) * value(b) * pow(2, -261, mod) % mod
     return r
 
 
@@ -298,3 +298,56 @@ def test_fq29_zero_test_and_storage_round_trip():
         unpacked = [(sum(w << (32 * j) for j, w in enumerate(words)) >> (29 * i)) & MASK for i in range(8)]
         unpacked.append(words[7] >> 8)
         assert unpacked == l
+
+
+# ---- the NTT's twiddle product (zk-gpu_amd/csrc/ntt.hip, mul_lazy): the same 29-bit product
+# modulo r, with the element's words unpacked as they are and the twiddle's R = 2^256 words shifted
+# left by five bits (its R = 2^261 form, unreduced)
+
+R_FR = O.R
+
+
+def fr_mod_table():
+    m = re.search(r"struct Fr29Mod \{.*?P\(int i\) \{(.*?)\}", HDR, re.S)
+    return [int(x, 16) for x in re.findall(r"0x([0-9a-f]{8})u", m.group(1))]
+
+
+def words(v):
+    return [(v >> (32 * j)) & 0xFFFFFFFF for j in range(8)]
+
+
+def from_r256_words(w):  # f29_from_r256_words: limbs of 32 x
+    x = sum(v << (32 * j) for j, v in enumerate(w))
+    return limbs(x << 5)
+
+
+def unpack(w):  # f29_unpack: limbs of x
+    return limbs(sum(v << (32 * j) for j, v in enumerate(w)))
+
+
+def ntt_mul(a, t):
+    """a < 4r plain, t < r in R = 2^256 form (t = w 2^256 mod r): returns a w mod r, below 2r."""
+    r = mul(unpack(words(a)), from_r256_words(words(t)), R_FR)
+    assert all(x < 1 << 29 for x in r[:8]) and value(r) < 2 * R_FR
+    return value(r)
+
+
+def test_fr_constants_follow_from_r():
+    assert fr_mod_table() == limbs(R_FR)
+    m = re.search(r"struct Fr29Mod \{.*?PINV = 0x([0-9a-f]+)u", HDR, re.S)
+    assert int(m.group(1), 16) == (-pow(R_FR, -1, 1 << 29)) % (1 << 29)
+    assert 128 * R_FR * R_FR < R_FR << 261  # v(a) v(32 t) bound the product needs
+
+
+def test_ntt_product_is_the_twiddle_product_below_2r():
+    rng = random.Random(2412)
+    cases = [(4 * R_FR - 1, R_FR - 1), (0, R_FR - 1), (4 * R_FR - 1, 0), (2 * R_FR, 1), (R_FR, R_FR - 1),
+             ((1 << 232) - 1, (1 << 253) - 1), (4 * R_FR - 1, (R_FR - 1) // 2)]
+    cases += [(rng.randrange(4 * R_FR), rng.randrange(R_FR)) for _ in range(300)]
+    for a, t in cases:
+        w = t * pow(2, -256, R_FR) % R_FR  # the twiddle value whose R = 2^256 form is t
+        assert ntt_mul(a, t) % R_FR == a * w % R_FR, (a, t)
+    # a twiddle of 1 (t = 2^256 mod r) maps a to itself mod r: the product by w^0
+    one = pow(2, 256, R_FR)
+    for a in (0, 1, R_FR - 1, R_FR, 4 * R_FR - 1):
+        assert ntt_mul(a, one) % R_FR == a % R_FR

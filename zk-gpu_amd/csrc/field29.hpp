@@ -98,9 +98,24 @@ ZKG_DEV F29 f29_const_one() {
     return r;
 }
 
-// Montgomery product a * b * 2^-261 (FIPS). Needs limbs < 2^30 and v(a) v(b) < p 2^261; the
-// result is normalised and below a b / 2^261 + p < 2p. Column sums: at most nine products below
-// 2^60, nine below 2^58 and a carry below 2^35, i.e. below 1.3 * 2^63.
+// The moduli the 29-bit product serves: limbs (0..7 below 2^29) and -modulus^-1 mod 2^29.
+struct Fq29Mod {  // p: the base field
+    static ZKG_DEV constexpr uint32_t P(int i) { return f29::P(i); }
+    static constexpr uint32_t PINV = f29::PINV;
+};
+struct Fr29Mod {  // r: the scalar field (the NTT's); 2^261 / r is about 169.28 as for p
+    static ZKG_DEV constexpr uint32_t P(int i) {
+        return i == 0 ? 0x10000001u : i == 1 ? 0x1f0fac9fu : i == 2 ? 0x0e5c2450u : i == 3 ? 0x07d090f3u
+             : i == 4 ? 0x1585d283u : i == 5 ? 0x02db40c0u : i == 6 ? 0x00a6e141u : i == 7 ? 0x0e5c2634u : 0x0030644eu;
+    }
+    static constexpr uint32_t PINV = 0x0fffffffu;  // r = 1 + 2^28 mod 2^29
+};
+
+// Montgomery product a * b * 2^-261 (FIPS) modulo M's modulus m (p unless stated). Needs limbs
+// < 2^30 and v(a) v(b) < m 2^261; the result is normalised and below a b / 2^261 + m < 2m.
+// Column sums: at most nine products below 2^60, nine below 2^58 and a carry below 2^35, i.e.
+// below 1.3 * 2^63.
+template <class M = Fq29Mod>
 ZKG_DEV F29 f29_mul(const F29& a, const F29& b) {
     uint64_t acc = 0;
     uint32_t m[9];
@@ -110,9 +125,9 @@ ZKG_DEV F29 f29_mul(const F29& a, const F29& b) {
 #pragma unroll
         for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
 #pragma unroll
-        for (int j = 0; j < k; ++j) acc += (uint64_t)m[j] * f29::P(k - j);
-        m[k] = ((uint32_t)acc * f29::PINV) & f29::MASK;
-        acc += (uint64_t)m[k] * f29::P(0);  // clears the low 29 bits
+        for (int j = 0; j < k; ++j) acc += (uint64_t)m[j] * M::P(k - j);
+        m[k] = ((uint32_t)acc * M::PINV) & f29::MASK;
+        acc += (uint64_t)m[k] * M::P(0);  // clears the low 29 bits
         acc >>= 29;
     }
 #pragma unroll
@@ -120,7 +135,7 @@ ZKG_DEV F29 f29_mul(const F29& a, const F29& b) {
 #pragma unroll
         for (int i = k - 8; i <= 8; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
 #pragma unroll
-        for (int j = k - 8; j <= 8; ++j) acc += (uint64_t)m[j] * f29::P(k - j);
+        for (int j = k - 8; j <= 8; ++j) acc += (uint64_t)m[j] * M::P(k - j);
         r.l[k - 9] = (uint32_t)acc & f29::MASK;
         acc >>= 29;
     }

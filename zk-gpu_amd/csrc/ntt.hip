@@ -22,6 +22,7 @@
 // Montgomery product (a - b) * tw yields the plain (a - b) * w^i. The inverse transform uses
 // w^-i twiddles and multiplies by n^-1 in the last pass's store.
 #include "field.hpp"
+#include "field29.hpp"
 #include "ntt.hpp"
 #include "tune.hpp"
 #include "host_math.hpp"
@@ -101,8 +102,23 @@ ZKG_DEV Fr sub_lazy(const Fr& x, const Fr& y) {
     return s;
 }
 
-// Montgomery a * b * 2^-256 for a < 4r, b < r: result < 2r (no final subtraction)
+// Montgomery a * b * 2^-256 for a < 4r, b < r (R = 2^256 form, as the twiddle and scale tables
+// hold it), computed in nine 29-bit limbs: a's limbs are a itself, b's words shifted left by five
+// bits are b in R = 2^261 form (value 32 b < 32 r), and the 29-bit product
+// a (32 b) 2^-261 = a b 2^-256 needs v(a) v(32 b) < 128 r^2 < r 2^261 (2^261 / r is about 169),
+// then returns a normalised value below 128 r^2 / 2^261 + r < 2r, packed back into eight words.
+// With the conversions about 290 instructions per product against about 390 for the 32-bit CIOS
+// product (mul_lazy_cios): one v_mad_u64_u32 per partial product and no carry chains. A twiddle
+// shared by two products is converted once (the compiler merges the identical conversions).
 ZKG_DEV Fr mul_lazy(const Fr& a, const Fr& b) {
+    const F29 t = f29_mul<Fr29Mod>(f29_unpack(a.w), f29_from_r256_words(b.w));
+    Fr r;
+    f29_to_words(t, r.w);
+    return r;
+}
+
+// The same product in 32-bit limbs (CIOS): result < 2r (no final subtraction).
+ZKG_DEV Fr mul_lazy_cios(const Fr& a, const Fr& b) {
     uint32_t t[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) t[j] = 0;
@@ -338,9 +354,11 @@ __global__ void ntt_twiddle_kernel(const uint32_t* __restrict__ hi, const uint32
 
 // ---------------------------------------------------------------- plan
 
-// Measured on MI355X (tuning runs at 2^16, 2^20, 2^22, 2^24): 2^10-element tiles are fastest
-// (256-lane workgroups at 100 VGPRs), except at 2^20 where 2^12 saves a pass.
-int ntt_default_tile_log(uint32_t log_n) { return log_n == 20 ? 12 : 10; }
+// Measured on MI355X (forward and inverse, every log_n in 12..26, tiles 2^10, 2^11, 2^12):
+// 2^10-element tiles (256-lane workgroups, 130 VGPRs, three waves per SIMD) are fastest up to
+// 2^18 and at 2^22; 2^11 (512 lanes, 127 VGPRs, four waves per SIMD) from 2^19 on, by 4-17%
+// (2^24: 1.99 against 2.25 ms). 2^12 tiles run radix-2 rounds only and are never ahead.
+int ntt_default_tile_log(uint32_t log_n) { return log_n >= 19 && log_n != 22 ? 11 : 10; }
 
 // The tile a plan uses when none is given: the tuning table's, else the default.
 static int ntt_tile_for(const Device* d, uint32_t log_n) {
