@@ -110,6 +110,22 @@ int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t it
  * from the tuning table (zkg_tune_msm / zkg_tune_load) for this GPU and size; untuned sizes use
  * default_window(n) with automatic item and sort sizes. Results equal any other plan's. */
 int zkg_msm_create_tuned(zkg_device* dev, int group, size_t n, zkg_msm** out);
+/* No counterpart in the reference (fixed-base precomputation for bases that stay resident, such as
+ * a proving key's): *out receives a new buffer of 2n points of `group` holding the n points of
+ * `bases` (library form, as zkg_msm_upload_bases* leave them) followed by 2^64 times each (64
+ * doublings and one inversion per point, on the device; infinity stays (0, 0)). Queued on the
+ * device's stream. Errors: "bases buffer holds ..." when `bases` has fewer than n points. */
+int zkg_msm_precompute_bases(zkg_device* dev, int group, const zkg_buffer* bases, size_t n, zkg_buffer** out);
+/* No counterpart in the reference: a plan whose zkg_msm_run takes a zkg_msm_precompute_bases
+ * buffer (2n points) as `bases`. Each GLV half-scalar splits at bit 64, so the stages run over 4n
+ * virtual points with ceil(64 / c) windows: the same bucket additions, half the buckets to reduce
+ * and about half the doublings of the window combine. Same result as zkg_msm_create's plan on the
+ * original bases. c in 9..=16 ("a plan over precomputed bases takes c in 9..=16, got {c}"),
+ * n < 2^29. */
+int zkg_msm_create_precomputed(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points,
+                               zkg_msm** out);
+/* Points a plan's bases buffer must hold: n, or 2n for a plan over precomputed bases. */
+size_t zkg_msm_base_points(const zkg_msm* plan);
 /* Points per work item the plan uses (the automatic choice when created with item_points = 0). */
 uint32_t zkg_msm_item_points(const zkg_msm* plan);
 int zkg_msm_destroy(zkg_msm* plan);

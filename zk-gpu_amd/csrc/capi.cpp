@@ -374,6 +374,40 @@ int zkg_msm_create_tuned(zkg_device* dev, int group, size_t n, zkg_msm** out) {
     ZKG_GUARD_END
 }
 
+int zkg_msm_create_precomputed(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points,
+                               zkg_msm** out) {
+    ZKG_REQUIRE(dev && out, "null argument");
+    ZKG_GUARD_BEGIN
+    *out = wrap_plan(dev->d, std::unique_ptr<zkg::MsmPlan>(new zkg::MsmPlan(dev->d, group, n, c, item_points, 0, true)));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_msm_precompute_bases(zkg_device* dev, int group, const zkg_buffer* bases, size_t n, zkg_buffer** out) {
+    ZKG_REQUIRE(dev && bases && out, "null argument");
+    ZKG_REQUIRE(group == ZKG_G1 || group == ZKG_G2, "group must be G1 (1) or G2 (2)");
+    ZKG_REQUIRE(n >= 1, "an MSM needs at least one point");
+    const uint64_t point_bytes = 64ull * group;
+    if (bases->b->bytes < n * point_bytes)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(bases->b->bytes) + " bytes; " +
+                                                std::to_string(n) + " points need " + std::to_string(n * point_bytes));
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    zkg::Buffer* b = zkg::buffer_alloc(dev->d, 2 * n * point_bytes);  // every word is written below
+    try {
+        zkg::msm_precompute_bases(group, static_cast<const uint32_t*>(bases->b->ptr), n,
+                                  static_cast<uint32_t*>(b->ptr), dev->d->stream);
+    } catch (...) {
+        zkg::buffer_free(b);
+        throw;
+    }
+    *out = new zkg_buffer{b};
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+size_t zkg_msm_base_points(const zkg_msm* m) { return m ? m->plan->base_points() : 0; }
+
 uint32_t zkg_msm_item_points(const zkg_msm* m) { return m ? m->plan->CH : 0; }
 
 int zkg_msm_destroy(zkg_msm* m) {
@@ -425,13 +459,13 @@ int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words) {
 // Msm::check_buffers (src/msm.rs:271-282)
 static int msm_check(const zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars) {
     ZKG_REQUIRE(m && bases && scalars, "null argument");
-    const uint64_t n = m->plan->n, point_bytes = 4ull * m->plan->aw;
+    const uint64_t n = m->plan->n, point_bytes = 4ull * m->plan->aw, pts = m->plan->base_points();
     if (scalars->b->bytes < n * 32)
         return zkg::fail(zkg::ZKG_ERR_SIZE, "scalars buffer holds " + std::to_string(scalars->b->bytes) + " bytes; " +
                                                 std::to_string(n) + " scalars need " + std::to_string(n * 32));
-    if (bases->b->bytes < n * point_bytes)
+    if (bases->b->bytes < pts * point_bytes)
         return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(bases->b->bytes) + " bytes; " +
-                                                std::to_string(n) + " points need " + std::to_string(n * point_bytes));
+                                                std::to_string(pts) + " points need " + std::to_string(pts * point_bytes));
     return zkg::ZKG_OK;
 }
 

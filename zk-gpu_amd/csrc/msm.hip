@@ -19,7 +19,9 @@
 //  combine    Horner's rule over the windows from the top, one wave with lane-parallel products.
 // With GLV (the default), "count" first splits every scalar into two signed 127-bit halves and
 // computes phi(P) = (beta x, y) for every base; the stages then run over 2n virtual points with
-// ceil(128 / c) windows instead of ceil(255 / c).
+// ceil(128 / c) windows instead of ceil(255 / c). A plan over precomputed bases (the bases and
+// their 2^64 multiples, msm_precompute_bases) also splits each half at bit 64: 4n virtual points,
+// ceil(64 / c) windows.
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
@@ -350,6 +352,35 @@ __global__ __launch_bounds__(256) void msm_digits_store_kernel(const uint32_t* _
     }
 }
 
+// Precomputed bases (bases buffer = P_0..P_{n-1}, 2^64 P_0..2^64 P_{n-1}): GLV half-scalar i < 2n
+// (k1 of point i for i < n, k2 of point i - n after) has 2W signed digits over its 128 bits, W in
+// each 64-bit half with the same widths, the carry running on across bit 64. The low half's
+// digits belong to the virtual point of P (phi(P) for k2), the high half's to that of 2^64 P
+// (phi(2^64 P)), both in windows 0..W-1. Virtual points: [0, n) P_i, [n, 2n) 2^64 P_i,
+// [2n, 3n) phi(P_i), [3n, 4n) phi(2^64 P_i): below 2n an entry of the bases buffer, above it phi of
+// entry v - 2n. The top digit takes no carry out: |k| < 2^127 leaves bit 127 clear.
+__global__ __launch_bounds__(256) void msm_digits_split_store_kernel(const uint32_t* __restrict__ halves, uint32_t n,
+                                                                    uint32_t W, uint32_t wb, uint32_t nw,
+                                                                    uint32_t* __restrict__ digits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n) return;
+    const size_t nv = 4 * (size_t)n;
+    uint32_t s[8];
+    const uint32_t sign = load_digit_scalar<true>(halves, i, s);
+    const uint32_t vlo = i < n ? i : n + i, vhi = vlo + n;
+    uint32_t carry = 0;
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t v_idx = h ? vhi : vlo;
+        for (uint32_t w = 0; w < W; ++w) {
+            const uint32_t width = window_width(w, wb, nw);
+            const uint32_t v = window_bits(s, 64 * h + window_pos(w, wb, nw), width) + carry;
+            uint32_t mag, neg;
+            signed_digit<true>(v, width, mag, neg, carry);
+            digits[(size_t)w * nv + v_idx] = mag ? ((mag - 1) | ((neg ^ sign) << 31)) : NO_DIGIT;
+        }
+    }
+}
+
 __global__ __launch_bounds__(SORT_THREADS) void msm_hist_kernel(const uint32_t* __restrict__ digits, uint32_t n,
                                                                uint32_t B, uint32_t P, uint32_t nblk,
                                                                uint32_t* __restrict__ table) {
@@ -527,8 +558,9 @@ uint32_t msm_plan_item_points(size_t n, uint32_t c) {
     return msm_auto_item_points(msm_glv_enabled(n) ? 2 * n : n, c);
 }
 
-MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_points, uint32_t sort_points)
-    : dev(d), group(grp) {
+MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_points, uint32_t sort_points,
+                 bool precomputed)
+    : dev(d), group(grp), pre(precomputed) {
     if (npts < 1) throw Error(ZKG_ERR_INVALID, "an MSM needs at least one point");
     if (win < 9 || win > 20) throw Error(ZKG_ERR_INVALID, "window size c must be in 9..=20, got " + std::to_string(win));
     if (grp != 1 && grp != 2) throw Error(ZKG_ERR_INVALID, "group must be G1 (1) or G2 (2)");
@@ -536,11 +568,19 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     n = (uint32_t)npts;
     c = win;
     glv = msm_glv_enabled(npts);
+    if (pre) {
+        if (win > 16) throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases takes c in 9..=16, got " +
+                                                       std::to_string(win));
+        // 4n virtual point indices must fit the 31-bit index field of a sorted entry
+        if (!glv || npts > 0x1fffffffu)
+            throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases needs the GLV split and n < 2^29");
+    }
     f29 = msm_f29_enabled();
-    nv = glv ? 2 * n : n;
-    W = glv ? (128 + c - 1) / c : (255 + c - 1) / c;
-    wb = glv ? 128 / W : c;
-    nw = glv ? 128 - wb * W : 0;
+    nv = pre ? 4 * n : glv ? 2 * n : n;
+    const uint32_t sbits = pre ? 64 : 128;  // bits per (half-)scalar the windows cover
+    W = glv ? (sbits + c - 1) / c : (255 + c - 1) / c;
+    wb = glv ? sbits / W : c;
+    nw = glv ? sbits - wb * W : 0;
     B = 1u << (c - 1);
     CH = item_points ? item_points : msm_auto_item_points(nv, c);
     L = msm_reduce_chunk(grp, c);
@@ -574,8 +614,8 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
         ZKG_HIP(hipMalloc(&scan_tmp, 4 * (3 * (size_t)nb + 1)));
         max_items_dev = scan_tmp + 3 * nb;
         if (glv) {
-            ZKG_HIP(hipMalloc(&split, 16 * (size_t)nv));
-            ZKG_HIP(hipMalloc(&endo_x, 2 * (size_t)n * aw));
+            ZKG_HIP(hipMalloc(&split, 32 * (size_t)n));
+            ZKG_HIP(hipMalloc(&endo_x, 2 * base_points() * aw));
         }
         ZKG_HIP(hipMalloc(&sorted, 4 * (size_t)W * nv));
         if (lds_sort) {
@@ -606,7 +646,7 @@ MsmPlan::~MsmPlan() {
     item_info = nullptr;
 }
 
-size_t MsmPlan::workspace_bytes() const { return msm_workspace_bytes(group, n, c, CH, sort_pts); }
+size_t MsmPlan::workspace_bytes() const { return msm_workspace_bytes(group, n, c, CH, sort_pts, pre); }
 
 bool msm_glv_enabled(size_t n) {
     // virtual point indices (2n of them) must fit the 31-bit index field of a sorted entry
@@ -634,10 +674,12 @@ bool msm_f29_enabled() {
     return !(e && e[0] == '0');
 }
 
-size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points) {
-    const bool glv = msm_glv_enabled(n);
-    const size_t nv = glv ? 2 * n : n;
-    const uint32_t W = glv ? (128 + c - 1) / c : (255 + c - 1) / c, B = 1u << (c - 1);
+size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points,
+                           bool precomputed) {
+    const bool glv = msm_glv_enabled(n), pre = precomputed && glv;
+    const size_t nv = pre ? 4 * n : glv ? 2 * n : n;
+    const uint32_t sbits = pre ? 64 : 128;
+    const uint32_t W = glv ? (sbits + c - 1) / c : (255 + c - 1) / c, B = 1u << (c - 1);
     const uint32_t CH = item_points ? item_points : msm_auto_item_points(nv, c);
     const uint32_t xw = group == 1 ? 32 : 64, T = B / msm_reduce_chunk(group, c);
     const size_t S = (size_t)W * B, max_items = (size_t)W * ((nv + CH - 1) / CH + B);
@@ -645,7 +687,7 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
     const size_t P = sort_points ? std::max<uint32_t>(sort_points, B) : msm_default_sort_points(nv, c);
     const size_t nblk = (nv + P - 1) / P;
     const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    return 4 * ((glv ? 4 * nv + n * xw / 2 : 0) + 4 * S + 2 + 3 * nb + 1 + (size_t)W * nv * (lds ? 2 : 1) +
+    return 4 * ((glv ? 8 * n + (pre ? 2 : 1) * n * xw / 2 : 0) + 4 * S + 2 + 3 * nb + 1 + (size_t)W * nv * (lds ? 2 : 1) +
                 (lds ? (size_t)W * nblk * B : 0) + max_items * (2 + (size_t)xw) + S * xw + (size_t)W * T * xw +
                 (size_t)W * xw + xw);
 }
@@ -685,13 +727,17 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
     const uint32_t* dscal = scalars;
     if (p.glv) {
         hipLaunchKernelGGL(msm_glv_split_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, scalars, p.n, p.split);
-        hipLaunchKernelGGL(msm_endo_kernel, dim3(ceil_div((size_t)p.n * p.group, 256)), dim3(256), 0, s, bases, p.n,
-                           p.group, p.endo_x);
+        const size_t nbase = p.base_points();
+        hipLaunchKernelGGL(msm_endo_kernel, dim3(ceil_div(nbase * p.group, 256)), dim3(256), 0, s, bases,
+                           (uint32_t)nbase, p.group, p.endo_x);
         dscal = p.split;
     }
     const unsigned pg = ceil_div(p.nv, 256);
     if (p.lds_sort) {
-        if (p.glv)
+        if (p.pre)
+            hipLaunchKernelGGL(msm_digits_split_store_kernel, dim3(ceil_div(2 * (size_t)p.n, 256)), dim3(256), 0, s,
+                               dscal, p.n, p.W, p.wb, p.nw, p.digits);
+        else if (p.glv)
             hipLaunchKernelGGL(msm_digits_store_kernel<true>, dim3(pg), dim3(256), 0, s, dscal, p.nv, p.W, p.wb,
                                p.nw, p.digits);
         else

@@ -16,8 +16,10 @@ uint32_t msm_default_window(size_t n);          // clamp(ilog2(max(n,1)) - 5, 11
 uint32_t msm_auto_item_points(size_t n, uint32_t c);
 // The work-item size a plan for (n, c) picks when none is given.
 uint32_t msm_plan_item_points(size_t n, uint32_t c);
-// Device bytes a plan for (group, n, c, item_points, sort_points) allocates (0 = automatic).
-size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0);
+// Device bytes a plan for (group, n, c, item_points, sort_points, precomputed) allocates
+// (0 = automatic).
+size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0,
+                           bool precomputed = false);
 // Whether plans use the GLV split (default on; ZKGPU_MSM_GLV=0 turns it off for comparisons).
 bool msm_glv_enabled(size_t n);
 // Whether plans use the 29-bit-limb field (field29.hpp): the whole G1 pipeline, and the G2 stages
@@ -37,9 +39,15 @@ struct MsmPlan {
     // so the MSM runs over nv = 2n virtual points (P_i with k1_i, phi(P_i) with k2_i) with
     // 128-bit scalars: half the windows, half the bucket reduction and half the doubling chain.
     bool glv = false;
+    // Precomputed bases (`pre`): the bases buffer holds P_0..P_{n-1} and then 2^64 P_0 ..
+    // 2^64 P_{n-1} (msm_precompute_bases), so each 128-bit half-scalar splits at bit 64 into two
+    // 64-bit ones: nv = 4n virtual points (P, 2^64 P, phi(P), phi(2^64 P)) over ceil(64 / c)
+    // windows. The same bucket additions as GLV alone, but half the windows: half the buckets to
+    // finalize and reduce and a doubling chain of about 64 - c instead of 128 - c.
+    bool pre = false;
     bool f29 = false;        // 29-bit-limb field (field29.hpp)
-    uint32_t nv = 0;         // virtual points: 2n with GLV, else n
-    uint32_t W = 0;          // windows, ceil(128 / c) with GLV, else ceil(255 / c)
+    uint32_t nv = 0;         // virtual points: 4n precomputed, 2n with GLV, else n
+    uint32_t W = 0;          // windows: ceil(64 / c) precomputed, ceil(128 / c) with GLV, else ceil(255 / c)
     // Window w covers bits [w*wb + min(w, nw), +wb + (w < nw)): without GLV wb = c, nw = 0; with
     // GLV the 128 bits are split evenly (c = 15: two windows of 15 bits and seven of 14), so no
     // window is left with a few bits and a few heavy buckets.
@@ -57,8 +65,8 @@ struct MsmPlan {
     uint32_t nblk = 0;       // sort blocks per window
 
     // workspace (device)
-    uint32_t* split = nullptr;       // nv * 4  GLV half-scalars: |k| in 127 bits, sign in bit 127
-    uint32_t* endo_x = nullptr;      // n * aw / 2  beta * x of every base (Montgomery form)
+    uint32_t* split = nullptr;       // 2n * 4  GLV half-scalars: |k| in 127 bits, sign in bit 127
+    uint32_t* endo_x = nullptr;      // n * aw / 2 (2n * aw / 2 precomputed)  beta * x of every base (Montgomery form)
     uint32_t* counts = nullptr;      // W*B  points per (window, bucket)
     uint32_t* cursors = nullptr;     // W*B  scatter cursors
     uint32_t* starts = nullptr;      // W*B + 1  exclusive scan of counts (global over windows)
@@ -76,8 +84,15 @@ struct MsmPlan {
     uint32_t* result = nullptr;      // xw        final XYZZ point
     void* result_handle = nullptr;   // zkg_buffer* wrapping `result` (owned by the C ABI layer)
 
-    // item_points / sort_points 0 = automatic (a quarter of the mean bucket / max(2^(c-1), 8192))
-    MsmPlan(Device* d, int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0);
+    // item_points / sort_points 0 = automatic (a quarter of the mean bucket / max(2^(c-1), 8192));
+    // precomputed: a plan over msm_precompute_bases tables (c <= 16, n < 2^29)
+    MsmPlan(Device* d, int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0,
+            bool precomputed = false);
+    // Virtual points below this index are entries of the bases buffer, the others phi of one
+    // (2^32 - 1 without GLV).
+    uint32_t n_real() const { return pre ? 2 * n : glv ? n : 0xffffffffu; }
+    // Points the bases buffer must hold: 2n for a precomputed plan, else n.
+    size_t base_points() const { return pre ? 2 * (size_t)n : n; }
     ~MsmPlan();
     // Queue the MSM on `stream`. With `stage_ms` set, wait around every stage and record
     // (stage name, milliseconds) as the reference's Msm::profile does.
@@ -95,5 +110,9 @@ void msm_glv_split(const uint32_t* scalars, uint32_t n, uint32_t* split, hipStre
 
 // Plain affine coordinates -> Montgomery form in place (`coords` elements of 8 words).
 void msm_convert_bases(uint32_t* xy, size_t coords, hipStream_t stream);
+
+// The bases table of a precomputed plan: out[0..n) = bases[0..n), out[n..2n) = 2^64 bases[i]
+// (library form; `out` holds 2n points of `group`).
+void msm_precompute_bases(int group, const uint32_t* bases, size_t n, uint32_t* out, hipStream_t stream);
 
 }  // namespace zkg

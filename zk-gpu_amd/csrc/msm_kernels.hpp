@@ -533,15 +533,15 @@ template <class Fe>
 void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
     if constexpr (std::is_same<Fe, Fq2_29>::value) {
         hipLaunchKernelGGL((msm_accumulate_kernel<Fq2, Fq2_29>), dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s,
-                           bases, (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
+                           bases, (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
                            p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
     } else if constexpr (std::is_same<Fe, Fq29>::value) {
         hipLaunchKernelGGL(msm_accumulate29_kernel<Fq29>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
-                           (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
+                           (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
                            p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
     } else {
         hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
-                           (const uint32_t*)p.endo_x, p.glv ? p.n : 0xffffffffu, p.sorted, p.starts, p.istarts,
+                           (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
                            p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
     }
 }

@@ -60,6 +60,14 @@ __global__ __launch_bounds__(256) void field_bench_kernel(uint32_t chain, uint32
 
 extern template void launch_point_op<Fq2>(int, const uint32_t*, const uint32_t*, uint32_t*, size_t, hipStream_t);
 extern template void launch_generator_multiples<Fq2>(const uint32_t*, uint32_t*, size_t, hipStream_t);
+extern template void launch_shift64<Fq2>(const uint32_t*, uint32_t*, size_t, hipStream_t);
+
+void msm_precompute_bases(int group, const uint32_t* bases, size_t n, uint32_t* out, hipStream_t stream) {
+    if (!n) return;
+    if (group == 1) launch_shift64<Fq>(bases, out, n, stream);
+    else launch_shift64<Fq2>(bases, out, n, stream);
+    ZKG_HIP(hipGetLastError());
+}
 
 }  // namespace zkg
 

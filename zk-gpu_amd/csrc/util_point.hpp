@@ -44,6 +44,36 @@ __global__ __launch_bounds__(256) void generator_multiples_kernel(const uint32_t
     store_plain_affine<Fe>(out + 2 * T::WORDS * i, acc);
 }
 
+// ---------------------------------------------------------------- precomputed MSM bases
+// out[i] = bases[i] and out[n + i] = 2^64 bases[i], i < n, in the library's form (Montgomery,
+// affine; infinity stays (0, 0)): 64 XYZZ doublings and one Fermat inversion per point
+// (1/ZZ = t ZZZ and 1/ZZZ = t ZZ with t = 1/(ZZ ZZZ)). No counterpart in the reference.
+template <class Fe>
+__global__ __launch_bounds__(256) void msm_shift64_kernel(const uint32_t* __restrict__ bases, uint32_t* __restrict__ out,
+                                                         size_t n) {
+    typedef FeTraits<Fe> T;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Affine<Fe> a = load_affine<Fe>(bases + 2 * T::WORDS * i);
+    T::store(out + 2 * T::WORDS * i, a.x);
+    T::store(out + 2 * T::WORDS * i + T::WORDS, a.y);
+    Xyzz<Fe> q = xyzz_from_affine(a);
+    for (int k = 0; k < 64; ++k) q = xyzz_dbl(q);
+    uint32_t* o = out + 2 * T::WORDS * (n + i);
+    if (xyzz_is_inf(q)) {
+        for (int k = 0; k < 2 * T::WORDS; ++k) o[k] = 0;
+        return;
+    }
+    const Fe t = fe_inverse(q.zz * q.zzz);
+    T::store(o, q.x * (t * q.zzz));
+    T::store(o + T::WORDS, q.y * (t * q.zz));
+}
+
+template <class Fe>
+void launch_shift64(const uint32_t* bases, uint32_t* out, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(msm_shift64_kernel<Fe>, dim3(ceil_div(n, 256)), dim3(256), 0, s, bases, out, n);
+}
+
 template <class Fe>
 void launch_point_op(int op, const uint32_t* p, const uint32_t* q, uint32_t* out, size_t count, hipStream_t s) {
     hipLaunchKernelGGL(point_op_kernel<Fe>, dim3(ceil_div(count, 64)), dim3(64), 0, s, op, p, q, out, count);

@@ -43,6 +43,11 @@ SIGNATURES = {
                                       c_void_pp]),
     "zkg_msm_create_tuned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, c_void_pp]),
     "zkg_msm_item_points": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "zkg_msm_create_precomputed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32,
+                                                  ctypes.c_uint32, c_void_pp]),
+    "zkg_msm_precompute_bases": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                                c_void_pp]),
+    "zkg_msm_base_points": (ctypes.c_size_t, [ctypes.c_void_p]),
     "zkg_msm_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_msm_group": (ctypes.c_int, [ctypes.c_void_p]),
     "zkg_msm_window": (ctypes.c_uint32, [ctypes.c_void_p]),

@@ -18,10 +18,14 @@ class Msm:
     """Plan for MSMs of n points in one group with window c (src/msm.rs:95-180). Not re-entrant:
     the plan owns its workspace and the result buffer returned by `run`."""
 
-    def __init__(self, gpu, group: Group, n: int, c: int, item_points: Optional[int] = None, _tuned: bool = False):
+    def __init__(self, gpu, group: Group, n: int, c: int, item_points: Optional[int] = None, _tuned: bool = False,
+                 _precomputed: bool = False):
         h = ctypes.c_void_p()
         if _tuned:
             check(_lib.load().zkg_msm_create_tuned(gpu.handle, group.value, int(n), ctypes.byref(h)))
+        elif _precomputed:
+            check(_lib.load().zkg_msm_create_precomputed(gpu.handle, group.value, int(n), int(c),
+                                                         int(item_points or 0), ctypes.byref(h)))
         else:
             check(_lib.load().zkg_msm_create(gpu.handle, group.value, int(n), int(c), int(item_points or 0),
                                              ctypes.byref(h)))
@@ -51,6 +55,24 @@ class Msm:
         """A plan with the tuning table's window, item and sort sizes for this GPU and size
         (default_window(n) when the size is untuned). No counterpart in the reference."""
         return Msm(gpu, group, n, 0, None, _tuned=True)
+
+    @staticmethod
+    def precomputed(gpu, group: Group, n: int, c: int, item_points: Optional[int] = None) -> "Msm":
+        """A plan whose `run` takes `precompute_bases`'s table (2n points) as its bases: 4n virtual
+        points over ceil(64 / c) windows, same result. c in 9..=16. No counterpart in the reference."""
+        return Msm(gpu, group, n, c, item_points, _precomputed=True)
+
+    @staticmethod
+    def precompute_bases(gpu, group: Group, bases: Buffer, n: int) -> Buffer:
+        """New buffer: the n points of `bases` (library form) followed by 2^64 times each, computed
+        on the device. No counterpart in the reference."""
+        h = ctypes.c_void_p()
+        check(_lib.load().zkg_msm_precompute_bases(gpu.handle, group.value, bases.handle, int(n), ctypes.byref(h)))
+        return Buffer(gpu, h.value)
+
+    def base_points(self) -> int:
+        """Points the bases buffer must hold: n, or 2n for a plan over precomputed bases."""
+        return int(_lib.load().zkg_msm_base_points(self._h))
 
     def item_points(self) -> int:
         return int(_lib.load().zkg_msm_item_points(self._h))
