@@ -583,7 +583,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     nw = glv ? sbits - wb * W : 0;
     B = 1u << (c - 1);
     CH = item_points ? item_points : msm_auto_item_points(nv, c);
-    L = msm_reduce_chunk(grp, c);
+    L = msm_reduce_chunk(grp, c, W);
     T = B / L;
     aw = group == 1 ? 16 : 32;
     xw = 2 * aw;
@@ -665,9 +665,14 @@ uint32_t msm_default_sort_points(size_t nv, uint32_t c) {
 
 // Buckets per lane of the bucket reduction: its serial chain is 2L additions plus the workgroup
 // tree, against W B / L lanes in all. Measured on MI355X: L = 4 for G1 (2^20: 2.90 ms, 3.00 with
-// L = 2) and for G2 at c = 16 (2^22: reduce 1.08 ms, 1.86 with L = 2); L = 2 for G2 with smaller
-// windows (2^18, c = 13: 4.52 ms, 4.59 with L = 4; L = 1 moves the time into the window sums).
-uint32_t msm_reduce_chunk(int group, uint32_t c) { return group == 2 && c <= 14 ? 2 : 4; }
+// L = 2; over precomputed bases at c = 16, 131072 buckets, L = 2 and 4 tie) and for G2 with 2^18
+// buckets or more (2^22, c = 16: reduce 1.08 ms, 1.86 with L = 2); L = 2 for G2 with fewer
+// buckets (2^18, c = 13, 40960 buckets: 4.52 ms, 4.59 with L = 4; precomputed 2^18 and 2^20 at
+// c = 16, 131072 buckets: reduce 0.99 ms, 1.07-1.09 with L = 4, 1.80 with L = 1, which moves the
+// time into the window sums).
+uint32_t msm_reduce_chunk(int group, uint32_t c, uint32_t windows) {
+    return group == 2 && ((uint64_t)windows << (c - 1)) < (1u << 18) ? 2 : 4;
+}
 
 bool msm_f29_enabled() {
     const char* e = std::getenv("ZKGPU_MSM_F29");
@@ -681,7 +686,7 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
     const uint32_t sbits = pre ? 64 : 128;
     const uint32_t W = glv ? (sbits + c - 1) / c : (255 + c - 1) / c, B = 1u << (c - 1);
     const uint32_t CH = item_points ? item_points : msm_auto_item_points(nv, c);
-    const uint32_t xw = group == 1 ? 32 : 64, T = B / msm_reduce_chunk(group, c);
+    const uint32_t xw = group == 1 ? 32 : 64, T = B / msm_reduce_chunk(group, c, W);
     const size_t S = (size_t)W * B, max_items = (size_t)W * ((nv + CH - 1) / CH + B);
     const bool lds = c <= 16;
     const size_t P = sort_points ? std::max<uint32_t>(sort_points, B) : msm_default_sort_points(nv, c);

@@ -28,8 +28,8 @@ bool msm_glv_enabled(size_t n);
 bool msm_f29_enabled();
 // Counting-sort block size (points) a plan over nv virtual points with window c uses by default.
 uint32_t msm_default_sort_points(size_t nv, uint32_t c);
-// Buckets per lane of the bucket-reduction stage.
-uint32_t msm_reduce_chunk(int group, uint32_t c);
+// Buckets per lane of the bucket-reduction stage (windows of 2^(c-1) buckets each).
+uint32_t msm_reduce_chunk(int group, uint32_t c, uint32_t windows);
 
 struct MsmPlan {
     Device* dev = nullptr;
