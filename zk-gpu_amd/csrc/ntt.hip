@@ -44,8 +44,8 @@ __device__ __forceinline__ size_t tile_addr(uint32_t wg, uint32_t row, uint32_t 
 
 // ---- lazy reduction: between the first load and the last store, elements live in [0, 2r).
 // x + y < 4r and x + 2r - y < 4r both fit 256 bits (r < 2^254). A Montgomery product whose
-// first factor is < 4r and second (a twiddle or n^-1) < r keeps its CIOS running value below
-// a + r < 5r < 2^256 and ends below a b / 2^256 + r < 2r, so it needs no final subtraction.
+// first factor is < 4r and second (a twiddle or n^-1) < r ends below 2r (mul_lazy states the
+// bound), so it needs no final subtraction.
 
 // word i of 2r
 ZKG_DEV uint32_t twice_r(int i) { return 2u * FrParams::mod(i) + (i ? FrParams::mod(i - 1) >> 31 : 0u); }
@@ -107,56 +107,13 @@ ZKG_DEV Fr sub_lazy(const Fr& x, const Fr& y) {
 // bits are b in R = 2^261 form (value 32 b < 32 r), and the 29-bit product
 // a (32 b) 2^-261 = a b 2^-256 needs v(a) v(32 b) < 128 r^2 < r 2^261 (2^261 / r is about 169),
 // then returns a normalised value below 128 r^2 / 2^261 + r < 2r, packed back into eight words.
-// With the conversions about 290 instructions per product against about 390 for the 32-bit CIOS
-// product (mul_lazy_cios): one v_mad_u64_u32 per partial product and no carry chains. A twiddle
+// With the conversions about 290 instructions per product against about 390 for a CIOS product in
+// 32-bit limbs (which needs an add-with-carry and a move besides each v_mad_u64_u32). A twiddle
 // shared by two products is converted once (the compiler merges the identical conversions).
 ZKG_DEV Fr mul_lazy(const Fr& a, const Fr& b) {
     const F29 t = f29_mul<Fr29Mod>(f29_unpack(a.w), f29_from_r256_words(b.w));
     Fr r;
     f29_to_words(t, r.w);
-    return r;
-}
-
-// The same product in 32-bit limbs (CIOS): result < 2r (no final subtraction).
-ZKG_DEV Fr mul_lazy_cios(const Fr& a, const Fr& b) {
-    uint32_t t[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) t[j] = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        uint32_t lo[8], hi[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint64_t s = (uint64_t)a.w[j] * b.w[i] + t[j];
-            lo[j] = (uint32_t)s;
-            hi[j] = (uint32_t)(s >> 32);
-        }
-        unsigned c = 0, c2;
-        t[0] = lo[0];
-#pragma unroll
-        for (int j = 1; j < 8; ++j) {
-            t[j] = __builtin_addc(lo[j], hi[j - 1], c, &c2);
-            c = c2;
-        }
-        const uint32_t top = hi[7] + c;
-        const uint32_t m = t[0] * FrParams::INV;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint64_t s = (uint64_t)m * FrParams::mod(j) + t[j];
-            lo[j] = (uint32_t)s;
-            hi[j] = (uint32_t)(s >> 32);
-        }
-        c = 0;
-#pragma unroll
-        for (int j = 1; j < 8; ++j) {
-            t[j - 1] = __builtin_addc(lo[j], hi[j - 1], c, &c2);
-            c = c2;
-        }
-        t[7] = top + hi[7] + c;
-    }
-    Fr r;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r.w[j] = t[j];
     return r;
 }
 
