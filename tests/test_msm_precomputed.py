@@ -167,3 +167,24 @@ def test_precomputed_argument_errors(gpu):
     # the original bases (n points) are not a table of 2n
     with pytest.raises(ZkgError, match=f"bases buffer holds {64 * n} bytes; {2 * n} points need {128 * n}"):
         pre.run(gpu, bb, gpu.buffer(uniform_fr_words(rng, n)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("group,log_n,c", [(1, 20, 16), (2, 18, 16), (1, 21, 15), (2, 20, 14)])
+def test_precomputed_plans_at_bench_sizes_closed_form(gpu, group, log_n, c):
+    """The precomputed plans the bench reports (G1 2^20 and G2 2^18 at c = 16) and two more
+    (size, window) pairs, exact at full size against the closed form (sum k_i s_i mod r) G."""
+    from conftest import quick
+    from test_gpu_msm import affine, device_bases
+    from zkgpu import Group, Msm
+
+    if quick():
+        pytest.skip("ZK_GPU_QUICK")
+    n = 1 << log_n
+    rng = np.random.default_rng(690 + 10 * group + log_n)
+    ks, sc = uniform_fr_words(rng, n), uniform_fr_words(rng, n)
+    bb = device_bases(gpu, group, ks)
+    table = Msm.precompute_bases(gpu, Group(group), bb, n)
+    m = Msm.precomputed(gpu, Group(group), n, c)
+    got = affine(gpu, m.run(gpu, table, gpu.buffer(sc)), group)
+    assert np.array_equal(got, O.point_mul(group, O.generator_words(group), O.closed_form_scalar(ks, sc)))
