@@ -303,3 +303,27 @@ def test_msm_seeded_fuzz(gpu):
         bases = O.multiples(1, O.generator_words(1), ks)
         sc = witness_scalars(rng, n) if rng.integers(0, 2) else uniform_fr_words(rng, n)
         assert np.array_equal(gpu_msm(gpu, 1, bases, sc, c, items), O.msm(1, bases, sc)), (seed, n, c, items)
+
+
+@pytest.mark.parametrize("c", list(range(9, 21)))
+def test_msm_every_window_vs_oracle(gpu, c):
+    """Every window the reference accepts (9..=20: the LDS counting sort up to 16, the atomic sort
+    above), G1 at n = 3000 and G2 at n = 700, against the oracle."""
+    rng = np.random.default_rng(3100 + c)
+    for group, n in ((1, 3000), (2, 700)):
+        ks = uniform_fr_words(rng, n)
+        bases = O.multiples(group, O.generator_words(group), ks)
+        sc = uniform_fr_words(rng, n)
+        assert np.array_equal(gpu_msm(gpu, group, bases, sc, c, 0), O.msm(group, bases, sc)), (group, c)
+
+
+@pytest.mark.parametrize("items", [1, 2, 3, 5, 8, 64])
+def test_msm_item_sizes_vs_oracle(gpu, items):
+    """The work-item sizes of the reference's test matrix besides 7 and 1024 (which
+    test_msm_random_vs_oracle covers), at a window where buckets hold ~20 points."""
+    rng = np.random.default_rng(3200 + items)
+    n = 6000
+    ks = uniform_fr_words(rng, n)
+    bases = O.multiples(1, O.generator_words(1), ks)
+    sc = uniform_fr_words(rng, n)
+    assert np.array_equal(gpu_msm(gpu, 1, bases, sc, 10, items), O.msm(1, bases, sc))
