@@ -2,6 +2,8 @@
 tests/golden.rs (MSM half) and the MSM half of tests/properties.rs, with the CPU oracle (and the
 closed form sum s_i (k_i G) = (sum k_i s_i) G) in place of arkworks. Results are compared as
 affine points after normalisation, word for word."""
+import os
+
 import numpy as np
 import pytest
 
@@ -327,3 +329,50 @@ def test_msm_item_sizes_vs_oracle(gpu, items):
     bases = O.multiples(1, O.generator_words(1), ks)
     sc = uniform_fr_words(rng, n)
     assert np.array_equal(gpu_msm(gpu, 1, bases, sc, 10, items), O.msm(1, bases, sc))
+
+
+def test_msm_nearly_empty_top_window(gpu):
+    """tests/properties.rs:268-280: with the reference's 255-bit digits and 14-bit windows the top
+    window holds 3 bits, so all points crowd into a few buckets; plans (14, 8), (14, 1), (11, 8),
+    with the GLV split on and off."""
+    from zkgpu import Group, Msm
+
+    rng = np.random.default_rng(17)
+    n = 20000
+    ks = uniform_fr_words(rng, n)
+    bases = O.multiples(1, O.generator_words(1), ks)
+    sc = uniform_fr_words(rng, n)
+    exp = O.msm(1, bases, sc)
+    bb = Msm.upload_bases(gpu, bases)
+    sb = gpu.buffer(sc)
+    old = os.environ.get("ZKGPU_MSM_GLV")
+    try:
+        for flag in ("0", "1"):
+            os.environ["ZKGPU_MSM_GLV"] = flag
+            for c, items in ((14, 8), (14, 1), (11, 8)):
+                m = Msm.for_group(gpu, Group.G1, n, c, items)
+                assert np.array_equal(affine(gpu, m.run(gpu, bb, sb), 1), exp), (flag, c, items)
+    finally:
+        if old is None:
+            os.environ.pop("ZKGPU_MSM_GLV", None)
+        else:
+            os.environ["ZKGPU_MSM_GLV"] = old
+
+
+def test_msm_generator_progression_2_18(gpu):
+    """tests/properties.rs:296-309: bases (i + 1) G for i < 2^18 (a progression, as the reference
+    builds them), plans (default) and (15, 8), exact against (sum (i + 1) s_i) G."""
+    from zkgpu import Group, Msm
+
+    if quick():
+        pytest.skip("ZK_GPU_QUICK")
+    n = 1 << 18
+    rng = np.random.default_rng(18)
+    ks = O.ints_to_words(range(1, n + 1))
+    sc = uniform_fr_words(rng, n)
+    bb = device_bases(gpu, 1, ks)
+    sb = gpu.buffer(sc)
+    want = O.point_mul(1, O.generator_words(1), O.closed_form_scalar(ks, sc))
+    for c, items in ((Msm.default_window(n), None), (15, 8)):
+        m = Msm.for_group(gpu, Group.G1, n, c, items)
+        assert np.array_equal(affine(gpu, m.run(gpu, bb, sb), 1), want), (c, items)
