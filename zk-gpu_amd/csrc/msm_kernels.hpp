@@ -280,7 +280,10 @@ __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restr
                                                         uint32_t* __restrict__ partials) {
     typedef FeTraits<Fe> Tr;
     constexpr int XW = 4 * Tr::WORDS;
-    __shared__ uint32_t lds[128 * XW];
+    // LDS slots padded to XW + 4 words: at XW words (a multiple of the 64 banks' half) the lanes of
+    // a wave storing or loading their points would share banks in pairs of lanes two apart
+    constexpr int XP = XW + 4;
+    __shared__ uint32_t lds[128 * XP];
     const uint32_t T = B / L;  // lanes per window, a multiple of blockDim.x
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t w = t / T, j = t % T, b0 = j * L;
@@ -304,9 +307,9 @@ __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restr
         }
     }
     for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XW * (threadIdx.x - off), r);
+        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XP * (threadIdx.x - off), r);
         __syncthreads();
-        if (threadIdx.x < off) r = xyzz_add(r, load_xyzz<Fe>(lds + XW * threadIdx.x));
+        if (threadIdx.x < off) r = xyzz_add(r, load_xyzz<Fe>(lds + XP * threadIdx.x));
         __syncthreads();
     }
     if (threadIdx.x == 0) store_xyzz(partials + (size_t)XW * blockIdx.x, r);
@@ -319,15 +322,16 @@ __global__ __launch_bounds__(64) void msm_window_sum_kernel(const uint32_t* __re
                                                            uint32_t top, uint32_t* __restrict__ wsums) {
     typedef FeTraits<Fe> Tr;
     constexpr int XW = 4 * Tr::WORDS;
-    __shared__ uint32_t lds[32 * XW];
+    constexpr int XP = XW + 4;  // padded LDS slots, as in msm_reduce_kernel
+    __shared__ uint32_t lds[32 * XP];
     const uint32_t w = blockIdx.x;
     const uint32_t* src = partials + (size_t)XW * w * P;
     Xyzz<Fe> acc = threadIdx.x < P ? load_xyzz<Fe>(src + XW * threadIdx.x) : xyzz_inf<Fe>();
     for (uint32_t k = threadIdx.x + 64; k < P; k += 64) acc = xyzz_add(acc, load_xyzz<Fe>(src + XW * k));
     for (uint32_t off = top; off > 0; off >>= 1) {
-        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XW * (threadIdx.x - off), acc);
+        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XP * (threadIdx.x - off), acc);
         __syncthreads();
-        if (threadIdx.x < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XW * threadIdx.x));
+        if (threadIdx.x < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XP * threadIdx.x));
         __syncthreads();
     }
     if (threadIdx.x == 0) store_xyzz(wsums + (size_t)XW * w, acc);
