@@ -529,6 +529,151 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
     if (lane == 0) store_xyzz_result(result, acc);
 }
 
+// ---------------------------------------------------------------- G1 combine on lazy 29-bit limbs
+// The same Horner chain for the 29-bit G1 pipeline, with the values kept lazily reduced between
+// products as in msm_accumulate29_kernel (no renormalisation after each addition or subtraction,
+// which the reduced Fq29 type pays for). Bounds in units of p, checked against the product's limit
+// (169) in tests/test_field29_model.py: the running point keeps X < 9, Y < 5, ZZ, ZZZ < 2 through
+// every doubling and addition; window sums (Fq29 storage) are below 2.
+
+ZKG_DEV F29 f29_sel(bool c, const F29& a, const F29& b) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = c ? a.l[i] : b.l[i];
+    return r;
+}
+ZKG_DEV F29 f29_pick(uint32_t lane, const F29& a0, const F29& a1, const F29& a2) {
+    return f29_sel(lane == 0, a0, f29_sel(lane == 1, a1, a2));
+}
+ZKG_DEV F29 f29_pick(uint32_t lane, const F29& a0, const F29& a1, const F29& a2, const F29& a3) {
+    return f29_sel(lane == 0, a0, f29_sel(lane == 1, a1, f29_sel(lane == 2, a2, a3)));
+}
+ZKG_DEV F29 f29_pick(uint32_t lane, const F29& a0, const F29& a1, const F29& a2, const F29& a3, const F29& a4,
+                     const F29& a5) {
+    return f29_sel(lane == 0, a0, f29_sel(lane == 1, a1, f29_sel(lane == 2, a2, f29_sel(lane == 3, a3,
+                                                                                      f29_sel(lane == 4, a4, a5)))));
+}
+ZKG_DEV F29 f29_shfl(const F29& x, int src) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = __shfl(x.l[i], src);
+    return r;
+}
+// limbs 0..7 below 2^29 again (the value is unchanged)
+ZKG_DEV F29 f29_carry(F29 a) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        a.l[i + 1] += a.l[i] >> 29;
+        a.l[i] &= f29::MASK;
+    }
+    return a;
+}
+
+struct Xyzz29 {
+    F29 x, y, zz, zzz;
+};
+
+// 2P (dbl-2008-s-1) with each level's products on lanes 0..3; P finite, X < 9, Y < 5, ZZ, ZZZ < 2.
+ZKG_DEV Xyzz29 xyzz29_dbl_lanes(const Xyzz29& p, uint32_t lane) {
+    const F29 u = f29_add(p.y, p.y);                      // < 10, limbs < 2^30
+    F29 a = f29_sel(lane == 0, u, p.x);
+    F29 m = f29_mul(a, a);                                // 10 * 10, 9 * 9 -> < 2
+    const F29 v = f29_shfl(m, 0), x2 = f29_shfl(m, 1);
+    const F29 mm = f29_carry(f29_add(f29_add(x2, x2), x2));  // 3 X^2 < 6, normalised
+    a = f29_pick(lane, u, p.x, v, mm);
+    F29 b = f29_pick(lane, v, v, p.zz, mm);
+    m = f29_mul(a, b);                                    // 10 * 2, 9 * 2, 2 * 2, 6 * 6 -> < 2
+    const F29 w = f29_shfl(m, 0), s = f29_shfl(m, 1), zz3 = f29_shfl(m, 2), m2 = f29_shfl(m, 3);
+    const F29 x3 = f29_sub<6>(m2, f29_add(s, s));         // 2S < 4 < 5: X3 < 8
+    const F29 d = f29_sub<10>(s, x3);                     // < 12
+    a = f29_pick(lane, mm, w, w);
+    b = f29_pick(lane, d, p.y, p.zzz);
+    m = f29_mul(a, b);                                    // 6 * 12, 2 * 5, 2 * 2 -> < 2
+    const F29 y3 = f29_sub<3>(f29_shfl(m, 0), f29_shfl(m, 1));  // < 5
+    return Xyzz29{x3, y3, zz3, f29_shfl(m, 2)};
+}
+
+// P + Q (add-2008-s) with every special case; P finite (bounds as above), Q finite with all
+// coordinates below 2. *inf is set when the sum is infinity.
+ZKG_DEV Xyzz29 xyzz29_add_lanes(const Xyzz29& p, const Xyzz29& q, uint32_t lane, bool* inf) {
+    F29 a = f29_pick(lane, p.x, q.x, p.y, q.y, p.zz, p.zzz);
+    F29 b = f29_pick(lane, q.zz, p.zz, q.zzz, p.zzz, q.zz, q.zzz);
+    F29 m = f29_mul(a, b);                                // 9 * 2, 2 * 2, 5 * 2, 2 * 2, 2 * 2, 2 * 2 -> < 2
+    const F29 u1 = f29_shfl(m, 0), u2 = f29_shfl(m, 1), s1 = f29_shfl(m, 2), s2 = f29_shfl(m, 3);
+    const F29 zz12 = f29_shfl(m, 4), zzz12 = f29_shfl(m, 5);
+    const F29 pp = f29_sub<3>(u2, u1), r = f29_sub<3>(s2, s1);  // < 5
+    *inf = false;
+    if (f29_is_zero_below12p(pp)) {
+        if (f29_is_zero_below12p(r)) return xyzz29_dbl_lanes(p, lane);
+        *inf = true;
+        return p;
+    }
+    a = f29_sel(lane == 0, pp, r);
+    m = f29_mul(a, a);                                    // 5 * 5 -> < 2
+    const F29 ppsq = f29_shfl(m, 0), r2 = f29_shfl(m, 1);
+    a = f29_pick(lane, pp, u1, zz12);
+    m = f29_mul(a, ppsq);                                 // 5 * 2, 2 * 2, 2 * 2 -> < 2
+    const F29 ppp = f29_shfl(m, 0), qq = f29_shfl(m, 1), zz3 = f29_shfl(m, 2);
+    const F29 x3 = f29_sub<7>(r2, f29_add(ppp, f29_add(qq, qq)));  // PPP + 2Q < 6: X3 < 9
+    const F29 d = f29_sub<10>(qq, x3);                   // < 12
+    a = f29_pick(lane, r, s1, zzz12);
+    b = f29_pick(lane, d, ppp, ppp);
+    m = f29_mul(a, b);                                    // 5 * 12, 2 * 2, 2 * 2 -> < 2
+    const F29 y3 = f29_sub<3>(f29_shfl(m, 0), f29_shfl(m, 1));  // < 5
+    return Xyzz29{x3, y3, zz3, f29_shfl(m, 2)};
+}
+
+template <class Fe>  // Fq29 (a template, so that every translation unit may instantiate it)
+__global__ __launch_bounds__(64) void msm_combine29_kernel(const uint32_t* __restrict__ wsums, uint32_t W,
+                                                          uint32_t wb, uint32_t nw, uint32_t* __restrict__ result) {
+    constexpr int XW = 4 * FeTraits<Fq29>::WORDS;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t zero = lane >> 6;  // every lane loads the same sums through a lane-dependent address
+    auto load = [&](uint32_t w, bool* is_inf) {
+        const Xyzz<Fq29> v = load_xyzz<Fq29>(wsums + (size_t)XW * (w + zero));
+        *is_inf = fq29_is_zero(v.zz);
+        Xyzz29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            r.x.l[i] = v.x.l[i];
+            r.y.l[i] = v.y.l[i];
+            r.zz.l[i] = v.zz.l[i];
+            r.zzz.l[i] = v.zzz.l[i];
+        }
+        return r;
+    };
+    bool inf;
+    Xyzz29 acc = load(W - 1, &inf);
+    for (int w = (int)W - 2; w >= 0; --w) {
+        const uint32_t width = wb + ((uint32_t)w < nw ? 1u : 0u);  // bits between window w and w + 1
+        if (!inf)
+            for (uint32_t k = 0; k < width; ++k) acc = xyzz29_dbl_lanes(acc, lane);  // no 2-torsion
+        bool q_inf;
+        const Xyzz29 q = load(w, &q_inf);
+        if (q_inf) continue;
+        if (inf) {
+            acc = q;
+            inf = false;
+            continue;
+        }
+        acc = xyzz29_add_lanes(acc, q, lane, &inf);
+    }
+    if (lane != 0) return;
+    if (inf) {
+        store_xyzz_result(result, xyzz_inf<Fq29>());
+        return;
+    }
+    Xyzz<Fq29> out;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {  // values below 12 p: store_result reduces them to canonical words
+        out.x.l[i] = acc.x.l[i];
+        out.y.l[i] = acc.y.l[i];
+        out.zz.l[i] = acc.zz.l[i];
+        out.zzz.l[i] = acc.zzz.l[i];
+    }
+    store_xyzz_result(result, out);
+}
+
 // ---------------------------------------------------------------- stage launchers
 // Each group's stages are instantiated in their own translation units (msm_g1.hip,
 // msm_g2_*.hip) so that the large, fully inlined G2 kernels compile in parallel.
@@ -580,7 +725,10 @@ void launch_tree(const MsmPlan& p, hipStream_t s) {
 
 template <class Fe>
 void launch_combine(const MsmPlan& p, hipStream_t s) {
-    hipLaunchKernelGGL(msm_combine_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.wb, p.nw, p.result);
+    if constexpr (std::is_same<Fe, Fq29>::value)
+        hipLaunchKernelGGL(msm_combine29_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.wb, p.nw, p.result);
+    else
+        hipLaunchKernelGGL(msm_combine_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.wb, p.nw, p.result);
 }
 
 }  // namespace zkg
