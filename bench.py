@@ -509,7 +509,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "An MSM would shard by points with one K-point gather; see DESIGN.md.", file=sys.stderr)
+              "An MSM would shard by points, each GPU summing its share, with one gather of K points.",
+              file=sys.stderr)
         sys.exit(2)
 
     gpu = zkgpu.Gpu(0)

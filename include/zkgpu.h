@@ -3,8 +3,8 @@
  *
  * This is the drop-in boundary for zerotruss/zk-gpu's public Rust API (src/lib.rs:1-6). Each
  * entry point names the reference item it replaces (file:line in the reference repository).
- * The reference has no C ABI of its own; a Rust facade binds these symbols one-to-one (the
- * binding is shown in INTEGRATION.md).
+ * The reference has no C ABI of its own; a Rust facade binds these symbols one-to-one through
+ * extern "C" declarations of the same names and argument types.
  *
  * Conventions
  *  - Field elements are 8 little-endian u32 words, plain integers below the modulus
