@@ -254,40 +254,40 @@ __global__ __launch_bounds__(256) void msm_merge_kernel(uint32_t* __restrict__ i
 template <class Fe>
 struct LaneOps;
 
-// With split = 1, two lanes per bucket: lane 2 slot + h sums the first (h = 0) or second half of
-// the bucket's at most 16 remaining item sums, and the pair adds their halves through a lane
-// shuffle, which halves the serial chain and doubles the waves; split = 0 is one lane per bucket.
+// 2^split lanes per bucket (split 0, 1 or 2): lane 2^split slot + h sums part h of the bucket's
+// at most 16 remaining item sums, and the lanes of a bucket add their parts through lane shuffles
+// (split rounds), which shortens the serial chain and multiplies the waves.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __restrict__ item_pts,
                                                           const uint32_t* __restrict__ istarts, uint32_t S,
                                                           uint32_t split, uint32_t* __restrict__ buckets) {
     typedef FeTraits<Fe> Tr;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t slot = t >> split, half = t & split;
-    const bool valid = slot < S;  // every lane reaches the shuffle
+    const uint32_t slot = t >> split, part = t & ((1u << split) - 1);
+    const bool valid = slot < S;  // every lane reaches the shuffles
     const size_t xw = 4 * Tr::WORDS;
     Xyzz<Fe> acc = xyzz_inf<Fe>();
     if (valid) {
         const uint32_t first = istarts[slot], total = istarts[slot + 1] - first;
         uint32_t stride = 1;
         while ((total + stride - 1) / stride > FINAL_GROUP) stride *= SUM_GROUP;
-        const uint32_t m = (total + stride - 1) / stride;  // terms k * stride, k < m
-        const uint32_t h = split ? (m + 1) / 2 : m;        // the first lane's share
-        const uint32_t lo = half ? h : 0u, hi = half ? m : h;
+        const uint32_t m = (total + stride - 1) / stride;        // terms k * stride, k < m
+        const uint32_t share = (m + (1u << split) - 1) >> split;  // terms per part
+        const uint32_t lo = min(m, part * share), hi = min(m, lo + share);
         if (lo < hi) {
             acc = load_xyzz<Fe>(item_pts + xw * (first + (size_t)lo * stride));
             for (uint32_t k = lo + 1; k < hi; ++k)
                 acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (first + (size_t)k * stride)));
         }
     }
-    if (split) {  // uniform over the grid
+    for (uint32_t r = 0; r < split; ++r) {  // uniform over the grid
         typedef LaneOps<Fe> Lx;
-        const int src = (int)(__lane_id() ^ 1u);
+        const int src = (int)(__lane_id() ^ (1u << r));
         const Xyzz<Fe> other{Lx::shfl(acc.x, src), Lx::shfl(acc.y, src), Lx::shfl(acc.zz, src),
                              Lx::shfl(acc.zzz, src)};
         acc = xyzz_add(acc, other);
     }
-    if (valid && !half) store_xyzz(buckets + xw * slot, acc);
+    if (valid && part == 0) store_xyzz(buckets + xw * slot, acc);
 }
 
 // ---------------------------------------------------------------- bucket reduction
@@ -727,10 +727,11 @@ void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
 template <class Fe>
 void launch_finalize(const MsmPlan& p, hipStream_t s) {
     const uint32_t S = p.W * p.B;
-    // Two lanes per bucket while the buckets alone leave the GPU short of waves. Measured on
-    // MI355X: G1 2^20 (147456 buckets) 0.20 -> 0.15 ms, G2 2^16 / 2^20 0.39 / 0.55 -> 0.27 / 0.47 ms;
-    // at 2^18 buckets and more one lane each is faster (G1 2^22 0.18 against 0.20, G2 0.62 against 0.80).
-    const uint32_t split = S < (1u << 18) ? 1u : 0u;
+    // Several lanes per bucket while the buckets alone leave the GPU short of waves. Measured on
+    // MI355X: G1 2^20 (147456 buckets, two lanes) 0.20 -> 0.16 ms, G2 2^20 0.55 -> 0.47 ms; G2 2^16
+    // (12288 buckets) 0.39 -> 0.27 ms with two lanes, 0.14 ms with four; at 2^18 buckets and more
+    // one lane each is faster (G1 2^22 0.18 against 0.20, G2 0.62 against 0.80 with two).
+    const uint32_t split = S < (1u << 15) ? 2u : S < (1u << 18) ? 1u : 0u;
     hipLaunchKernelGGL(msm_finalize_kernel<Fe>, dim3(ceil_div((size_t)S << split, 256)), dim3(256), 0, s, p.item_pts,
                        p.istarts, S, split, p.buckets);
 }
