@@ -106,6 +106,15 @@ uint32_t zkg_msm_default_window(size_t n);
  * Errors: n = 0 -> "an MSM needs at least one point"; c outside 9..=20 ->
  * "window size c must be in 9..=20, got {c}". */
 int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t item_points, zkg_msm** out);
+/* The checks Msm::for_group makes (src/msm.rs:96-98) plus this library's size limits, without a
+ * device and without allocating: ZKG_OK and the plan's device workspace in bytes when
+ * zkg_msm_create (precomputed = 0) or zkg_msm_create_precomputed (precomputed = 1) would accept
+ * (group, n, c, item_points); otherwise the error those calls return. The library indexes the
+ * sorted (point, digit) entries with 32 bits, so plans needing 2^32 or more of them (W * nv, with
+ * W windows over nv virtual points) are rejected: "an MSM of {n} points at c = {c} needs {k}
+ * sorted entries; at most 2^32 - 1 fit one plan". */
+int zkg_msm_plan_check(int group, size_t n, uint32_t c, uint32_t item_points, int precomputed,
+                       size_t* workspace_bytes);
 /* No counterpart in the reference: a plan whose window, work-item size and sort-block size come
  * from the tuning table (zkg_tune_msm / zkg_tune_load) for this GPU and size; untuned sizes use
  * default_window(n) with automatic item and sort sizes. Results equal any other plan's. */

@@ -94,7 +94,15 @@ def test_host_module_mirror():
     assert host.words32(host.from_words32(list(range(8)))) == list(range(8))
     x = host.from_int(123)
     assert host.mul_mod(x, host.inverse(x, r), r) == host.from_int(1)
-    assert host.to_int(host.montgomery_r(r)) == (1 << 256) % host.to_int(r)
+    # the reference's contract: R = 2^260 (src/host.rs:84-91), reached by 260 doublings there
+    rr = 1
+    for _ in range(260):
+        rr = (rr + rr) % host.to_int(r)
+    assert host.to_int(host.montgomery_r(r)) == rr
+    assert host.from_montgomery(host.mul_mod(x, host.montgomery_r(r), r), r) == x
+    # this library's device form: R = 2^256
+    assert host.to_int(host.library_montgomery_r(r)) == (1 << 256) % host.to_int(r)
+    assert host.from_library_montgomery(host.mul_mod(x, host.library_montgomery_r(r), r), r) == x
 
 
 def test_device_open_without_gpu_fails_loudly():
@@ -125,3 +133,34 @@ def test_kernels_contain_no_device_calls():
     script = os.path.join(ROOT, "zk-gpu_amd", "check_code_objects.py")
     r = subprocess.run([sys.executable, script] + objs, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_plan_bounds_reject_32_bit_index_overflow():
+    """Plans whose sorted (point, digit) entries W * nv would reach 2^32 are rejected on the host,
+    before any allocation; the reference's own argument errors keep their text (src/msm.rs:96-98)."""
+    from zkgpu import Group, Msm, ZkgError
+
+    # accepted: the bench's plans; workspace grows with n
+    small = Msm.plan_check(Group.G1, 1 << 16, 11)
+    big = Msm.plan_check(Group.G1, 1 << 20, 15)
+    g2 = Msm.plan_check(Group.G2, 1 << 20, 15)
+    assert 0 < small < big < g2
+    # GLV: nv = 2n virtual points over ceil(128 / c) windows
+    assert Msm.plan_check(Group.G1, (1 << 28) - 1, 16) > 0  # 8 (2^29 - 2) < 2^32
+    for n, c in [(1 << 28, 16), (1 << 29, 16), (150_000_000, 9), ((1 << 30) - 1, 20)]:
+        windows = -(-128 // c)
+        assert windows * 2 * n >= 1 << 32
+        with pytest.raises(ZkgError, match="sorted entries; at most 2\\^32 - 1 fit one plan"):
+            Msm.plan_check(Group.G1, n, c)
+    # precomputed bases: 4n virtual points over ceil(64 / c) windows, n < 2^29
+    assert Msm.plan_check(Group.G1, 1 << 20, 16, precomputed=True) > 0
+    with pytest.raises(ZkgError, match="sorted entries"):
+        Msm.plan_check(Group.G1, (1 << 29) - 1, 9, precomputed=True)  # 8 * 4 (2^29 - 1) >= 2^32
+    with pytest.raises(ZkgError, match="c in 9..=16, got 17"):
+        Msm.plan_check(Group.G1, 1000, 17, precomputed=True)
+    with pytest.raises(ZkgError, match="an MSM needs at least one point"):
+        Msm.plan_check(Group.G1, 0, 11)
+    with pytest.raises(ZkgError, match="window size c must be in 9..=20, got 8"):
+        Msm.plan_check(Group.G1, 10, 8)
+    with pytest.raises(ZkgError, match="at most 2\\^31 - 1 points"):
+        Msm.plan_check(Group.G1, 1 << 31, 16)

@@ -363,6 +363,15 @@ int zkg_msm_create(zkg_device* dev, int group, size_t n, uint32_t c, uint32_t it
     ZKG_GUARD_END
 }
 
+int zkg_msm_plan_check(int group, size_t n, uint32_t c, uint32_t item_points, int precomputed,
+                       size_t* workspace_bytes) {
+    ZKG_GUARD_BEGIN
+    const zkg::MsmShape sh = zkg::msm_shape(group, n, c, item_points, 0, precomputed != 0);
+    if (workspace_bytes) *workspace_bytes = sh.workspace_bytes(n);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
 int zkg_msm_create_tuned(zkg_device* dev, int group, size_t n, zkg_msm** out) {
     ZKG_REQUIRE(dev && out, "null argument");
     ZKG_GUARD_BEGIN

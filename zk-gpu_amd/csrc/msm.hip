@@ -558,43 +558,90 @@ uint32_t msm_plan_item_points(size_t n, uint32_t c) {
     return msm_auto_item_points(msm_glv_enabled(n) ? 2 * n : n, c);
 }
 
+MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uint32_t sort_points,
+                   bool precomputed) {
+    if (npts < 1) throw Error(ZKG_ERR_INVALID, "an MSM needs at least one point");
+    if (c < 9 || c > 20) throw Error(ZKG_ERR_INVALID, "window size c must be in 9..=20, got " + std::to_string(c));
+    if (group != 1 && group != 2) throw Error(ZKG_ERR_INVALID, "group must be G1 (1) or G2 (2)");
+    if (npts > 0x7fffffffu) throw Error(ZKG_ERR_INVALID, "an MSM holds at most 2^31 - 1 points");
+    MsmShape sh;
+    sh.pre = precomputed;
+    sh.glv = msm_glv_enabled(npts);
+    if (sh.pre) {
+        if (c > 16)
+            throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases takes c in 9..=16, got " + std::to_string(c));
+        // 4n virtual point indices must fit the 31-bit index field of a sorted entry
+        if (!sh.glv || npts > 0x1fffffffu)
+            throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases needs the GLV split and n < 2^29");
+    }
+    const uint64_t n = npts;
+    const uint64_t nv = sh.pre ? 4 * n : sh.glv ? 2 * n : n;
+    const uint32_t sbits = sh.pre ? 64 : 128;  // bits per (half-)scalar the windows cover
+    sh.nv = (uint32_t)nv;
+    sh.W = sh.glv ? (sbits + c - 1) / c : (255 + c - 1) / c;
+    sh.wb = sh.glv ? sbits / sh.W : c;
+    sh.nw = sh.glv ? sbits - sh.wb * sh.W : 0;
+    sh.B = 1u << (c - 1);
+    // sorted entries are addressed by 32-bit positions (the slot scan's starts) and hold W of
+    // them per virtual point
+    if ((uint64_t)sh.W * nv > 0xffffffffull)
+        throw Error(ZKG_ERR_INVALID, "an MSM of " + std::to_string(n) + " points at c = " + std::to_string(c) +
+                                         " needs " + std::to_string((uint64_t)sh.W * nv) +
+                                         " sorted entries; at most 2^32 - 1 fit one plan");
+    sh.CH = item_points ? item_points : msm_auto_item_points(nv, c);
+    sh.L = msm_reduce_chunk(group, c, sh.W);
+    sh.T = sh.B / sh.L;
+    sh.aw = group == 1 ? 16 : 32;
+    sh.xw = 2 * sh.aw;
+    const uint64_t S = (uint64_t)sh.W * sh.B;
+    sh.max_items = (size_t)sh.W * ((nv + sh.CH - 1) / sh.CH + sh.B);
+    if (sh.max_items >= 0xffffffffull) throw Error(ZKG_ERR_INVALID, "too many work items for one plan");
+    for (uint64_t m = (nv + sh.CH - 1) / sh.CH; m > FINAL_GROUP; m = (m + SUM_GROUP - 1) / SUM_GROUP) ++sh.merge_rounds;
+    sh.scan_blocks = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    sh.lds_sort = c <= 16;
+    sh.sort_pts = sort_points ? std::max<uint32_t>(sort_points, sh.B) : msm_default_sort_points(nv, c);
+    sh.nblk = (uint32_t)((nv + sh.sort_pts - 1) / sh.sort_pts);
+    return sh;
+}
+
+// Bytes of the allocations MsmPlan::MsmPlan makes for this shape (n real points).
+size_t MsmShape::workspace_bytes(size_t n) const {
+    const size_t S = (size_t)W * B;
+    const size_t words = (glv ? 8 * n + (pre ? 2 : 1) * n * xw / 2 : 0)  // split, endo_x
+                         + 4 * S + 2                                      // counts, cursors, starts, istarts
+                         + 3 * scan_blocks + 1                            // scan_tmp
+                         + (size_t)W * nv * (lds_sort ? 2 : 1)            // sorted (+ digits)
+                         + (lds_sort ? (size_t)W * nblk * B : 0)          // table
+                         + max_items * (2 + (size_t)xw)                   // item_info, item_pts
+                         + S * xw + (size_t)W * T * xw + (size_t)W * xw + xw;  // buckets, partials, wsums, result
+    return 4 * words;
+}
+
 MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_points, uint32_t sort_points,
                  bool precomputed)
     : dev(d), group(grp), pre(precomputed) {
-    if (npts < 1) throw Error(ZKG_ERR_INVALID, "an MSM needs at least one point");
-    if (win < 9 || win > 20) throw Error(ZKG_ERR_INVALID, "window size c must be in 9..=20, got " + std::to_string(win));
-    if (grp != 1 && grp != 2) throw Error(ZKG_ERR_INVALID, "group must be G1 (1) or G2 (2)");
-    if (npts > 0x7fffffffu) throw Error(ZKG_ERR_INVALID, "an MSM holds at most 2^31 - 1 points");
+    const MsmShape sh = msm_shape(grp, npts, win, item_points, sort_points, precomputed);
     n = (uint32_t)npts;
     c = win;
-    glv = msm_glv_enabled(npts);
-    if (pre) {
-        if (win > 16) throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases takes c in 9..=16, got " +
-                                                       std::to_string(win));
-        // 4n virtual point indices must fit the 31-bit index field of a sorted entry
-        if (!glv || npts > 0x1fffffffu)
-            throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases needs the GLV split and n < 2^29");
-    }
+    glv = sh.glv;
     f29 = msm_f29_enabled();
-    nv = pre ? 4 * n : glv ? 2 * n : n;
-    const uint32_t sbits = pre ? 64 : 128;  // bits per (half-)scalar the windows cover
-    W = glv ? (sbits + c - 1) / c : (255 + c - 1) / c;
-    wb = glv ? sbits / W : c;
-    nw = glv ? sbits - wb * W : 0;
-    B = 1u << (c - 1);
-    CH = item_points ? item_points : msm_auto_item_points(nv, c);
-    L = msm_reduce_chunk(grp, c, W);
-    T = B / L;
-    aw = group == 1 ? 16 : 32;
-    xw = 2 * aw;
+    nv = sh.nv;
+    W = sh.W;
+    wb = sh.wb;
+    nw = sh.nw;
+    B = sh.B;
+    CH = sh.CH;
+    L = sh.L;
+    T = sh.T;
+    aw = sh.aw;
+    xw = sh.xw;
+    max_items = sh.max_items;
+    merge_rounds = sh.merge_rounds;
+    lds_sort = sh.lds_sort;
+    sort_pts = sh.sort_pts;
+    nblk = sh.nblk;
     const size_t S = (size_t)W * B;
-    max_items = (size_t)W * (((size_t)nv + CH - 1) / CH + B);
-    if (max_items >= 0xffffffffull) throw Error(ZKG_ERR_INVALID, "too many work items for one plan");
-    for (uint64_t m = ((uint64_t)nv + CH - 1) / CH; m > FINAL_GROUP; m = (m + SUM_GROUP - 1) / SUM_GROUP) ++merge_rounds;
-    const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    lds_sort = c <= 16;
-    sort_pts = sort_points ? std::max<uint32_t>(sort_points, B) : msm_default_sort_points(nv, c);
-    nblk = (nv + sort_pts - 1) / sort_pts;
+    const size_t nb = sh.scan_blocks;
     device_bind(d);
     if (lds_sort) {
         // 4 * B bytes of LDS per sort block, up to 128 KB at c = 16. The limit is a property of
@@ -628,7 +675,10 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
         ZKG_HIP(hipMalloc(&partials, 4 * (size_t)W * T * xw));
         ZKG_HIP(hipMalloc(&wsums, 4 * (size_t)W * xw));
         ZKG_HIP(hipMalloc(&result, 4 * (size_t)xw));
+        // Zeroed before the constructor returns: a plan may next run on another stream (a batch's
+        // or a server slot's), which nothing would order after a queued clear.
         ZKG_HIP(hipMemsetAsync(result, 0, 4 * (size_t)xw, d->stream));
+        ZKG_HIP(hipStreamSynchronize(d->stream));
     } catch (...) {
         this->~MsmPlan();
         throw;
@@ -681,20 +731,7 @@ bool msm_f29_enabled() {
 
 size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points,
                            bool precomputed) {
-    const bool glv = msm_glv_enabled(n), pre = precomputed && glv;
-    const size_t nv = pre ? 4 * n : glv ? 2 * n : n;
-    const uint32_t sbits = pre ? 64 : 128;
-    const uint32_t W = glv ? (sbits + c - 1) / c : (255 + c - 1) / c, B = 1u << (c - 1);
-    const uint32_t CH = item_points ? item_points : msm_auto_item_points(nv, c);
-    const uint32_t xw = group == 1 ? 32 : 64, T = B / msm_reduce_chunk(group, c, W);
-    const size_t S = (size_t)W * B, max_items = (size_t)W * ((nv + CH - 1) / CH + B);
-    const bool lds = c <= 16;
-    const size_t P = sort_points ? std::max<uint32_t>(sort_points, B) : msm_default_sort_points(nv, c);
-    const size_t nblk = (nv + P - 1) / P;
-    const size_t nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    return 4 * ((glv ? 8 * n + (pre ? 2 : 1) * n * xw / 2 : 0) + 4 * S + 2 + 3 * nb + 1 + (size_t)W * nv * (lds ? 2 : 1) +
-                (lds ? (size_t)W * nblk * B : 0) + max_items * (2 + (size_t)xw) + S * xw + (size_t)W * T * xw +
-                (size_t)W * xw + xw);
+    return msm_shape(group, n, c, item_points, sort_points, precomputed).workspace_bytes(n);
 }
 
 template <class Fe>

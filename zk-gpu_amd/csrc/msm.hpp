@@ -16,8 +16,23 @@ uint32_t msm_default_window(size_t n);          // clamp(ilog2(max(n,1)) - 5, 11
 uint32_t msm_auto_item_points(size_t n, uint32_t c);
 // The work-item size a plan for (n, c) picks when none is given.
 uint32_t msm_plan_item_points(size_t n, uint32_t c);
+// Everything a plan for (group, n, c, item_points, sort_points, precomputed) derives from its
+// arguments (0 = automatic), with the argument checks of Msm::for_group (src/msm.rs:96-98) and the
+// index-range checks of this library's kernels: positions into the sorted point list, work-item
+// numbers and virtual point indices are 32-bit, so a plan whose W * nv entries or work items
+// reach 2^32 is rejected here (an Error with a message), before anything is allocated.
+struct MsmShape {
+    bool glv = false, pre = false, lds_sort = false;
+    uint32_t nv = 0, W = 0, wb = 0, nw = 0, B = 0, CH = 0, L = 0, T = 0, aw = 0, xw = 0;
+    size_t max_items = 0;
+    uint32_t merge_rounds = 0, sort_pts = 0, nblk = 0;
+    size_t scan_blocks = 0;  // blocks of the slot scan
+    size_t workspace_bytes(size_t n) const;
+};
+MsmShape msm_shape(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0,
+                   bool precomputed = false);
 // Device bytes a plan for (group, n, c, item_points, sort_points, precomputed) allocates
-// (0 = automatic).
+// (0 = automatic); throws as msm_shape for arguments no plan accepts.
 size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0,
                            bool precomputed = false);
 // Whether plans use the GLV split (default on; ZKGPU_MSM_GLV=0 turns it off for comparisons).

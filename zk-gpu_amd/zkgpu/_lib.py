@@ -42,6 +42,8 @@ SIGNATURES = {
     "zkg_msm_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                       c_void_pp]),
     "zkg_msm_create_tuned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, c_void_pp]),
+    "zkg_msm_plan_check": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
     "zkg_msm_item_points": (ctypes.c_uint32, [ctypes.c_void_p]),
     "zkg_msm_create_precomputed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32,
                                                   ctypes.c_uint32, c_void_pp]),

@@ -1,9 +1,12 @@
 """256-bit host arithmetic — mirror of the reference's `host` module (src/host.rs:1-115).
 
 U256 is a tuple of four little-endian u64 limbs, as in the reference ([u64; 4]).
-`montgomery_r` returns the library's Montgomery radix R mod m; this library's kernels use
-R = 2^256 (the reference's WGSL kernels used 2^260), so `from_montgomery` decodes this
-library's Montgomery words.
+
+`montgomery_r` and `from_montgomery` keep the reference's contract unchanged: R = 2^260, the radix
+of the reference's WGSL kernels (src/host.rs:84-91,112-115), so callers that decode 2^260-form
+words with them get the same values as before. This library's kernels use R = 2^256: its device
+words (uploaded bases, MSM results, .zkey coordinates) are decoded with `library_montgomery_r` and
+`from_library_montgomery`.
 """
 from __future__ import annotations
 
@@ -13,7 +16,8 @@ U256 = Tuple[int, int, int, int]
 
 FQ: U256 = (0x3C208C16D87CFD47, 0x97816A916871CA8D, 0xB85045B68181585D, 0x30644E72E131A029)
 FR: U256 = (0x43E1F593F0000001, 0x2833E84879B97091, 0xB85045B68181585D, 0x30644E72E131A029)
-MONTGOMERY_BITS = 256
+MONTGOMERY_BITS = 260  # the reference's radix (src/host.rs:84)
+LIBRARY_MONTGOMERY_BITS = 256  # this library's device form
 
 
 def to_int(a: Sequence[int]) -> int:
@@ -45,8 +49,13 @@ def shr(a: U256, k: int) -> U256:
 
 
 def montgomery_r(m: U256) -> U256:
-    """R mod m for the library's Montgomery radix R = 2^256."""
+    """R mod m for the reference kernels' Montgomery radix R = 2^260 (src/host.rs:84-91)."""
     return from_int((1 << MONTGOMERY_BITS) % to_int(m))
+
+
+def library_montgomery_r(m: U256) -> U256:
+    """R mod m for this library's Montgomery radix R = 2^256 (its device words)."""
+    return from_int((1 << LIBRARY_MONTGOMERY_BITS) % to_int(m))
 
 
 def root_of_unity(log_n: int) -> U256:
@@ -69,5 +78,12 @@ def from_words32(w) -> U256:
 
 
 def from_montgomery(x: U256, m: U256) -> U256:
-    """x * R^-1 mod m: a Montgomery-form value of this library back to its plain integer."""
+    """x * R^-1 mod m with R = 2^260: a value in the reference kernels' Montgomery form back to its
+    plain integer (src/host.rs:112-115)."""
     return mul_mod(x, inverse(montgomery_r(m), m), m)
+
+
+def from_library_montgomery(x: U256, m: U256) -> U256:
+    """x * 2^-256 mod m: a Montgomery-form word of this library's device buffers back to its plain
+    integer."""
+    return mul_mod(x, inverse(library_montgomery_r(m), m), m)

@@ -51,6 +51,18 @@ class Msm:
         return Msm(gpu, group, n, c, item_points)
 
     @staticmethod
+    def plan_check(group: Group, n: int, c: int, item_points: Optional[int] = None,
+                   precomputed: bool = False) -> int:
+        """Device workspace bytes of the plan `for_group` (or `precomputed`) would make, checked
+        without a device; raises ZkgError with the same message as the constructor for a plan it
+        would reject (the reference's argument checks, src/msm.rs:96-98, and this library's 32-bit
+        index limits). No counterpart in the reference."""
+        out = ctypes.c_size_t()
+        check(_lib.load().zkg_msm_plan_check(group.value, int(n), int(c), int(item_points or 0), int(precomputed),
+                                             ctypes.byref(out)))
+        return int(out.value)
+
+    @staticmethod
     def tuned(gpu, group: Group, n: int) -> "Msm":
         """A plan with the tuning table's window, item and sort sizes for this GPU and size
         (default_window(n) when the size is untuned). No counterpart in the reference."""
