@@ -147,14 +147,32 @@ size_t zkg_msm_points(const zkg_msm* plan);
  * library's Montgomery form. Group-agnostic: n_words is 8 per coordinate. */
 int zkg_msm_upload_bases(zkg_device* dev, const uint32_t* plain_xy, size_t n_words, zkg_buffer** out);
 /* Msm::upload_bases_montgomery_256 (src/msm.rs:190-197): coordinates already in Montgomery form
- * with R = 2^256 (the snarkjs/ffjavascript form). The library's R is 2^256, so this is a copy. */
+ * with R = 2^256 (the snarkjs/ffjavascript form). The library's R is 2^256, so this is a copy
+ * followed by a reduction of every word below p on the device (any 256-bit word is accepted, as the
+ * reference's scaling product accepts it). */
 int zkg_msm_upload_bases_montgomery_256(zkg_device* dev, const uint32_t* xy, size_t n_words, zkg_buffer** out);
 /* Convert an existing device buffer of plain coordinates to the library's form in place
  * (what upload_bases does after its copy; for bases produced on the device). */
 int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words);
+/* No counterpart in the reference (it checks nothing about its inputs, README.md:218-219): counts
+ * the first n points of `bases` (library form, as the upload calls leave them) that are neither
+ * infinity nor on the curve of `group` (G1: y^2 = x^3 + 3; G2: the twist y^2 = x^3 + 3/(9+u)) and,
+ * for G2, in the order-r subgroup (r P = O, by double-and-add on the device); *first_bad = the
+ * smallest such index (0 when *bad = 0). Waits for the device.
+ *
+ * Why it matters: plans split scalars with the GLV endomorphism phi(x, y) = (beta x, y), which
+ * acts as multiplication by lambda only on the order-r subgroup. G1 has cofactor 1 (on the curve
+ * is enough), but the G2 twist has cofactor 2p - r: a G2 base on the twist but outside the
+ * subgroup gives a different sum than the reference's plain Pippenger. Bases of a Groth16 proving
+ * key are subgroup points; zkg_zkey_upload and zkg_server_add_zkey run this check themselves. */
+int zkg_msm_check_bases(zkg_device* dev, int group, const zkg_buffer* bases, size_t n, uint64_t* bad,
+                        uint64_t* first_bad);
 /* Msm::run (src/msm.rs:284-298): queue sum(scalars[i] * bases[i]). *result receives the
  * plan-owned buffer holding one XYZZ point (32 words G1, 64 words G2, Montgomery form), valid
- * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...". */
+ * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...".
+ * Precondition (not checked here, see zkg_msm_check_bases): G2 bases lie in the order-r subgroup.
+ * ZKGPU_MSM_GLV=0 in the environment at plan creation selects plans without the GLV split, which
+ * sum any points of the twist exactly as the reference does. */
 int zkg_msm_run(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
 /* Msm::run_yielding (src/msm.rs:300-318): as zkg_msm_run, but waits for the GPU after the
  * heavy stages (accumulate, finalize, reduce chunks, combine), so a display sharing the GPU gets it
@@ -283,7 +301,8 @@ int zkg_server_add_key_shaped(zkg_server* server, const zkg_key_shape* shape, zk
 /* Upload a snarkjs key's A, B1, C, H and B2 sections into server-owned HBM (freed with the
  * server) as a key of shape (log2 domainSize; nVars, nVars, nVars - nPublic - 1, domainSize,
  * nVars; c_offset = nPublic + 1): the witness is snarkjs's full assignment and polynomial 6's
- * outputs are the H scalars. Fails unless log2(domainSize) is in 10..=27. */
+ * outputs are the H scalars. Fails unless log2(domainSize) is in 10..=27, and (as zkg_zkey_upload)
+ * when a section holds a point off the curve, or a B2 point outside the order-r subgroup. */
 int zkg_server_add_zkey(zkg_server* server, const zkg_zkey* zkey, int* key_id);
 int zkg_server_key_shape(zkg_server* server, int key_id, zkg_key_shape* out);
 int zkg_server_slot_bytes(zkg_server* server, uint32_t log_n, size_t* bytes);
@@ -318,8 +337,12 @@ int zkg_zkey_open(const char* path, zkg_zkey** out);
 int zkg_zkey_close(zkg_zkey* zkey);
 int zkg_zkey_get_info(const zkg_zkey* zkey, zkg_zkey_info* out);
 int zkg_zkey_points(const zkg_zkey* zkey, int section, size_t* count);
-/* A point section as MSM bases (the stored words are the library's form: a copy,
- * as zkg_msm_upload_bases_montgomery_256). */
+/* A point section as MSM bases (the stored words are the library's form: a copy reduced below p,
+ * as zkg_msm_upload_bases_montgomery_256), then checked on the device as zkg_msm_check_bases does:
+ * a section holding a point that is neither infinity nor on the curve (G1) / on the twist and in
+ * the order-r subgroup (B2) is rejected ("the .zkey section {s} holds {k} point(s) off the curve
+ * ..." with the first index) and no buffer is returned. Swapped coordinates or plain
+ * (non-Montgomery) words fail this check. */
 int zkg_zkey_upload(zkg_device* dev, const zkg_zkey* zkey, int section, zkg_buffer** out);
 
 /* ------------------------------------------------------------------ test and benchmark hooks

@@ -689,6 +689,34 @@ void orc_poly_eval(const uint32_t* coeffs, size_t n, const uint32_t* x, uint32_t
     f_to_words(out, acc, F);
 }
 
+// The same evaluation over `threads` threads: P(x) = sum_t x^lo_t P_t(x), each part P_t (the
+// coefficients lo_t .. hi_t - 1) by Horner on its own thread.
+void orc_poly_eval_threads(const uint32_t* coeffs, size_t n, const uint32_t* x, uint32_t* out, int threads) {
+    const Field& F = FR();
+    const int t = std::max(1, threads);
+    const Fe xv = f_from_words(x, F);
+    std::vector<Fe> part(t, Fe{{0, 0, 0, 0}});
+    std::vector<std::thread> pool;
+    for (int k = 0; k < t; ++k) {
+        const size_t lo = n * k / t, hi = n * (k + 1) / t;
+        pool.emplace_back([&, k, lo, hi] {
+            Fe acc = Fe{{0, 0, 0, 0}};
+            for (size_t i = hi; i-- > lo;) {
+                f_mul(acc, acc, xv, F);
+                f_add(acc, acc, f_from_words(coeffs + 8 * i, F), F);
+            }
+            const uint64_t e[4] = {(uint64_t)lo, 0, 0, 0};
+            Fe shifted;
+            f_mul(shifted, acc, f_pow(xv, e, F), F);
+            part[k] = shifted;
+        });
+    }
+    for (auto& th : pool) th.join();
+    Fe acc = Fe{{0, 0, 0, 0}};
+    for (const Fe& v : part) f_add(acc, acc, v, F);
+    f_to_words(out, acc, F);
+}
+
 // sum a_i b_i mod r (plain words in, plain words out; every a_i, b_i < r). With bases k_i G this is
 // the scalar of the closed form sum s_i (k_i G) = (sum k_i s_i) G.
 void orc_dot_fr(const uint32_t* a, const uint32_t* b, size_t n, uint32_t* out, int threads) {

@@ -76,10 +76,11 @@ def lib():
         L.orc_point_add.argtypes = [i, u32p, u32p, u32p]
         L.orc_point_mul.argtypes = [i, u32p, u32p, u32p]
         L.orc_dot_fr.argtypes = [u32p, u32p, sz, u32p, i]
+        L.orc_poly_eval_threads.argtypes = [u32p, sz, u32p, u32p, i]
         for f in ("orc_ntt", "orc_poly_eval", "orc_msm", "orc_msm_naive", "orc_multiples",
                   "orc_point_add", "orc_point_mul", "orc_field_mul", "orc_field_add",
                   "orc_field_sub", "orc_field_inv", "orc_field_pow", "orc_fq2_mul",
-                  "orc_fq2_inv", "orc_root_of_unity", "orc_dot_fr"):
+                  "orc_fq2_inv", "orc_root_of_unity", "orc_dot_fr", "orc_poly_eval_threads"):
             getattr(L, f).restype = None
         _lib = L
     return _lib
@@ -255,10 +256,16 @@ def groth16_h_odd_coset(a_words, b_words, c_words, log_n: int) -> np.ndarray:
     return ints_to_words([(x * y - z) % R for x, y, z in zip(*evals)])
 
 
-def poly_eval(words, x: int) -> int:
+def poly_eval(words, x: int, threads: int | None = None) -> int:
+    """sum_i c_i x^i mod r by Horner; threads > 1 (the default: the host's cores) splits the
+    coefficients into parts evaluated in parallel and scaled by x^start."""
     coeffs = _u32(words)
     out = np.zeros(8, dtype=np.uint32)
-    lib().orc_poly_eval(coeffs, coeffs.size // 8, int_to_words(x), out)
+    t = threads or default_threads()
+    if t == 1:
+        lib().orc_poly_eval(coeffs, coeffs.size // 8, int_to_words(x), out)
+    else:
+        lib().orc_poly_eval_threads(coeffs, coeffs.size // 8, int_to_words(x), out, t)
     return words_to_int(out)
 
 

@@ -70,6 +70,39 @@ def test_ntt_large_sizes(gpu, log_n):
     assert np.array_equal(gpu.read(buf, x.size), x)
 
 
+@pytest.mark.parametrize("log_n", [25, 26, 27])
+def test_ntt_largest_sizes(gpu, log_n):
+    """The sizes up to the reference's limit (src/ntt.rs:27: log_n in 10..=27): Horner evaluation of
+    the input at omega^j for sampled j (independent of any FFT structure) equals output j, for the
+    forward and the inverse transform (inverse: (1/n) sum x_i omega^-ij), and the round trip
+    restores the input. Elements are uniform below 2^252 (< r), drawn as words directly."""
+    from zkgpu import Ntt
+
+    if quick():
+        pytest.skip("ZK_GPU_QUICK")
+    rng = np.random.default_rng(300 + log_n)
+    n = 1 << log_n
+    x = rng.integers(0, 1 << 32, size=(n, 8), dtype=np.uint32)
+    x[:, 7] &= 0x0FFFFFFF
+    x = x.reshape(-1)
+    ntt = Ntt(gpu, log_n)
+    buf = gpu.buffer(x)
+    ntt.forward(gpu, buf)
+    y = gpu.read(buf, x.size)
+    w = O.py_root_of_unity(log_n)
+    js = (1, int(rng.integers(2, n - 1)), n - 1)
+    for j in js:
+        assert O.poly_eval(x, pow(w, j, O.R)) == O.words_to_int(y[8 * j: 8 * j + 8]), ("forward", j)
+    ntt.inverse(gpu, buf)
+    assert np.array_equal(gpu.read(buf, x.size), x)
+    ntt.inverse(gpu, buf)
+    z = gpu.read(buf, x.size)
+    n_inv = pow(n, O.R - 2, O.R)
+    winv = pow(w, O.R - 2, O.R)
+    for j in js[:2]:
+        assert O.poly_eval(x, pow(winv, j, O.R)) * n_inv % O.R == O.words_to_int(z[8 * j: 8 * j + 8]), ("inverse", j)
+
+
 def test_ntt_of_a_delta_is_a_power_sequence(gpu):
     """tests/properties.rs:25-46."""
     from zkgpu import Ntt

@@ -92,6 +92,20 @@ def test_ntt_horner_and_roundtrip():
     assert np.array_equal(O.ntt(y, log_n, inverse=True), x)
 
 
+def test_poly_eval_threaded_vs_textbook():
+    """The threaded evaluation (parts scaled by x^start) equals Horner on one thread and the
+    textbook sum on Python integers, for uneven splits and more threads than coefficients."""
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 3, 7, 100, 1001):
+        c = uniform_fr_words(rng, n)
+        ints = O.words_to_ints(c)
+        for x in (0, 1, 2, O.R - 1, int(rng.integers(1, 1 << 62)) ** 4 % O.R):
+            exp = sum(ci * pow(x, i, O.R) for i, ci in enumerate(ints)) % O.R
+            assert O.poly_eval(c, x, threads=1) == exp, (n, x)
+            for t in (2, 3, 16):
+                assert O.poly_eval(c, x, threads=t) == exp, (n, x, t)
+
+
 def test_g1_point_ops_vs_textbook():
     rng = np.random.default_rng(11)
     ks = [int.from_bytes(rng.bytes(32), "little") % O.R for _ in range(6)]

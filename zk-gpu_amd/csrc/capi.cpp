@@ -434,14 +434,17 @@ int zkg_msm_group(const zkg_msm* m) { return m ? m->plan->group : 0; }
 uint32_t zkg_msm_window(const zkg_msm* m) { return m ? m->plan->c : 0; }
 size_t zkg_msm_points(const zkg_msm* m) { return m ? m->plan->n : 0; }
 
+// convert: plain coordinates -> Montgomery form; else Montgomery-2^256 words, reduced below p
 static int upload(zkg_device* dev, const uint32_t* xy, size_t n_words, zkg_buffer** out, bool convert) {
     ZKG_REQUIRE(dev && out && (xy || n_words == 0), "null argument");
     ZKG_REQUIRE(n_words % 8 == 0, "coordinates are 8 words each");
     int rc = zkg_buffer_create(dev, xy, n_words, out);
     if (rc != zkg::ZKG_OK) return rc;
-    if (!convert || n_words == 0) return zkg::ZKG_OK;
+    if (n_words == 0) return zkg::ZKG_OK;
     ZKG_GUARD_BEGIN
-    zkg::msm_convert_bases(static_cast<uint32_t*>((*out)->b->ptr), n_words / 8, dev->d->stream);
+    uint32_t* p = static_cast<uint32_t*>((*out)->b->ptr);
+    if (convert) zkg::msm_convert_bases(p, n_words / 8, dev->d->stream);
+    else zkg::msm_reduce_coords(p, n_words / 8, dev->d->stream);
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
@@ -461,6 +464,20 @@ int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words) {
     ZKG_GUARD_BEGIN
     zkg::device_bind(buf->b->dev);
     zkg::msm_convert_bases(static_cast<uint32_t*>(buf->b->ptr), n_words / 8, buf->b->dev->stream);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
+int zkg_msm_check_bases(zkg_device* dev, int group, const zkg_buffer* bases, size_t n, uint64_t* bad,
+                        uint64_t* first_bad) {
+    ZKG_REQUIRE(dev && bases && bad && first_bad, "null argument");
+    ZKG_REQUIRE(group == ZKG_G1 || group == ZKG_G2, "group must be G1 (1) or G2 (2)");
+    const uint64_t point_bytes = group == ZKG_G1 ? 64 : 128;
+    if (bases->b->bytes < n * point_bytes)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(bases->b->bytes) + " bytes; " +
+                                                std::to_string(n) + " points need " + std::to_string(n * point_bytes));
+    ZKG_GUARD_BEGIN
+    zkg::msm_check_bases(dev->d, group, static_cast<const uint32_t*>(bases->b->ptr), n, bad, first_bad);
     return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
@@ -612,6 +629,21 @@ int zkg_batch_captured(const zkg_batch* batch) { return batch && batch->b->captu
 #include "serve.hpp"
 #include "zkey.hpp"
 
+// A section's uploaded points must be infinity or on the curve (and, for B2, in the order-r
+// subgroup): a file whose coordinates are swapped, not in Montgomery form, or cut from another
+// curve fails here instead of giving wrong MSMs.
+static void zkey_check_section(zkg::Device* d, int section, const uint32_t* words, size_t points) {
+    const int group = (int)(zkg::Zkey::point_words(section) / 16);
+    uint64_t bad = 0, first = 0;
+    zkg::msm_check_bases(d, group, words, points, &bad, &first);
+    if (bad)
+        throw zkg::Error(zkg::ZKG_ERR_INVALID,
+                         "the .zkey section " + std::to_string(section) + " holds " + std::to_string(bad) + " point" +
+                             (bad == 1 ? "" : "s") + (group == 1 ? " off the curve" : " off the twist or outside the order-r subgroup") +
+                             " (first at index " + std::to_string(first) + ")");
+}
+
+
 struct zkg_server {
     zkg::Server* s;
     zkg::Device* d;
@@ -707,8 +739,11 @@ int zkg_server_add_zkey(zkg_server* server, const zkg_zkey* zkey, int* key_id) {
     for (int m = 0; m < 5; ++m) {
         const size_t nbytes = 4 * zkg::Zkey::point_words(sections[m]) * shape.points[m];
         owned.push_back(zkg::buffer_alloc(server->d, nbytes));
-        // the stored words are already the library's Montgomery form (R = 2^256): a plain copy
+        // the stored words are the library's Montgomery form (R = 2^256), reduced below p on the
+        // device as upload_bases_montgomery_256 does
         ZKG_HIP(hipMemcpy(owned.back()->ptr, z.words(sections[m]), nbytes, hipMemcpyHostToDevice));
+        zkg::msm_reduce_coords(static_cast<uint32_t*>(owned.back()->ptr), nbytes / 32, server->d->stream);
+        zkey_check_section(server->d, sections[m], static_cast<const uint32_t*>(owned.back()->ptr), shape.points[m]);
         bytes += nbytes;
     }
     const uint32_t* g1[4];
@@ -901,8 +936,18 @@ int zkg_zkey_points(const zkg_zkey* zkey, int section, size_t* count) {
 int zkg_zkey_upload(zkg_device* dev, const zkg_zkey* zkey, int section, zkg_buffer** out) {
     ZKG_REQUIRE(dev && zkey && out, "null argument");
     ZKG_GUARD_BEGIN
-    const size_t words = zkey->z->points(section) * zkg::Zkey::point_words(section);
-    return zkg_msm_upload_bases_montgomery_256(dev, zkey->z->words(section), words, out);
+    const size_t points = zkey->z->points(section);
+    const size_t words = points * zkg::Zkey::point_words(section);
+    int rc = zkg_msm_upload_bases_montgomery_256(dev, zkey->z->words(section), words, out);
+    if (rc != zkg::ZKG_OK) return rc;
+    try {
+        zkey_check_section(dev->d, section, static_cast<const uint32_t*>((*out)->b->ptr), points);
+    } catch (...) {
+        zkg_buffer_free(*out);
+        *out = nullptr;
+        throw;
+    }
+    return zkg::ZKG_OK;
     ZKG_GUARD_END
 }
 

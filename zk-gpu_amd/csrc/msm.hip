@@ -71,6 +71,11 @@ static void fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(ceil_div(count, 256)), dim3(256), 0, s, p, value, count);
 }
 
+void fill_words(uint32_t* p, uint32_t value, size_t count, hipStream_t s) {
+    fill_u32(p, value, count, s);
+    ZKG_HIP(hipGetLastError());
+}
+
 __global__ void to_mont_kernel(uint32_t* xy, size_t coords) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= coords) return;
@@ -80,6 +85,28 @@ __global__ void to_mont_kernel(uint32_t* xy, size_t coords) {
 void msm_convert_bases(uint32_t* xy, size_t coords, hipStream_t stream) {
     if (!coords) return;
     hipLaunchKernelGGL(to_mont_kernel, dim3(ceil_div(coords, 256)), dim3(256), 0, stream, xy, coords);
+    ZKG_HIP(hipGetLastError());
+}
+
+// Any 256-bit word -> the same residue below p, in place. The kernels assume canonical
+// coordinates; Montgomery-2^256 input (snarkjs / ffjavascript) may hold values up to 2^256 - 1,
+// which the reference's upload reduces by its scaling product (src/msm.rs:190-197). 2^256 < 6p,
+// so five conditional subtractions reach [0, p).
+__global__ void reduce_coords_kernel(uint32_t* xy, size_t coords) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= coords) return;
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = xy[8 * i + k];
+#pragma unroll
+    for (int rep = 0; rep < 5; ++rep) fp_reduce_once<FqParams>(w);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xy[8 * i + k] = w[k];
+}
+
+void msm_reduce_coords(uint32_t* xy, size_t coords, hipStream_t stream) {
+    if (!coords) return;
+    hipLaunchKernelGGL(reduce_coords_kernel, dim3(ceil_div(coords, 256)), dim3(256), 0, stream, xy, coords);
     ZKG_HIP(hipGetLastError());
 }
 
@@ -734,6 +761,31 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
     return msm_shape(group, n, c, item_points, sort_points, precomputed).workspace_bytes(n);
 }
 
+bool poison_enabled() {
+    const char* e = std::getenv("ZKGPU_POISON");
+    return e && e[0] == '1';
+}
+
+// Debug mode (ZKGPU_POISON=1): before a run, every workspace array of the plan and its result are
+// filled with 0xA5A5A5A5 by a kernel, so a stage that reads a word no earlier stage of the same
+// run wrote gives a wrong result instead of a stale right one.
+static void poison_workspace(const MsmPlan& p, hipStream_t s) {
+    const size_t S = (size_t)p.W * p.B, nb = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const struct {
+        void* ptr;
+        size_t bytes;
+    } arrays[] = {
+        {p.counts, 4 * S}, {p.cursors, 4 * S}, {p.starts, 4 * (S + 1)}, {p.istarts, 4 * (S + 1)},
+        {p.scan_tmp, 4 * (3 * nb + 1)}, {p.split, p.glv ? 32 * (size_t)p.n : 0},
+        {p.endo_x, p.glv ? 2 * p.base_points() * p.aw : 0}, {p.sorted, 4 * (size_t)p.W * p.nv},
+        {p.digits, p.lds_sort ? 4 * (size_t)p.W * p.nv : 0}, {p.table, p.lds_sort ? 4 * (size_t)p.W * p.nblk * p.B : 0},
+        {p.item_info, 8 * p.max_items}, {p.item_pts, 4 * p.max_items * p.xw}, {p.buckets, 4 * S * p.xw},
+        {p.partials, 4 * (size_t)p.W * p.T * p.xw}, {p.wsums, 4 * (size_t)p.W * p.xw}, {p.result, 4 * (size_t)p.xw}};
+    for (const auto& a : arrays)
+        if (a.ptr && a.bytes) fill_u32(static_cast<uint32_t*>(a.ptr), 0xA5A5A5A5u, a.bytes / 4, s);
+    ZKG_HIP(hipGetLastError());
+}
+
 template <class Fe>
 // prof: wait around every stage and record its time. yield_heavy: wait after the heavy stages
 // (accumulate, finalize, reduce, combine) so that other users of the GPU get it in between
@@ -761,6 +813,7 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         if (!found) prof->emplace_back(name, ms);
         t0 = t1;
     };
+    if (poison_enabled()) poison_workspace(p, s);
     if (prof) {
         ZKG_HIP(hipStreamSynchronize(s));
         t0 = std::chrono::steady_clock::now();

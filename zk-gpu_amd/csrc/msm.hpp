@@ -125,6 +125,12 @@ void msm_glv_split(const uint32_t* scalars, uint32_t n, uint32_t* split, hipStre
 
 // Plain affine coordinates -> Montgomery form in place (`coords` elements of 8 words).
 void msm_convert_bases(uint32_t* xy, size_t coords, hipStream_t stream);
+// Checks n bases (library form) of `group`: bad = how many are neither infinity nor on the curve
+// and (G2) in the order-r subgroup, first_bad = the smallest such index (0 when none). Waits for
+// the device.
+void msm_check_bases(Device* d, int group, const uint32_t* bases, size_t n, uint64_t* bad, uint64_t* first_bad);
+// Montgomery-2^256 coordinates of any 256-bit value -> canonical (below p) in place.
+void msm_reduce_coords(uint32_t* xy, size_t coords, hipStream_t stream);
 
 // The bases table of a precomputed plan: out[0..n) = bases[0..n), out[n..2n) = 2^64 bases[i]
 // (library form; `out` holds 2n points of `group`).

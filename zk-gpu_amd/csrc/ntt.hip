@@ -458,6 +458,7 @@ void NttPlan::run(uint32_t* data, bool inverse, hipStream_t stream, const uint32
 
 void NttPlan::run_with(uint32_t* data, bool inverse, hipStream_t stream, const uint32_t* scale_tab, uint32_t* scr) const {
     if (!scr && passes.size() > 1) throw Error(ZKG_ERR_INTERNAL, "NTT plan run without scratch");
+    if (scr && poison_enabled()) fill_words(scr, 0xA5A5A5A5u, (size_t)8 << log_n, stream);
     for (size_t i = 0; i < passes.size(); ++i)
         launch_pass(tile_log, pass_args(i, data, scr, inverse, scale_tab), passes[i].wgs, stream);
 }
@@ -588,6 +589,7 @@ void NttBatch::launch(hipStream_t stream) const {
 
 void NttBatch::run(hipStream_t stream) {
     prepare(stream);
+    if (poison_enabled() && scratch_.ptr) fill_words(static_cast<uint32_t*>(scratch_.ptr), 0xA5A5A5A5u, scratch_.bytes / 4, stream);
     launch(stream);
 }
 

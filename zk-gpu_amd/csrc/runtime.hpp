@@ -93,4 +93,10 @@ struct Scratch {
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// p[0 .. count) = value, queued on s as a kernel (not a memset, so it may be graph-captured).
+void fill_words(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
+// Debug mode ZKGPU_POISON=1: plans fill their workspace (MSM) or scratch (NTT) with 0xA5A5A5A5
+// before every run, so that reading a word the run did not write gives a wrong result.
+bool poison_enabled();
+
 }  // namespace zkg

@@ -62,6 +62,26 @@ extern template void launch_point_op<Fq2>(int, const uint32_t*, const uint32_t*,
 extern template void launch_generator_multiples<Fq2>(const uint32_t*, uint32_t*, size_t, hipStream_t);
 extern template void launch_shift64<Fq2>(const uint32_t*, uint32_t*, size_t, hipStream_t);
 
+extern template void launch_check_bases<Fq2>(const uint32_t*, size_t, unsigned long long*, hipStream_t);
+
+void msm_check_bases(Device* d, int group, const uint32_t* bases, size_t n, uint64_t* bad, uint64_t* first_bad) {
+    device_bind(d);
+    DeviceArray<unsigned long long> flags(2);
+    const unsigned long long init[2] = {0ull, ~0ull};
+    hipStream_t s = d->stream;
+    ZKG_HIP(hipMemcpyAsync(flags.ptr, init, sizeof init, hipMemcpyHostToDevice, s));
+    if (n) {
+        if (group == 1) launch_check_bases<Fq>(bases, n, flags.ptr, s);
+        else launch_check_bases<Fq2>(bases, n, flags.ptr, s);
+        ZKG_HIP(hipGetLastError());
+    }
+    unsigned long long out[2];
+    ZKG_HIP(hipMemcpyAsync(out, flags.ptr, sizeof out, hipMemcpyDeviceToHost, s));
+    ZKG_HIP(hipStreamSynchronize(s));
+    *bad = out[0];
+    *first_bad = out[0] ? out[1] : 0;
+}
+
 void msm_precompute_bases(int group, const uint32_t* bases, size_t n, uint32_t* out, hipStream_t stream) {
     if (!n) return;
     if (group == 1) launch_shift64<Fq>(bases, out, n, stream);

@@ -108,6 +108,16 @@ class Msm:
         """Convert plain coordinates already on the device to the library's form, in place."""
         check(_lib.load().zkg_msm_convert_bases(buf.handle, int(words)))
 
+    @staticmethod
+    def check_bases(gpu, group: Group, bases: Buffer, n: int):
+        """(count, first index) of the first n bases that are neither infinity nor on the curve (G2:
+        the twist and the order-r subgroup). No counterpart in the reference: plans assume G2 bases
+        in the subgroup (the GLV endomorphism acts as lambda only there)."""
+        bad, first = ctypes.c_uint64(), ctypes.c_uint64()
+        check(_lib.load().zkg_msm_check_bases(gpu.handle, group.value, bases.handle, int(n), ctypes.byref(bad),
+                                              ctypes.byref(first)))
+        return int(bad.value), int(first.value)
+
     def group(self) -> Group:
         return self._group
 
