@@ -174,6 +174,21 @@ int zkg_msm_check_bases(zkg_device* dev, int group, const zkg_buffer* bases, siz
  * ZKGPU_MSM_GLV=0 in the environment at plan creation selects plans without the GLV split, which
  * sum any points of the twist exactly as the reference does. */
 int zkg_msm_run(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the MSM that
+ * zkg_msm_create(dev, group, n, c, 0) + zkg_msm_run compute, with the same kernels and the same
+ * result words written to the caller's `result` (32 words G1, 64 words G2: one XYZZ point in the
+ * library's form, as zkg_msm_run's result buffer), but no plan and no allocation: the workspace
+ * is carved from a static device array of 64 MiB per GPU (held from zkg_device_open until the
+ * process ends), which covers e.g. G1 up to n = 2^16 at the default window, G1 2^17 at c = 12,
+ * G2 2^16 at c = 11. Larger workspaces are rejected ("an MSM of {n} points at c = {c} needs a
+ * workspace of {k} bytes; the static scratch holds 67108864"); plan and buffer errors as
+ * zkg_msm_create / zkg_msm_run. Queued on the device's stream (returns before the GPU finishes).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (from several host
+ * threads, or this call next to a batch or server using it) race. Every stage writes the scratch
+ * it reads within the same call, so nothing is carried over from the previous call. */
+int zkg_msm_run_noalloc(zkg_device* dev, int group, size_t n, uint32_t c, const zkg_buffer* bases,
+                        const zkg_buffer* scalars, zkg_buffer* result);
 /* Msm::run_yielding (src/msm.rs:300-318): as zkg_msm_run, but waits for the GPU after the
  * heavy stages (accumulate, finalize, reduce chunks, combine), so a display sharing the GPU gets it
  * in between. Same result. */

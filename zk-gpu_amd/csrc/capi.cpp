@@ -507,6 +507,36 @@ int zkg_msm_run(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, 
     ZKG_GUARD_END
 }
 
+int zkg_msm_run_noalloc(zkg_device* dev, int group, size_t n, uint32_t c, const zkg_buffer* bases,
+                        const zkg_buffer* scalars, zkg_buffer* result) {
+    ZKG_REQUIRE(dev && bases && scalars && result, "null argument");
+    ZKG_GUARD_BEGIN
+    const zkg::MsmShape sh = zkg::msm_shape(group, n, c, 0);
+    const size_t need = zkg::msm_carved_bytes(sh, n);
+    if (need > zkg::MSM_SCRATCH_BYTES)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "an MSM of " + std::to_string(n) + " points at c = " + std::to_string(c) +
+                                                " needs a workspace of " + std::to_string(need) +
+                                                " bytes; the static scratch holds " +
+                                                std::to_string(zkg::MSM_SCRATCH_BYTES));
+    const uint64_t point_bytes = 4ull * sh.aw;
+    if (scalars->b->bytes < n * 32)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "scalars buffer holds " + std::to_string(scalars->b->bytes) + " bytes; " +
+                                                std::to_string(n) + " scalars need " + std::to_string(n * 32));
+    if (bases->b->bytes < n * point_bytes)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "bases buffer holds " + std::to_string(bases->b->bytes) + " bytes; " +
+                                                std::to_string(n) + " points need " + std::to_string(n * point_bytes));
+    if (result->b->bytes < 4ull * sh.xw)
+        return zkg::fail(zkg::ZKG_ERR_SIZE, "result buffer holds " + std::to_string(result->b->bytes) +
+                                                " bytes; the point needs " + std::to_string(4ull * sh.xw));
+    if (!dev->d->msm_scratch) return zkg::fail(zkg::ZKG_ERR_DEVICE, "no static scratch on this device");
+    zkg::device_bind(dev->d);
+    const zkg::MsmPlan plan(dev->d, group, n, c, sh, dev->d->msm_scratch, static_cast<uint32_t*>(result->b->ptr));
+    plan.run(static_cast<const uint32_t*>(bases->b->ptr), static_cast<const uint32_t*>(scalars->b->ptr),
+             dev->d->stream);
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
 int zkg_msm_run_yielding(zkg_msm* m, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result) {
     int rc = msm_check(m, bases, scalars);
     if (rc != zkg::ZKG_OK) return rc;

@@ -712,7 +712,101 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     }
 }
 
+// The workspace arrays of a plan in one region: each array's byte size, in a fixed order, each
+// starting at a 256-byte boundary. `base` = nullptr only sizes the region.
+static size_t carve(const MsmShape& sh, size_t n, uint8_t* base, MsmPlan* p) {
+    const size_t S = (size_t)sh.W * sh.B;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> uint8_t* {
+        uint8_t* at = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    uint32_t* counts = (uint32_t*)take(4 * S);
+    uint32_t* cursors = (uint32_t*)take(4 * S);
+    uint32_t* starts = (uint32_t*)take(4 * (S + 1));
+    uint32_t* istarts = (uint32_t*)take(4 * (S + 1));
+    uint32_t* scan_tmp = (uint32_t*)take(4 * (3 * sh.scan_blocks + 1));
+    uint32_t* split = sh.glv ? (uint32_t*)take(32 * n) : nullptr;
+    uint32_t* endo_x = sh.glv ? (uint32_t*)take(2 * (sh.pre ? 2 * n : n) * sh.aw) : nullptr;
+    uint32_t* sorted = (uint32_t*)take(4 * (size_t)sh.W * sh.nv);
+    uint32_t* digits = sh.lds_sort ? (uint32_t*)take(4 * (size_t)sh.W * sh.nv) : nullptr;
+    uint32_t* table = sh.lds_sort ? (uint32_t*)take(4 * (size_t)sh.W * sh.nblk * sh.B) : nullptr;
+    uint2* item_info = (uint2*)take(8 * sh.max_items);
+    uint32_t* item_pts = (uint32_t*)take(4 * sh.max_items * sh.xw);
+    uint32_t* buckets = (uint32_t*)take(4 * S * sh.xw);
+    uint32_t* partials = (uint32_t*)take(4 * (size_t)sh.W * sh.T * sh.xw);
+    uint32_t* wsums = (uint32_t*)take(4 * (size_t)sh.W * sh.xw);
+    if (p) {
+        p->counts = counts;
+        p->cursors = cursors;
+        p->starts = starts;
+        p->istarts = istarts;
+        p->scan_tmp = scan_tmp;
+        p->max_items_dev = scan_tmp + 3 * sh.scan_blocks;
+        p->split = split;
+        p->endo_x = endo_x;
+        p->sorted = sorted;
+        p->digits = digits;
+        p->table = table;
+        p->item_info = item_info;
+        p->item_pts = item_pts;
+        p->buckets = buckets;
+        p->partials = partials;
+        p->wsums = wsums;
+    }
+    return off;
+}
+
+size_t msm_carved_bytes(const MsmShape& sh, size_t n) { return carve(sh, n, nullptr, nullptr); }
+
+// The static scratch of the no-allocation MSM (zkg_msm_run_noalloc): each GPU the library runs
+// on holds its own copy until the process ends. Host code never passes `scratch` itself to a
+// kernel or a copy (on the host the name is a stand-in, not a device address): it uses the address
+// msm_scratch_address returns for the current device, which device_open records per device.
+static __device__ __align__(256) uint8_t scratch[MSM_SCRATCH_BYTES];
+
+void* msm_scratch_address() {
+    void* ptr = nullptr;
+    ZKG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    return ptr;
+}
+
+MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, const MsmShape& sh, void* workspace,
+                 uint32_t* result_words)
+    : dev(d), group(grp), pre(sh.pre), owned(false) {
+    n = (uint32_t)npts;
+    c = win;
+    glv = sh.glv;
+    f29 = msm_f29_enabled();
+    nv = sh.nv;
+    W = sh.W;
+    wb = sh.wb;
+    nw = sh.nw;
+    B = sh.B;
+    CH = sh.CH;
+    L = sh.L;
+    T = sh.T;
+    aw = sh.aw;
+    xw = sh.xw;
+    max_items = sh.max_items;
+    merge_rounds = sh.merge_rounds;
+    lds_sort = sh.lds_sort;
+    sort_pts = sh.sort_pts;
+    nblk = sh.nblk;
+    if (lds_sort) {  // the kernels' LDS limit, as the allocating constructor raises it (no allocation)
+        const int lds_max = 4 << 15;
+        ZKG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(msm_hist_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+        ZKG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(msm_place_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    }
+    carve(sh, npts, static_cast<uint8_t*>(workspace), this);
+    result = result_words;
+}
+
 MsmPlan::~MsmPlan() {
+    if (!owned) return;  // carved from a caller's region: nothing to free
     for (uint32_t* p : {split, endo_x, counts, cursors, starts, istarts, scan_tmp, sorted, digits, table, item_pts,
                         buckets, partials, wsums, result})
         if (p) (void)hipFree(p);

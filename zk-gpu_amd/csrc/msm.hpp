@@ -31,6 +31,13 @@ struct MsmShape {
 };
 MsmShape msm_shape(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0,
                    bool precomputed = false);
+// Bytes of one region holding every workspace array of a plan of shape `sh` over n points, each
+// array starting at a 256-byte boundary (the no-allocation plans carve their arrays from it).
+size_t msm_carved_bytes(const MsmShape& sh, size_t n);
+// Size of the static device array that zkg_msm_run_noalloc carves its workspace from (one per GPU,
+// held until the process ends), and its device address on the calling thread's current device.
+static constexpr size_t MSM_SCRATCH_BYTES = (size_t)64 << 20;
+void* msm_scratch_address();
 // Device bytes a plan for (group, n, c, item_points, sort_points, precomputed) allocates
 // (0 = automatic); throws as msm_shape for arguments no plan accepts.
 size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0,
@@ -99,10 +106,16 @@ struct MsmPlan {
     uint32_t* result = nullptr;      // xw        final XYZZ point
     void* result_handle = nullptr;   // zkg_buffer* wrapping `result` (owned by the C ABI layer)
 
+    bool owned = true;       // workspace from hipMalloc (freed with the plan) or carved from a caller's region
+
     // item_points / sort_points 0 = automatic (a quarter of the mean bucket / max(2^(c-1), 8192));
     // precomputed: a plan over msm_precompute_bases tables (c <= 16, n < 2^29)
     MsmPlan(Device* d, int group, size_t n, uint32_t c, uint32_t item_points, uint32_t sort_points = 0,
             bool precomputed = false);
+    // A plan that allocates nothing: its workspace is carved from `workspace` (256-byte aligned,
+    // at least msm_carved_bytes(sh) bytes, where sh = msm_shape of the same arguments) and its
+    // result goes to `result_words` (xw words). Nothing is freed when it is destroyed.
+    MsmPlan(Device* d, int group, size_t n, uint32_t c, const MsmShape& sh, void* workspace, uint32_t* result_words);
     // Virtual points below this index are entries of the bases buffer, the others phi of one
     // (2^32 - 1 without GLV).
     uint32_t n_real() const { return pre ? 2 * n : glv ? n : 0xffffffffu; }

@@ -1,6 +1,8 @@
 // Device runtime implementation (see runtime.hpp).
 #include "runtime.hpp"
 
+#include "msm.hpp"
+
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -52,6 +54,13 @@ Device* device_open(int ordinal) {
     if (se != hipSuccess) {
         delete d;
         throw Error(ZKG_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(se));
+    }
+    try {
+        d->msm_scratch = msm_scratch_address();  // this device's copy of the static array
+    } catch (...) {
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        throw;
     }
     std::lock_guard<std::mutex> lk(g_devices_mu);
     g_devices.push_back(d);

@@ -51,6 +51,9 @@ struct Device {
     std::string name;
     ErrorFn on_error = nullptr;  // guarded by the device registry's lock
     void* on_error_user = nullptr;
+    // This device's copy of the no-allocation MSM's static scratch array (msm_scratch_address,
+    // taken at device_open with the device current).
+    void* msm_scratch = nullptr;
 };
 
 struct Buffer {

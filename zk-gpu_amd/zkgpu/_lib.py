@@ -59,6 +59,8 @@ SIGNATURES = {
     "zkg_msm_convert_bases": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
     "zkg_msm_check_bases": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "zkg_msm_run_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "zkg_msm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_void_pp]),
     "zkg_msm_run_yielding": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_void_pp]),
     "zkg_msm_result": (ctypes.c_int, [ctypes.c_void_p, c_void_pp]),
