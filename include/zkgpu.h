@@ -167,8 +167,8 @@ int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words);
  * key are subgroup points; zkg_zkey_upload and zkg_server_add_zkey run this check themselves. */
 int zkg_msm_check_bases(zkg_device* dev, int group, const zkg_buffer* bases, size_t n, uint64_t* bad,
                         uint64_t* first_bad);
-/* Msm::run (src/msm.rs:284-298): queue sum(scalars[i] * bases[i]). *result receives the
- * plan-owned buffer holding one XYZZ point (32 words G1, 64 words G2, Montgomery form), valid
+/* Msm::run (src/msm.rs:284-298): queue sum(scalars[i] * bases[i]). *result (unless `result` is
+ * NULL; zkg_msm_result gives the same handle at any time) receives the plan-owned buffer holding one XYZZ point (32 words G1, 64 words G2, Montgomery form), valid
  * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...".
  * Precondition (not checked here, see zkg_msm_check_bases): G2 bases lie in the order-r subgroup.
  * ZKGPU_MSM_GLV=0 in the environment at plan creation selects plans without the GLV split, which
