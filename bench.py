@@ -127,7 +127,52 @@ def affine_words(words, group):
     return np.array(sum((host.words32(v) for v in (xa, xb, ya, yb)), []), np.uint32)
 
 
-def cpu_baseline(plain_bases_g1, ks, scalars, gpu_affine, g2, ntt_input, ntt_output, log_n_ntt):
+def glv_digit_counts(gpu, scalars, c):
+    """Nonzero signed digits of the actual scalars as the plan splits them: each scalar becomes two
+    GLV half-scalars (|k| < 2^127, sign in bit 127; split on the device by the library's own
+    kernel, which tests/test_gpu_glv.py checks against a lattice decomposition), and each half
+    takes W = ceil(128 / c) windows of balanced width with the plan's digit rule (v = bits + carry;
+    v > 2^(w-1) wraps to v - 2^w with carry 1). Returns (nonzero digits, W, widths)."""
+    n = scalars.size // 8
+    halves = np.zeros(8 * n, np.uint32)
+    _lib.check(_lib.load().zkg_test_glv_split(gpu.handle, scalars.ctypes.data_as(_lib.c_u32p), n,
+                                              halves.ctypes.data_as(_lib.c_u32p)))
+    h = halves.reshape(2 * n, 4).astype(np.uint64)
+    lo = h[:, 0] | (h[:, 1] << np.uint64(32))
+    hi = h[:, 2] | ((h[:, 3] & np.uint64(0x7FFFFFFF)) << np.uint64(32))
+    W = -(-128 // c)
+    wb, nw = 128 // W, 128 - (128 // W) * W
+    widths = [wb + (1 if w < nw else 0) for w in range(W)]
+    carry = np.zeros(2 * n, np.uint64)
+    nnz, pos = 0, 0
+    for width in widths:
+        mask = np.uint64((1 << width) - 1)
+        if pos + width <= 64:
+            bits = (lo >> np.uint64(pos)) & mask
+        elif pos >= 64:
+            bits = (hi >> np.uint64(pos - 64)) & mask
+        else:
+            bits = ((lo >> np.uint64(pos)) | (hi << np.uint64(64 - pos))) & mask
+        v = bits + carry
+        wrap = v > np.uint64(1 << (width - 1))
+        mag = np.where(wrap, np.uint64(1 << width) - v, v)
+        carry = wrap.astype(np.uint64)
+        nnz += int(np.count_nonzero(mag))
+        pos += width
+    return nnz, W, widths
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(plain_bases_g1, ks, scalars, gpu_affine, g2, ntt_input, ntt_output, log_n_ntt, groth=None):
     """CPU leg: the oracle (a multi-threaded C++ restatement: Jacobian Pippenger, radix-2 NTT) is
     the checker of the GPU results and is timed on this host's cores on the same inputs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -145,6 +190,11 @@ def cpu_baseline(plain_bases_g1, ks, scalars, gpu_affine, g2, ntt_input, ntt_out
     for j in (1, 12345):
         if O.poly_eval(ntt_input, pow(w, j, O.R)) != O.words_to_int(ntt_output[8 * j: 8 * j + 8]):
             raise SystemExit(f"NTT 2^{log_n_ntt} output differs at j={j}: refusing to report a time")
+    if groth is not None:  # the Groth16 set: every MSM (plain, batched, tuned) against its closed form
+        for label, (group, gks, gsc, got) in groth.items():
+            want = O.point_mul(group, O.generator_words(group), O.closed_form_scalar(gks, gsc))
+            if not np.array_equal(want, got):
+                raise SystemExit(f"Groth16 set MSM {label} differs from the closed form: refusing to report a time")
     t0 = time.perf_counter()
     cpu_res = O.msm(1, plain_bases_g1, scalars, threads=threads)
     cpu_ms = (time.perf_counter() - t0) * 1e3
@@ -157,6 +207,8 @@ def cpu_baseline(plain_bases_g1, ks, scalars, gpu_affine, g2, ntt_input, ntt_out
         "value": round(cpu_ms, 3),
         "unit": "ms",
         "cores": threads,
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
         "kind": "port",
         "sample": f"the full G1 MSM 2^20 on the same inputs (oracle: C++ Jacobian Pippenger, {threads} threads); "
                   f"NTT 2^{log_n_ntt} forward took {cpu_ntt_ms:.1f} ms on the same cores",
@@ -252,11 +304,15 @@ def groth16_msm_leg(gpu, ev, rng, args, log_n=22):
     n = 1 << log_n
     c = Msm.default_window(n)
     sets = [(1, n), (1, n), (1, n), (1, n - 1), (2, n)]
-    bases = [library_bases(gpu, g, fast_ks(rng, m)) for g, m in sets]
-    wit = gpu.buffer(witness_like(rng, n))
-    hsc = gpu.buffer(uniform_fr(rng, n - 1))
+    names = ["A", "B1", "C", "H", "B2"]
+    host_ks = [fast_ks(rng, m) for _, m in sets]
+    bases = [library_bases(gpu, g, k) for (g, _), k in zip(sets, host_ks)]
+    wit_h, hsc_h = witness_like(rng, n), uniform_fr(rng, n - 1)
+    wit, hsc = gpu.buffer(wit_h), gpu.buffer(hsc_h)
     scal = [wit, wit, wit, hsc, wit]
+    scal_h = [wit_h, wit_h, wit_h, hsc_h, wit_h]
     plain_plans = [Msm.for_group(gpu, Group(g), m, c) for g, m in sets]
+    default_plans = [Msm.for_group(gpu, Group(g), m, c) for g, m in sets]
     t0 = time.perf_counter()
     tuned = {g: tune.tune_msm(gpu, g, n) for g in (1, 2)}
     tune_s = time.perf_counter() - t0
@@ -264,21 +320,31 @@ def groth16_msm_leg(gpu, ev, rng, args, log_n=22):
     batch = Batch(gpu)
     for p, b, sc in zip(plans, bases, scal):
         batch.add_msm(p, b, sc)
+    batch_default = Batch(gpu)  # the same batch with default-window plans: batching alone, no tuning
+    for p, b, sc in zip(default_plans, bases, scal):
+        batch_default.add_msm(p, b, sc)
 
     def plain():
         for p, b, sc in zip(plain_plans, bases, scal):
             p.run(gpu, b, sc)
 
+    def results(ps):
+        gpu.wait()
+        return [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(ps, sets)]
+
     plain()
-    gpu.wait()
-    want = [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(plain_plans, sets)]
+    want = results(plain_plans)
     batch.run()
-    gpu.wait()
-    got = [affine_words(gpu.read(p.result_buffer(gpu), 16 * g * 2), g) for p, (g, _) in zip(plans, sets)]
-    if not all(np.array_equal(a, b) for a, b in zip(want, got)):
+    got = results(plans)
+    batch_default.run()
+    got_default = results(default_plans)
+    if not all(np.array_equal(a, b) and np.array_equal(a, d) for a, b, d in zip(want, got, got_default)):
         raise SystemExit("batched Groth16 MSM set differs from the plain calls: refusing to report a time")
+    # kept for the exactness gate of the CPU leg: each MSM against (sum k_i s_i) G
+    gate = {names[j]: (g, host_ks[j], scal_h[j][: 8 * m], want[j]) for j, (g, m) in enumerate(sets)}
     steps = max(3, args.steps // 5)
     plain_ms, _ = time_calls(gpu, ev, plain, steps, 1)
+    bat_default_ms, _ = time_calls(gpu, ev, batch_default.run, steps, 1)
     bat_ms, _ = time_calls(gpu, ev, batch.run, steps, 1)
     batch.capture()
     batch.run()
@@ -289,13 +355,16 @@ def groth16_msm_leg(gpu, ev, rng, args, log_n=22):
     graph_ms, _ = time_calls(gpu, ev, batch.run, steps, 1)
     res = {"n": n, "msms": "A,B1,C,H(n-1) G1 + B2 G2, witness-like scalars (H uniform)",
            "plain": {"c": c, "ms": round(plain_ms, 3)},
+           "batched_default_plans": {"c": c, "ms": round(bat_default_ms, 3),
+                                     "speedup": round(plain_ms / bat_default_ms, 3)},
            "batched": {"ms": round(bat_ms, 3), "graph_ms": round(graph_ms, 3), "tuned_g1": tuned[1],
                        "tuned_g2": tuned[2], "tuning_seconds": round(tune_s, 2)},
-           "speedup": round(plain_ms / min(bat_ms, graph_ms), 3)}
-    log(f"Groth16 MSM set 2^{log_n}: plain {plain_ms:.2f} ms, batched+tuned {bat_ms:.2f} ms "
-        f"({plain_ms / bat_ms:.2f}x); tuned {tuned}")
-    del batch, plans, plain_plans, bases
-    return res
+           "speedup": round(plain_ms / min(bat_ms, graph_ms), 3),
+           "exactness": "each MSM equal to the closed form (sum k_i s_i) G (CPU leg); batched = plain"}
+    log(f"Groth16 MSM set 2^{log_n}: plain {plain_ms:.2f} ms, batched (default plans) {bat_default_ms:.2f} ms, "
+        f"batched+tuned {bat_ms:.2f} ms ({plain_ms / bat_ms:.2f}x); tuned {tuned}")
+    del batch, batch_default, plans, plain_plans, default_plans, bases
+    return res, gate
 
 
 def serving_leg(gpu, rng, args):
@@ -527,7 +596,7 @@ def main():
         res = {}
         if args.workload == "batch":
             res["batched_ntt_64"] = batched_ntt_leg(gpu, ev, rng, args)
-            res["groth16_msm_set_2^22"] = groth16_msm_leg(gpu, ev, rng, args)
+            res["groth16_msm_set_2^22"] = groth16_msm_leg(gpu, ev, rng, args)[0]
         else:
             res["serving"] = serving_leg(gpu, rng, args)
         print(json.dumps(res), flush=True)
@@ -642,15 +711,39 @@ def main():
     _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 0, threads, chain, 2, ctypes.byref(ms)))
     mulmod_gps = threads * chain * 2 / (ms.value * 1e-3) / 1e9
 
-    # Fq multiplications per G1 MSM (madd 10, add 14, dbl 9): accumulation of every nonzero digit
-    W1 = (255 + c1 - 1) // c1
-    fq_mults = n1 * W1 * 10  # upper bound: every digit nonzero
-    msm_gmulmod = fq_mults / (g1_ms * 1e-3) / 1e9
+    _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 2, threads, chain, 2, ctypes.byref(ms)))
+    _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 2, threads, chain, 2, ctypes.byref(ms)))
+    mul29_gps = threads * chain * 2 / (ms.value * 1e-3) / 1e9
 
-    batch_res = groth = serve = None
+    # Algorithmic Fq products of the G1 MSM 2^20 (madd 10, add 14, dbl 9 products per operation):
+    # one mixed addition per nonzero signed digit of the actual GLV half-scalars, the bucket
+    # reduction's 2 additions per bucket, and the window combine's doublings and additions.
+    nnz, W1, widths = glv_digit_counts(gpu, sc1, c1)
+    B1 = 1 << (c1 - 1)
+    acc_products = 10 * nnz
+    call_products = acc_products + 14 * 2 * W1 * B1 + 9 * sum(widths[:-1]) + 14 * (W1 - 1)
+    acc_ms = dict(stages)["accumulate"]
+    msm_roof = {
+        "kernel": "msm_accumulate29_kernel (G1 2^20, c = %d, GLV: %d windows over 2^21 half-scalars)" % (c1, W1),
+        "bound": "valu",
+        "nonzero_digits": nnz,
+        "fq_products_accumulate": acc_products,
+        "fq_products_call": call_products,
+        "accumulate_ms": round(acc_ms, 4),
+        "achieved_G_products_per_s": round(acc_products / (acc_ms * 1e-3) / 1e9, 1),
+        "peak_G_products_per_s": round(mul29_gps, 1),
+        "frac": round(acc_products / (acc_ms * 1e-3) / 1e9 / mul29_gps, 4),
+        "call_G_products_per_s": round(call_products / (g1_ms * 1e-3) / 1e9, 1),
+        "call_frac": round(call_products / (g1_ms * 1e-3) / 1e9 / mul29_gps, 4),
+        "note": "peak = the measured rate of independent 29-bit Montgomery products (the accumulation's own "
+                "arithmetic) over the whole GPU; accumulate_ms from Msm::profile",
+    }
+    log(f"MSM roofline: {msm_roof}")
+
+    batch_res = groth = serve = groth_gate = None
     if args.workload == "all":
         batch_res = batched_ntt_leg(gpu, ev, rng, args)
-        groth = groth16_msm_leg(gpu, ev, rng, args)
+        groth, groth_gate = groth16_msm_leg(gpu, ev, rng, args)
         serve = serving_leg(gpu, rng, args)
 
     cpu = None
@@ -661,7 +754,8 @@ def main():
         return
     if not args.no_cpu:
         log("CPU baseline (oracle) on the same inputs")
-        cpu = cpu_baseline(plain_g1, ks1, sc1, g1_affine, (ks2, sc2, g2_affine), ntt24_in, ntt24_out, 24)
+        cpu = cpu_baseline(plain_g1, ks1, sc1, g1_affine, (ks2, sc2, g2_affine), ntt24_in, ntt24_out, 24,
+                           groth_gate)
 
     n24 = ntt[24]
     per_pass_ms = n24["forward_ms"] / n24["passes"]
@@ -695,7 +789,8 @@ def main():
             "groth16_msm_set_2^22": groth,
             "serving": serve,
             "field_mul_Gmulmod_per_s": round(mulmod_gps, 1),
-            "g1_msm_fq_mults_per_s_G": round(msm_gmulmod, 1),
+            "field_mul29_Gmulmod_per_s": round(mul29_gps, 1),
+            "g1_msm_2^20_roofline": msm_roof,
             "device": info,
         },
         "roofline": {

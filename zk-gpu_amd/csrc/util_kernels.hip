@@ -12,6 +12,7 @@
 #include "api_types.hpp"
 #include "curve.hpp"
 #include "device_util.hpp"
+#include "field29.hpp"
 #include "msm.hpp"
 #include "util_point.hpp"
 #include "runtime.hpp"
@@ -55,6 +56,29 @@ __global__ __launch_bounds__(256) void field_bench_kernel(uint32_t chain, uint32
     for (int w = 0; w < WAYS; ++w)
 #pragma unroll
         for (int k = 0; k < 8; ++k) acc ^= x[w].w[k];
+    if (acc == 0x9e3779b9u) sink[0] = acc;  // practically never true; keeps the chains live
+}
+
+// The same chains in the MSM's nine 29-bit limbs (f29_mul modulo p): operands normalised below
+// 2p, so each product (bound 4p^2, far inside the 169p^2 the product allows) is again below 2p.
+template <int WAYS>
+__global__ __launch_bounds__(256) void field29_bench_kernel(uint32_t chain, uint32_t* sink) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    F29 x[WAYS], y;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) y.l[k] = (t * 2654435761u + k * 40503u) & (k == 8 ? 0xfffffu : f29::MASK);
+#pragma unroll
+    for (int w = 0; w < WAYS; ++w)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) x[w].l[k] = (t + 97u * w + k) & (k == 8 ? 0xfffffu : f29::MASK);
+    for (uint32_t i = 0; i < chain; ++i)
+#pragma unroll
+        for (int w = 0; w < WAYS; ++w) x[w] = f29_mul(x[w], y);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int w = 0; w < WAYS; ++w)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc ^= x[w].l[k];
     if (acc == 0x9e3779b9u) sink[0] = acc;  // practically never true; keeps the chains live
 }
 
@@ -189,8 +213,8 @@ int zkg_test_point_op(zkg_device* dev, int group, int op, const uint32_t* p, con
 
 int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t chain, uint32_t ways, double* ms) {
     if (!dev || !ms) return zkg::fail(zkg::ZKG_ERR_INVALID, "null argument");
-    if (field < 0 || field > 1 || threads == 0 || threads % 256 || !(ways == 1 || ways == 2 || ways == 4))
-        return zkg::fail(zkg::ZKG_ERR_INVALID, "field 0/1, threads a multiple of 256, ways 1/2/4");
+    if (field < 0 || field > 2 || threads == 0 || threads % 256 || !(ways == 1 || ways == 2 || ways == 4))
+        return zkg::fail(zkg::ZKG_ERR_INVALID, "field 0/1/2, threads a multiple of 256, ways 1/2/4");
     ZKG_GUARD_BEGIN
     zkg::device_bind(dev->d);
     hipStream_t s = dev->d->stream;
@@ -206,8 +230,14 @@ int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t c
     else hipLaunchKernelGGL((zkg::field_bench_kernel<P, 4>), grid, block, 0, s, chain, sink.ptr);
     if (field == 0) {
         ZKG_BENCH_LAUNCH(zkg::FqParams)
-    } else {
+    } else if (field == 1) {
         ZKG_BENCH_LAUNCH(zkg::FrParams)
+    } else if (ways == 1) {
+        hipLaunchKernelGGL(zkg::field29_bench_kernel<1>, grid, block, 0, s, chain, sink.ptr);
+    } else if (ways == 2) {
+        hipLaunchKernelGGL(zkg::field29_bench_kernel<2>, grid, block, 0, s, chain, sink.ptr);
+    } else {
+        hipLaunchKernelGGL(zkg::field29_bench_kernel<4>, grid, block, 0, s, chain, sink.ptr);
     }
 #undef ZKG_BENCH_LAUNCH
     ZKG_HIP(hipGetLastError());
