@@ -422,6 +422,74 @@ ZKG_DEV Fq2_29 fe_select(bool c, const Fq2_29& x, const Fq2_29& y) {
     return Fq2_29{fe_select(c, x.a, y.a), fe_select(c, x.b, y.b)};
 }
 
+// ---------------------------------------------------------------- Fq2 over a pair of lanes
+// The same Fq2 elements, in the same memory form as Fq2_29, held by two adjacent lanes: the even
+// lane holds component a, the odd lane component b. A product costs each lane two Fq29 products
+// (after exchanging components with its partner: even lane ac - bd, odd lane ad + bc) where
+// Karatsuba's formula on one lane costs three in sequence, a square costs one ((a + b)(a - b) and
+// 2ab), and a point takes half the registers. The G2 stages after the bucket accumulation are
+// chains of point additions on few lanes, so they run with shorter chains and more waves this
+// way. Both lanes of a pair must execute every operation together (branches on pair-uniform
+// conditions only: is_zero combines both lanes' answers).
+struct Fq2P {
+    Fq29 v;
+};
+
+ZKG_DEV bool pair_odd() { return (__lane_id() & 1u) != 0; }
+
+ZKG_DEV Fq29 pair_other(const Fq29& x) {
+    Fq29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = __shfl_xor(x.l[i], 1);
+    return r;
+}
+
+ZKG_DEV Fq2P operator+(const Fq2P& x, const Fq2P& y) { return Fq2P{x.v + y.v}; }
+ZKG_DEV Fq2P operator-(const Fq2P& x, const Fq2P& y) { return Fq2P{x.v - y.v}; }
+ZKG_DEV Fq2P operator*(const Fq2P& x, const Fq2P& y) {
+    const bool odd = pair_odd();
+    const Fq29 px = pair_other(x.v), py = pair_other(y.v);
+    // even (a): x.a y.a - x.b y.b = x y - px py; odd (b): x.a y.b + x.b y.a = px y + x py
+    const Fq29 u = fe_select(odd, px, x.v) * y.v;
+    const Fq29 w = fe_select(odd, x.v, px) * py;
+    return Fq2P{fe_select(odd, u + w, u - w)};
+}
+
+template <>
+struct FeTraits<Fq2P> {
+    static constexpr int WORDS = 16;  // memory: the whole element, a then b (each lane reads its half)
+    static ZKG_DEV Fq2P zero() { return Fq2P{FeTraits<Fq29>::zero()}; }
+    static ZKG_DEV Fq2P one() { return Fq2P{fe_select(pair_odd(), FeTraits<Fq29>::zero(), FeTraits<Fq29>::one())}; }
+    static ZKG_DEV bool is_zero(const Fq2P& x) {
+        const int z = fq29_is_zero(x.v) ? 1 : 0;
+        return (z & __shfl_xor(z, 1)) != 0;
+    }
+    static ZKG_DEV Fq2P sqr(const Fq2P& x) {  // even: (a + b)(a - b); odd: (b + b) a
+        const bool odd = pair_odd();
+        const Fq29 p = pair_other(x.v);
+        const Fq29 l = fe_select(odd, x.v + x.v, x.v + p);
+        const Fq29 r = fe_select(odd, p, x.v - p);
+        return Fq2P{l * r};
+    }
+    static ZKG_DEV Fq2P load(const uint32_t* p) { return Fq2P{FeTraits<Fq29>::load(p + (pair_odd() ? 8 : 0))}; }
+    static ZKG_DEV void store(uint32_t* p, const Fq2P& x) { FeTraits<Fq29>::store(p + (pair_odd() ? 8 : 0), x.v); }
+    static ZKG_DEV void store_result(uint32_t* p, const Fq2P& x) {
+        FeTraits<Fq29>::store_result(p + (pair_odd() ? 8 : 0), x.v);
+    }
+};
+
+ZKG_DEV Fq2P fe_select(bool c, const Fq2P& x, const Fq2P& y) { return Fq2P{fe_select(c, x.v, y.v)}; }
+
+// Lanes that hold one field element: 2 for Fq2P, else 1.
+template <class Fe>
+struct FeLanes {
+    static constexpr uint32_t value = 1;
+};
+template <>
+struct FeLanes<Fq2P> {
+    static constexpr uint32_t value = 2;
+};
+
 // The storage form of the Fq29 / Fq2_29 stages, from canonical 32-bit coordinates.
 ZKG_DEV Fq29 to_storage_field(const Fq& x, const Fq29*) { return fq29_from_fq(x); }
 ZKG_DEV Fq2_29 to_storage_field(const Fq2& x, const Fq2_29*) { return Fq2_29{fq29_from_fq(x.a), fq29_from_fq(x.b)}; }

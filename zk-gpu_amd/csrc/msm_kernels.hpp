@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void msm_merge_kernel(uint32_t* __restrict__ i
     // no bucket needs this round (the scan recorded the most items of any bucket): exit before
     // reading any item
     if ((max_items[0] + stride - 1) / stride <= FINAL_GROUP) return;
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / FeLanes<Fe>::value;  // pair-uniform
     if (item >= istarts[S]) return;
     const uint2 info = item_info[item];
     const uint32_t k = info.x, total = info.y;
@@ -262,7 +262,8 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
                                                           const uint32_t* __restrict__ istarts, uint32_t S,
                                                           uint32_t split, uint32_t* __restrict__ buckets) {
     typedef FeTraits<Fe> Tr;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr uint32_t LN = FeLanes<Fe>::value;  // lanes per element (the lanes of a pair stay together)
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / LN;
     const uint32_t slot = t >> split, part = t & ((1u << split) - 1);
     const bool valid = slot < S;  // every lane reaches the shuffles
     const size_t xw = 4 * Tr::WORDS;
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
     }
     for (uint32_t r = 0; r < split; ++r) {  // uniform over the grid
         typedef LaneOps<Fe> Lx;
-        const int src = (int)(__lane_id() ^ (1u << r));
+        const int src = (int)(__lane_id() ^ (LN << r));
         const Xyzz<Fe> other{Lx::shfl(acc.x, src), Lx::shfl(acc.y, src), Lx::shfl(acc.zz, src),
                              Lx::shfl(acc.zzz, src)};
         acc = xyzz_add(acc, other);
@@ -292,13 +293,23 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
 
 // ---------------------------------------------------------------- bucket reduction
 
+// Waves per SIMD the reduction kernel is compiled for (1: no constraint).
+template <class Fe>
+struct ReduceWaves {
+    static constexpr int value = 1;
+};
+template <>
+struct ReduceWaves<Fq2P> {
+    static constexpr int value = 2;
+};
+
 // Lane t of window w takes buckets b0 .. b0+L-1 (b0 = j*L) and computes
 // sum_k (k+1) bucket[b0+k] + b0 * sum_k bucket[b0+k]  =  sum_k (b0+k+1) bucket[b0+k];
 // the lanes of a workgroup (all in one window) then add their results in an LDS tree, leaving
 // one partial sum per workgroup. Each loop has a single point-addition site (operands chosen by
 // selects), which keeps the fully inlined kernel small.
 template <class Fe>
-__global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restrict__ buckets, uint32_t W,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ReduceWaves<Fe>::value))) void msm_reduce_kernel(const uint32_t* __restrict__ buckets, uint32_t W,
                                                         uint32_t B, uint32_t L, uint32_t c, uint32_t wb, uint32_t nw,
                                                         uint32_t* __restrict__ partials) {
     typedef FeTraits<Fe> Tr;
@@ -307,8 +318,10 @@ __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restr
     // a wave storing or loading their points would share banks in pairs of lanes two apart
     constexpr int XP = XW + 4;
     __shared__ uint32_t lds[128 * XP];
-    const uint32_t T = B / L;  // lanes per window, a multiple of blockDim.x
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr uint32_t LN = FeLanes<Fe>::value;      // lanes per element
+    const uint32_t tid = threadIdx.x / LN, nthr = blockDim.x / LN;  // logical threads
+    const uint32_t T = B / L;  // logical threads per window, a multiple of nthr
+    const uint32_t t = blockIdx.x * nthr + tid;
     const uint32_t w = t / T, j = t % T, b0 = j * L;
     Xyzz<Fe> r = xyzz_inf<Fe>();
     if (b0 < (1u << (wb + (w < nw ? 1u : 0u) - 1))) {  // else beyond this window's digit range: empty
@@ -329,13 +342,13 @@ __global__ __launch_bounds__(256) void msm_reduce_kernel(const uint32_t* __restr
             if (k < 0 || ((b0 >> k) & 1u)) r = xyzz_add(r, select_xyzz(k < 0, tot, run));
         }
     }
-    for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XP * (threadIdx.x - off), r);
+    for (uint32_t off = nthr / 2; off > 0; off >>= 1) {
+        if (tid >= off && tid < 2 * off) store_xyzz(lds + XP * (tid - off), r);
         __syncthreads();
-        if (threadIdx.x < off) r = xyzz_add(r, load_xyzz<Fe>(lds + XP * threadIdx.x));
+        if (tid < off) r = xyzz_add(r, load_xyzz<Fe>(lds + XP * tid));
         __syncthreads();
     }
-    if (threadIdx.x == 0) store_xyzz(partials + (size_t)XW * blockIdx.x, r);
+    if (tid == 0) store_xyzz(partials + (size_t)XW * blockIdx.x, r);
 }
 
 // One wave per window adds the window's P workgroup partials (P <= 64 at c <= 16 needs no
@@ -347,17 +360,19 @@ __global__ __launch_bounds__(64) void msm_window_sum_kernel(const uint32_t* __re
     constexpr int XW = 4 * Tr::WORDS;
     constexpr int XP = XW + 4;  // padded LDS slots, as in msm_reduce_kernel
     __shared__ uint32_t lds[32 * XP];
+    constexpr uint32_t LN = FeLanes<Fe>::value;                       // lanes per element
+    const uint32_t tid = threadIdx.x / LN, nthr = blockDim.x / LN;  // logical threads
     const uint32_t w = blockIdx.x;
     const uint32_t* src = partials + (size_t)XW * w * P;
-    Xyzz<Fe> acc = threadIdx.x < P ? load_xyzz<Fe>(src + XW * threadIdx.x) : xyzz_inf<Fe>();
-    for (uint32_t k = threadIdx.x + 64; k < P; k += 64) acc = xyzz_add(acc, load_xyzz<Fe>(src + XW * k));
+    Xyzz<Fe> acc = tid < P ? load_xyzz<Fe>(src + XW * tid) : xyzz_inf<Fe>();
+    for (uint32_t k = tid + nthr; k < P; k += nthr) acc = xyzz_add(acc, load_xyzz<Fe>(src + XW * k));
     for (uint32_t off = top; off > 0; off >>= 1) {
-        if (threadIdx.x >= off && threadIdx.x < 2 * off) store_xyzz(lds + XP * (threadIdx.x - off), acc);
+        if (tid >= off && tid < 2 * off) store_xyzz(lds + XP * (tid - off), acc);
         __syncthreads();
-        if (threadIdx.x < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XP * threadIdx.x));
+        if (tid < off) acc = xyzz_add(acc, load_xyzz<Fe>(lds + XP * tid));
         __syncthreads();
     }
-    if (threadIdx.x == 0) store_xyzz(wsums + (size_t)XW * w, acc);
+    if (tid == 0) store_xyzz(wsums + (size_t)XW * w, acc);
 }
 
 // ---------------------------------------------------------------- window combine
@@ -425,6 +440,11 @@ struct LaneOps<Fq2_29> : LaneOps2<Fq29, Fq2_29> {
     static ZKG_DEV Fq2_29 shfl(const Fq2_29& x, int src) {
         return Fq2_29{LaneOps<Fq29>::shfl(x.a, src), LaneOps<Fq29>::shfl(x.b, src)};
     }
+};
+
+template <>
+struct LaneOps<Fq2P> {  // lane shuffles of the G2 tail stages: each lane moves its own component
+    static ZKG_DEV Fq2P shfl(const Fq2P& x, int src) { return Fq2P{LaneOps<Fq29>::shfl(x.v, src)}; }
 };
 
 template <>
@@ -718,9 +738,23 @@ void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
     }
 }
 
+// The field of the merge, finalize, reduce and tree-sum kernels: G2's Fq2_29 stages run on lane
+// pairs (Fq2P, the same memory form), which halves their chains' latency per product and their
+// registers per lane; the accumulation and the combine keep their own fields.
+template <class Fe>
+struct TailField {
+    using type = Fe;
+};
+template <>
+struct TailField<Fq2_29> {
+    using type = Fq2P;
+};
+
 template <class Fe>
 void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
-    hipLaunchKernelGGL(msm_merge_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, p.item_pts,
+    using Ft = typename TailField<Fe>::type;
+    constexpr uint32_t LN = FeLanes<Ft>::value;
+    hipLaunchKernelGGL(msm_merge_kernel<Ft>, dim3(ceil_div(p.max_items * LN, 256)), dim3(256), 0, s, p.item_pts,
                        p.item_info, p.istarts, p.W * p.B, stride, (const uint32_t*)p.max_items_dev);
 }
 
@@ -732,23 +766,35 @@ void launch_finalize(const MsmPlan& p, hipStream_t s) {
     // (12288 buckets) 0.39 -> 0.27 ms with two lanes, 0.14 ms with four; at 2^18 buckets and more
     // one lane each is faster (G1 2^22 0.18 against 0.20, G2 0.62 against 0.80 with two).
     const uint32_t split = S < (1u << 15) ? 2u : S < (1u << 18) ? 1u : 0u;
-    hipLaunchKernelGGL(msm_finalize_kernel<Fe>, dim3(ceil_div((size_t)S << split, 256)), dim3(256), 0, s, p.item_pts,
-                       p.istarts, S, split, p.buckets);
+    using Ft = typename TailField<Fe>::type;
+    constexpr uint32_t LN = FeLanes<Ft>::value;
+    hipLaunchKernelGGL(msm_finalize_kernel<Ft>, dim3(ceil_div(((size_t)S << split) * LN, 256)), dim3(256), 0, s,
+                       p.item_pts, p.istarts, S, split, p.buckets);
+}
+
+// Logical threads (one per element) per workgroup of the bucket reduction: 256 lanes.
+template <class Ft>
+constexpr uint32_t reduce_block() {
+    return 256 / FeLanes<Ft>::value;
 }
 
 template <class Fe>
 void launch_reduce(const MsmPlan& p, hipStream_t s) {
-    const uint32_t rb = std::min<uint32_t>(256, p.T);  // a workgroup never spans two windows
-    hipLaunchKernelGGL(msm_reduce_kernel<Fe>, dim3((size_t)p.W * p.T / rb), dim3(rb), 0, s, p.buckets, p.W, p.B, p.L,
-                       p.c, p.wb, p.nw, p.partials);
+    using Ft = typename TailField<Fe>::type;
+    constexpr uint32_t LN = FeLanes<Ft>::value;
+    const uint32_t rb = std::min<uint32_t>(reduce_block<Ft>(), p.T);  // a workgroup never spans two windows
+    hipLaunchKernelGGL(msm_reduce_kernel<Ft>, dim3((size_t)p.W * p.T / rb), dim3(rb * LN), 0, s, p.buckets, p.W, p.B,
+                       p.L, p.c, p.wb, p.nw, p.partials);
 }
 
 template <class Fe>
 void launch_tree(const MsmPlan& p, hipStream_t s) {
-    const uint32_t P = p.T / std::min<uint32_t>(256, p.T);  // workgroup partials per window
+    using Ft = typename TailField<Fe>::type;
+    constexpr uint32_t LN = FeLanes<Ft>::value;
+    const uint32_t P = p.T / std::min<uint32_t>(reduce_block<Ft>(), p.T);  // workgroup partials per window
     uint32_t top = 1;
-    while (top < std::min<uint32_t>(P, 64)) top <<= 1;
-    hipLaunchKernelGGL(msm_window_sum_kernel<Fe>, dim3(p.W), dim3(64), 0, s, p.partials, P, top / 2, p.wsums);
+    while (top < std::min<uint32_t>(P, 64 / LN)) top <<= 1;
+    hipLaunchKernelGGL(msm_window_sum_kernel<Ft>, dim3(p.W), dim3(64), 0, s, p.partials, P, top / 2, p.wsums);
 }
 
 template <class Fe>
