@@ -100,3 +100,41 @@ def test_g2_f29_and_32bit_stages_match_oracle(gpu, n, c):
             os.environ.pop("ZKGPU_MSM_F29", None)
         else:
             os.environ["ZKGPU_MSM_F29"] = old
+
+
+@pytest.mark.parametrize("n,c,item_points,witness", [
+    (3000, 12, 0, False),    # buckets of about 3 entries, 8 per lane: lanes meet several buckets
+    (20000, 15, 0, True),    # 14- and 15-bit windows (half of the 14-bit windows' slots empty), heavy buckets
+    (5000, 11, 1, False),    # one entry per lane
+    (4000, 9, 64, True),     # long lanes over small windows, merge rounds
+    (60000, 13, 0, False),
+])
+def test_lane_cut_accumulation_matches_oracle(gpu, n, c, item_points, witness):
+    """The G1 accumulation with lanes cut at multiples of the item size of the whole sorted list
+    (ZKGPU_MSM_LANES=1; by default only for large plans) and cut at bucket starts (=0), on the
+    special-case inputs above (doubling, cancellation, infinity, zero and r - 1 scalars) and
+    witness-like scalars, against the oracle: partial sums stashed at bucket boundaries, lanes that
+    meet three or more buckets, and empty slots between windows."""
+    from conftest import witness_scalars
+    from test_gpu_msm import affine
+    from zkgpu import Group, Msm
+
+    rng = np.random.default_rng(1430 + n + c)
+    ks, sc = _inputs(rng, n)
+    if witness:
+        sc = witness_scalars(rng, n)
+    bases = O.multiples(1, O.generator_words(1), ks)
+    exp = O.msm(1, bases, sc)
+    bb = Msm.upload_bases(gpu, bases)
+    sb = gpu.buffer(sc)
+    old = os.environ.get("ZKGPU_MSM_LANES")
+    try:
+        for flag in ("1", "0"):
+            os.environ["ZKGPU_MSM_LANES"] = flag
+            m = Msm.for_group(gpu, Group.G1, n, c, item_points or None)
+            assert np.array_equal(affine(gpu, m.run(gpu, bb, sb), 1), exp), (flag, n, c, item_points)
+    finally:
+        if old is None:
+            os.environ.pop("ZKGPU_MSM_LANES", None)
+        else:
+            os.environ["ZKGPU_MSM_LANES"] = old

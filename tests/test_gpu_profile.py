@@ -19,8 +19,10 @@ def _names(rounds):
 
 
 def _merge_rounds(nv, item_points):
-    """Rounds of fan-in 4 until a bucket of nv / item_points work items has at most 16 left."""
-    m, r = -(-nv // item_points), 0
+    """Rounds of fan-in 4 until the fullest possible bucket has at most 16 partial sums left: a
+    bucket of all nv virtual points meets ceil((nv - 1) / item_points) + 1 lanes of item_points
+    consecutive sorted entries."""
+    m, r = -(-(nv - 1) // item_points) + 1, 0
     while m > 16:
         m, r = -(-m // 4), r + 1
     return r

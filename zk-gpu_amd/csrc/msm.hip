@@ -471,10 +471,17 @@ static constexpr int SCAN_THREADS = 1024;
 static constexpr int SCAN_PER_THREAD = 4;
 static constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_PER_THREAD;
 
-__device__ __forceinline__ uint32_t items_of(uint32_t count, uint32_t ch) { return (count + ch - 1) / ch; }
+// Work-item slots of a bucket of `count` sorted entries: the most lanes of ch consecutive entries
+// of the whole sorted list that a run of `count` entries can meet, ceil((count - 1) / ch) + 1 (the
+// G1 accumulation's lanes are cut at multiples of ch, not at bucket starts); kernels that cut
+// items at bucket starts use ceil(count / ch) of them and mark the spare slot empty.
+__device__ __forceinline__ uint32_t items_of(uint32_t count, uint32_t ch, uint32_t lane_cut) {
+    return !count ? 0u : lane_cut ? (count + ch - 2) / ch + 1 : (count + ch - 1) / ch;
+}
 
 __global__ __launch_bounds__(SCAN_THREADS) void scan_block_kernel(const uint32_t* __restrict__ counts, uint32_t S,
-                                                                 uint32_t ch, uint32_t* __restrict__ starts,
+                                                                 uint32_t ch, uint32_t lane_cut,
+                                                                 uint32_t* __restrict__ starts,
                                                                  uint32_t* __restrict__ istarts,
                                                                  uint32_t* __restrict__ block_sums) {
     __shared__ uint32_t sp[SCAN_THREADS], si[SCAN_THREADS];
@@ -486,8 +493,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_block_kernel(const uint32_t
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         cp[k] = base + k < S ? counts[base + k] : 0;
         tp += cp[k];
-        ti += items_of(cp[k], ch);
-        mi = max(mi, items_of(cp[k], ch));
+        ti += items_of(cp[k], ch, lane_cut);
+        mi = max(mi, items_of(cp[k], ch, lane_cut));
     }
     sp[threadIdx.x] = tp;
     si[threadIdx.x] = ti;
@@ -511,7 +518,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_block_kernel(const uint32_t
             istarts[base + k] = ri;
         }
         rp += cp[k];
-        ri += items_of(cp[k], ch);
+        ri += items_of(cp[k], ch, lane_cut);
     }
     if (threadIdx.x == SCAN_THREADS - 1) {
         block_sums[2 * blockIdx.x] = sp[threadIdx.x];
@@ -621,9 +628,12 @@ MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uin
     sh.aw = group == 1 ? 16 : 32;
     sh.xw = 2 * sh.aw;
     const uint64_t S = (uint64_t)sh.W * sh.B;
-    sh.max_items = (size_t)sh.W * ((nv + sh.CH - 1) / sh.CH + sh.B);
+    // slots: sum over buckets of ceil((count - 1) / CH) + 1 <= W nv / CH + 2 W B
+    sh.max_items = (size_t)sh.W * ((nv + sh.CH - 1) / sh.CH + 2 * (size_t)sh.B);
     if (sh.max_items >= 0xffffffffull) throw Error(ZKG_ERR_INVALID, "too many work items for one plan");
-    for (uint64_t m = (nv + sh.CH - 1) / sh.CH; m > FINAL_GROUP; m = (m + SUM_GROUP - 1) / SUM_GROUP) ++sh.merge_rounds;
+    // the most slots one bucket can hold (all nv virtual points in one bucket)
+    for (uint64_t m = (nv + sh.CH - 2) / sh.CH + 1; m > FINAL_GROUP; m = (m + SUM_GROUP - 1) / SUM_GROUP)
+        ++sh.merge_rounds;
     sh.scan_blocks = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
     sh.lds_sort = c <= 16;
     sh.sort_pts = sort_points ? std::max<uint32_t>(sort_points, sh.B) : msm_default_sort_points(nv, c);
@@ -652,6 +662,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     c = win;
     glv = sh.glv;
     f29 = msm_f29_enabled();
+    lanes = msm_lanes_enabled((uint64_t)sh.W * sh.nv);
     nv = sh.nv;
     W = sh.W;
     wb = sh.wb;
@@ -779,6 +790,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, const MsmShape& 
     c = win;
     glv = sh.glv;
     f29 = msm_f29_enabled();
+    lanes = msm_lanes_enabled((uint64_t)sh.W * sh.nv);
     nv = sh.nv;
     W = sh.W;
     wb = sh.wb;
@@ -843,6 +855,13 @@ uint32_t msm_default_sort_points(size_t nv, uint32_t c) {
 // time into the window sums).
 uint32_t msm_reduce_chunk(int group, uint32_t c, uint32_t windows) {
     return group == 2 && ((uint64_t)windows << (c - 1)) < (1u << 18) ? 2 : 4;
+}
+
+bool msm_lanes_enabled(uint64_t entries) {
+    const char* e = std::getenv("ZKGPU_MSM_LANES");
+    if (e && e[0] == '1') return true;
+    if (e && e[0] == '0') return false;
+    return entries >= (1ull << 23);
 }
 
 bool msm_f29_enabled() {
@@ -947,8 +966,8 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
                                p.wb, p.nw, p.counts, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     }
     stage("count");
-    hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(SCAN_THREADS), 0, s, p.counts, S, p.CH, p.starts, p.istarts,
-                       p.scan_tmp);
+    hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(SCAN_THREADS), 0, s, p.counts, S, p.CH,
+                       p.lane_cut() ? 1u : 0u, p.starts, p.istarts, p.scan_tmp);
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, p.scan_tmp, nb, S, p.starts, p.istarts);
     hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.scan_tmp, S, p.starts, p.istarts);
     stage("scan");

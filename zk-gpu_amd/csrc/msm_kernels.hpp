@@ -51,7 +51,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         else hi = mid;
     }
     const uint32_t slot = lo;
-    const uint32_t skip = (item - istarts[slot]) * ch;
+    // items cut at the bucket's start: ceil(count / ch) of the bucket's slots; the spare one (the
+    // slot layout leaves room for lanes cut at multiples of ch instead) holds no items
+    const uint32_t idx = item - istarts[slot], nitems = (counts[slot] + ch - 1) / ch;
+    if (idx >= nitems) {
+        item_info[item] = make_uint2(0u, 0u);
+        return;
+    }
+    const uint32_t skip = idx * ch;
     const uint32_t len = min(ch, counts[slot] - skip);
     const uint32_t* entries = sorted + starts[slot] + skip;
     Xyzz<Fe> acc = xyzz_inf<Fe>();
@@ -102,8 +109,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                           to_storage_field(acc.zzz, tag)};
         store_xyzz(item_pts + (size_t)(4 * Tr::WORDS) * item, o);
     }
-    // (position in the bucket, items in the bucket) for the merge rounds
-    item_info[item] = make_uint2(item - istarts[slot], istarts[slot + 1] - istarts[slot]);
+    // (position in the bucket, items in the bucket) for the merge rounds and the finalize
+    item_info[item] = make_uint2(idx, nitems);
 }
 
 // G1 accumulation with the field in nine 29-bit limbs (field29.hpp): the same work split and the
@@ -115,6 +122,184 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 // instantiation (only Fe = Fq29, the G1 pipeline in 29-bit limbs).
 template <class Fe>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void msm_accumulate29_kernel(
+                                                            const uint32_t* __restrict__ bases,
+                                                            const uint32_t* __restrict__ endo_x, uint32_t n_real,
+                                                            const uint32_t* __restrict__ sorted,
+                                                            const uint32_t* __restrict__ starts,
+                                                            const uint32_t* __restrict__ istarts, uint32_t S,
+                                                            uint32_t ch, uint2* __restrict__ item_info,
+                                                            uint32_t* __restrict__ item_pts) {
+    static_assert(std::is_same<Fe, Fq29>::value, "29-bit limbs are for the base field of G1");
+    // Lane k adds the sorted entries [k ch, (k + 1) ch) of all windows' buckets together: every lane
+    // of a wave runs the same number of additions (cutting lanes at bucket starts instead leaves each
+    // bucket's last, shorter item idle for the rest of its wave). Where its range crosses into the
+    // next bucket the lane keeps the finished sum in a second accumulator and starts again; both
+    // are stored after the loop (a lane that crosses twice, only possible with buckets shorter than
+    // ch, stores the older one in the loop). A bucket's partial sums are those of the lanes its
+    // entries meet, at slots istarts[b] + (k - the lane of its first entry).
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = starts[S];
+    const uint64_t pos0 = (uint64_t)lane * ch;
+    if (pos0 >= total) return;
+    const uint32_t p0 = (uint32_t)pos0, end = (uint32_t)min<uint64_t>(pos0 + ch, total);
+    // the bucket holding entry p: the last slot in [from, S) whose start is <= p (empty slots share
+    // their start with the next, so the last one among equals is the non-empty bucket holding p).
+    // A binary search: after a window whose digits use only part of its slots, thousands of
+    // empty slots may separate two non-empty buckets.
+    auto bucket_of = [&](uint32_t p, uint32_t from) {
+        uint32_t lo = from, hi = S;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (starts[mid] <= p) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    uint32_t slot = bucket_of(p0, 0), bend = starts[slot + 1];
+    // the next non-empty bucket and its end, found before the loop: a load inside the loop would
+    // make the wait for it also wait for the point read ahead (lanes meeting three buckets, only
+    // possible with buckets shorter than ch, look the third one up in the loop)
+    uint32_t nslot = slot, nbend = bend;
+    if (bend < end) {
+        nslot = bucket_of(bend, slot + 1);
+        nbend = starts[nslot + 1];
+    }
+    // stores the partial sum of bucket b held in limbs (X, Y, ZZ, ZZZ; infinity when inf)
+    auto emit = [&](uint32_t b, const F29& X, const F29& Y, const F29& ZZ, const F29& ZZZ, bool inf) {
+        const uint32_t s0 = starts[b], first = s0 / ch, last = (starts[b + 1] - 1) / ch;
+        const uint32_t k = lane - first, nitems = last - first + 1, out = istarts[b] + k;
+        Xyzz<Fq29> o = xyzz_inf<Fq29>();
+        if (!inf) {
+            const F29 one = f29_const_one();
+            const F29 x = f29_mul(X, one), y = f29_mul(Y, one);  // value-preserving, below 2p
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                o.x.l[i] = x.l[i];
+                o.y.l[i] = y.l[i];
+                o.zz.l[i] = ZZ.l[i];
+                o.zzz.l[i] = ZZZ.l[i];
+            }
+        }
+        store_xyzz(item_pts + (size_t)32 * out, o);
+        item_info[out] = make_uint2(k, nitems);
+        // the lane of the bucket's last entry marks the bucket's spare slot, if any, as empty
+        if (lane == last && istarts[b] + nitems < istarts[b + 1]) item_info[istarts[b] + nitems] = make_uint2(0u, 0u);
+    };
+    const F29 one = f29_const_one();
+    F29 X = one, Y = one, ZZ = one, ZZZ = one;
+    bool inf = true;
+    // the stashed sum of the previous bucket: in LDS (36 words per lane), so that it takes no
+    // registers from the additions
+    __shared__ uint32_t stash_lds[36 * 256];
+    uint32_t* const stash = stash_lds + threadIdx.x;  // word i at stash[256 i]: lanes on distinct banks
+    bool s_inf = true, stashed = false;
+    uint32_t s_slot = 0;
+    auto put = [&](const F29& v, int at) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) stash[256 * (at + i)] = v.l[i];
+    };
+    auto get = [&](int at) {
+        F29 v;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) v.l[i] = stash[256 * (at + i)];
+        return v;
+    };
+    // software pipelining: entry j + 1 is read while entry j is added (the gathered 64-byte reads
+    // would otherwise wait behind each addition)
+    uint32_t e_next = sorted[p0];
+    Fq nx, ny;
+    {
+        const uint32_t* xp;
+        const uint32_t* yp;
+        point_address<8>(e_next, n_real, bases, endo_x, &xp, &yp);
+        nx = fp_load<FqParams>(xp);
+        ny = fp_load<FqParams>(yp);
+    }
+    for (uint32_t pos = p0; pos < end; ++pos) {
+        const uint32_t e = e_next;
+        const Fq qx = nx, qy = ny;
+        if (pos + 1 < end) {
+            e_next = sorted[pos + 1];
+            const uint32_t* xp;
+            const uint32_t* yp;
+            point_address<8>(e_next, n_real, bases, endo_x, &xp, &yp);
+            nx = fp_load<FqParams>(xp);
+            ny = fp_load<FqParams>(yp);
+        }
+        if (pos == bend) {  // entry pos starts the next non-empty bucket
+            if (stashed) emit(s_slot, get(0), get(9), get(18), get(27), s_inf);  // only with buckets shorter than ch
+            put(X, 0);
+            put(Y, 9);
+            put(ZZ, 18);
+            put(ZZZ, 27);
+            s_inf = inf;
+            s_slot = slot;
+            stashed = true;
+            inf = true;
+            ZZ = one;
+            ZZZ = one;
+            slot = nslot;
+            bend = nbend;
+            if (bend < end) {  // a third bucket in this lane's range
+                nslot = bucket_of(bend, slot + 1);
+                nbend = starts[nslot + 1];
+            }
+        }
+        if (fp_is_zero(qx) && fp_is_zero(qy)) continue;  // the point at infinity
+        const F29 xs = f29_from_r256_words(qx.w), ys = f29_from_r256_words(qy.w);  // < 32
+        const bool neg = e >> 31;
+        // ZZ = ZZZ = 1 while the sum is empty, so these two products are then the point itself,
+        // reduced: a lane starting a bucket shares the addition's first products with the others
+        // instead of running a path of its own
+        const F29 u2 = f29_mul(xs, ZZ);    // 32 * 2 -> < 2
+        F29 s2 = f29_mul(ys, ZZZ);         // < 2
+        if (neg) s2 = f29_neg<3>(s2);      // < 3
+        if (inf) {
+            X = u2;  // < 2
+            Y = s2;  // < 3
+            inf = false;
+            continue;
+        }
+        const F29 pp = f29_sub<10>(u2, X);  // < 12
+        const F29 r = f29_sub<6>(s2, Y);    // < 9
+        if (f29_is_zero_below12p(pp)) {
+            if (f29_is_zero_below12p(r)) {  // the same point: double (rare; in the 32-bit field)
+                Xyzz<Fq> a{f29_to_fq(X), f29_to_fq(Y), f29_to_fq(ZZ), f29_to_fq(ZZZ)};
+                a = xyzz_dbl(a);
+                X = f29_from_fq(a.x);
+                Y = f29_from_fq(a.y);
+                ZZ = f29_from_fq(a.zz);
+                ZZZ = f29_from_fq(a.zzz);
+            } else {
+                inf = true;  // P + (-P)
+                ZZ = one;
+                ZZZ = one;
+            }
+            continue;
+        }
+        const F29 ppsq = f29_mul(pp, pp);              // 12 * 12 -> < 2
+        const F29 ppp = f29_mul(pp, ppsq);             // 12 * 2 -> < 2
+        const F29 q = f29_mul(X, ppsq);                // 9 * 2 -> < 2
+        const F29 rr = f29_mul(r, r);                  // 9 * 9 -> < 2
+        const F29 t = f29_add(ppp, f29_add(q, q));     // < 6, limbs < 3 * 2^29
+        const F29 x3 = f29_sub<7>(rr, t);              // < 9
+        const F29 d = f29_sub<10>(q, x3);              // < 12
+        const F29 y3 = f29_sub<3>(f29_mul(r, d), f29_mul(Y, ppp));  // 9 * 12, 5 * 2 -> < 5
+        ZZ = f29_mul(ZZ, ppsq);                        // < 2
+        ZZZ = f29_mul(ZZZ, ppp);                       // < 2
+        X = x3;
+        Y = y3;
+    }
+    // item sums in the Fq29 memory form (R = 2^261 value below 2p), read by the Fq29 stages
+    if (stashed) emit(s_slot, get(0), get(9), get(18), get(27), s_inf);
+    emit(slot, X, Y, ZZ, ZZZ, inf);
+}
+
+// The same accumulation with work items cut at bucket starts (ceil(count / CH) items per bucket,
+// one lane each): the default for plans sorting fewer than 2^23 entries, where the lane-cut kernel's
+// second stored partial per lane and the finalize's extra partials outweigh the fuller waves.
+template <class Fe>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void msm_accumulate29_bucket_kernel(
                                                             const uint32_t* __restrict__ bases,
                                                             const uint32_t* __restrict__ endo_x, uint32_t n_real,
                                                             const uint32_t* __restrict__ sorted,
@@ -259,7 +444,8 @@ struct LaneOps;
 // (split rounds), which shortens the serial chain and multiplies the waves.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __restrict__ item_pts,
-                                                          const uint32_t* __restrict__ istarts, uint32_t S,
+                                                          const uint32_t* __restrict__ istarts,
+                                                          const uint2* __restrict__ item_info, uint32_t S,
                                                           uint32_t split, uint32_t* __restrict__ buckets) {
     typedef FeTraits<Fe> Tr;
     constexpr uint32_t LN = FeLanes<Fe>::value;  // lanes per element (the lanes of a pair stay together)
@@ -269,7 +455,8 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
     const size_t xw = 4 * Tr::WORDS;
     Xyzz<Fe> acc = xyzz_inf<Fe>();
     if (valid) {
-        const uint32_t first = istarts[slot], total = istarts[slot + 1] - first;
+        // the bucket's item count (its slots may include a spare one): its first slot's info
+        const uint32_t first = istarts[slot], total = istarts[slot + 1] > first ? item_info[first].y : 0u;
         uint32_t stride = 1;
         while ((total + stride - 1) / stride > FINAL_GROUP) stride *= SUM_GROUP;
         const uint32_t m = (total + stride - 1) / stride;        // terms k * stride, k < m
@@ -728,9 +915,17 @@ void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
                            bases, (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
                            p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
     } else if constexpr (std::is_same<Fe, Fq29>::value) {
-        hipLaunchKernelGGL(msm_accumulate29_kernel<Fq29>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
-                           (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
-                           p.counts, p.W * p.B, p.CH, p.item_info, p.item_pts);
+        if (!p.lane_cut()) {
+            hipLaunchKernelGGL(msm_accumulate29_bucket_kernel<Fq29>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s,
+                               bases, (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts, p.counts,
+                               p.W * p.B, p.CH, p.item_info, p.item_pts);
+            return;
+        }
+        // one lane per CH sorted entries (at most W nv of them)
+        const size_t lanes = ((size_t)p.W * p.nv + p.CH - 1) / p.CH;
+        hipLaunchKernelGGL(msm_accumulate29_kernel<Fq29>, dim3(ceil_div(lanes, 256)), dim3(256), 0, s, bases,
+                           (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts, p.W * p.B, p.CH,
+                           p.item_info, p.item_pts);
     } else {
         hipLaunchKernelGGL(msm_accumulate_kernel<Fe>, dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s, bases,
                            (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
@@ -769,7 +964,7 @@ void launch_finalize(const MsmPlan& p, hipStream_t s) {
     using Ft = typename TailField<Fe>::type;
     constexpr uint32_t LN = FeLanes<Ft>::value;
     hipLaunchKernelGGL(msm_finalize_kernel<Ft>, dim3(ceil_div(((size_t)S << split) * LN, 256)), dim3(256), 0, s,
-                       p.item_pts, p.istarts, S, split, p.buckets);
+                       p.item_pts, p.istarts, (const uint2*)p.item_info, S, split, p.buckets);
 }
 
 // Logical threads (one per element) per workgroup of the bucket reduction: 256 lanes.
