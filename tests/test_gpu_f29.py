@@ -138,3 +138,38 @@ def test_lane_cut_accumulation_matches_oracle(gpu, n, c, item_points, witness):
             os.environ.pop("ZKGPU_MSM_LANES", None)
         else:
             os.environ["ZKGPU_MSM_LANES"] = old
+
+
+@pytest.mark.parametrize("group,f29,n,c,item_points", [
+    (2, "1", 1500, 11, 0),
+    (2, "1", 2500, 9, 8),
+    (2, "0", 2000, 10, 0),
+    (1, "0", 3000, 12, 0),
+    (1, "0", 20000, 15, 0),
+])
+def test_lane_cut_accumulation_other_fields(gpu, group, f29, n, c, item_points):
+    """The lane-cut accumulation of the generic kernel: G2 (32-bit Fq2 additions, item sums stored
+    for the 29-bit stages or the 32-bit ones) and the 32-bit G1 pipeline, against the oracle, with
+    ZKGPU_MSM_LANES=1 and =0."""
+    from test_gpu_msm import affine
+    from zkgpu import Group, Msm
+
+    rng = np.random.default_rng(1440 + n + c + group)
+    ks, sc = _inputs(rng, n)
+    bases = O.multiples(group, O.generator_words(group), ks)
+    exp = O.msm(group, bases, sc)
+    bb = Msm.upload_bases(gpu, bases)
+    sb = gpu.buffer(sc)
+    old = {k: os.environ.get(k) for k in ("ZKGPU_MSM_LANES", "ZKGPU_MSM_F29")}
+    try:
+        os.environ["ZKGPU_MSM_F29"] = f29
+        for flag in ("1", "0"):
+            os.environ["ZKGPU_MSM_LANES"] = flag
+            m = Msm.for_group(gpu, Group(group), n, c, item_points or None)
+            assert np.array_equal(affine(gpu, m.run(gpu, bb, sb), group), exp), (flag, group, f29, n, c)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v

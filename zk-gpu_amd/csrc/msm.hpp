@@ -124,14 +124,14 @@ struct MsmPlan {
     uint32_t n_real() const { return pre ? 2 * n : glv ? n : 0xffffffffu; }
     // Points the bases buffer must hold: 2n for a precomputed plan, else n.
     size_t base_points() const { return pre ? 2 * (size_t)n : n; }
-    // Whether the bucket accumulation cuts its lanes at multiples of CH of the sorted list (the
-    // G1 29-bit kernel; a bucket then takes up to ceil((count - 1) / CH) + 1 slots) rather than at
-    // bucket starts (ceil(count / CH) slots).
-    bool lane_cut() const { return group == 1 && f29 && lanes; }
-    // Lanes cut at multiples of CH for the G1 29-bit accumulation: by default when the plan sorts
-    // at least 2^23 entries (W nv), where it measured faster (G1 2^20 c = 15 accumulate 1.52 ->
-    // 1.46 ms, 2^22 c = 16 5.98 -> 5.81 ms; G1 2^16 0.21 -> 0.22 ms); ZKGPU_MSM_LANES=1 or 0
-    // forces it on or off.
+    // Whether the bucket accumulation cuts its lanes at multiples of CH of the sorted list (a
+    // bucket then takes up to ceil((count - 1) / CH) + 1 slots) rather than at bucket starts
+    // (ceil(count / CH) slots).
+    bool lane_cut() const { return lanes; }
+    // Lanes cut at multiples of CH for the bucket accumulation: by default when the plan sorts at
+    // least 2^23 entries (W nv), where it measured faster (G1 2^20 c = 15 accumulate 1.52 -> 1.46
+    // ms, 2^22 c = 16 5.98 -> 5.81 ms; G1 2^16 0.21 -> 0.22 ms); ZKGPU_MSM_LANES=1 or 0 forces it
+    // on or off.
     bool lanes = false;
     ~MsmPlan();
     // Queue the MSM on `stream`. With `stage_ms` set, wait around every stage and record

@@ -113,6 +113,101 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     item_info[item] = make_uint2(idx, nitems);
 }
 
+// The accumulation of msm_accumulate_kernel with lanes cut at multiples of ch of the whole sorted
+// list, as msm_accumulate29_kernel does for the G1 29-bit pipeline (there: why, and the layout of
+// the partial sums): every lane of a wave runs ch additions; a lane stashes its finished sum in
+// LDS where its range enters the next bucket and stores both after the loop.
+template <class Fe, class Out = Fe>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void msm_accumulate_lanes_kernel(
+                                                            const uint32_t* __restrict__ bases,
+                                                            const uint32_t* __restrict__ endo_x, uint32_t n_real,
+                                                            const uint32_t* __restrict__ sorted,
+                                                            const uint32_t* __restrict__ starts,
+                                                            const uint32_t* __restrict__ istarts, uint32_t S,
+                                                            uint32_t ch, uint2* __restrict__ item_info,
+                                                            uint32_t* __restrict__ item_pts) {
+    typedef FeTraits<Fe> Tr;
+    constexpr int XW = 4 * Tr::WORDS;  // words of one XYZZ point (the same in Out's memory form)
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = starts[S];
+    const uint64_t pos0 = (uint64_t)lane * ch;
+    if (pos0 >= total) return;
+    const uint32_t p0 = (uint32_t)pos0, end = (uint32_t)min<uint64_t>(pos0 + ch, total);
+    auto bucket_of = [&](uint32_t p, uint32_t from) {  // last slot in [from, S) starting at or before p
+        uint32_t lo = from, hi = S;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (starts[mid] <= p) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    uint32_t slot = bucket_of(p0, 0), bend = starts[slot + 1];
+    uint32_t nslot = slot, nbend = bend;
+    if (bend < end) {
+        nslot = bucket_of(bend, slot + 1);
+        nbend = starts[nslot + 1];
+    }
+    auto emit = [&](uint32_t b, const Xyzz<Fe>& acc) {
+        const uint32_t first = starts[b] / ch, last = (starts[b + 1] - 1) / ch;
+        const uint32_t k = lane - first, nitems = last - first + 1, out = istarts[b] + k;
+        if constexpr (std::is_same<Fe, Out>::value) {
+            store_xyzz(item_pts + (size_t)XW * out, acc);
+        } else {
+            const Out* tag = nullptr;
+            Xyzz<Out> o = xyzz_inf<Out>();
+            if (!xyzz_is_inf(acc))
+                o = Xyzz<Out>{to_storage_field(acc.x, tag), to_storage_field(acc.y, tag),
+                              to_storage_field(acc.zz, tag), to_storage_field(acc.zzz, tag)};
+            store_xyzz(item_pts + (size_t)XW * out, o);
+        }
+        item_info[out] = make_uint2(k, nitems);
+        if (lane == last && istarts[b] + nitems < istarts[b + 1]) item_info[istarts[b] + nitems] = make_uint2(0u, 0u);
+    };
+    // the stashed sum of the previous bucket, word i at stash[256 i] (lanes on distinct banks)
+    __shared__ uint32_t stash_lds[XW * 256];
+    uint32_t* const stash = stash_lds + threadIdx.x;
+    auto put = [&](const Xyzz<Fe>& v) {
+        uint32_t w[XW];
+        store_xyzz(w, v);
+#pragma unroll
+        for (int i = 0; i < XW; ++i) stash[256 * i] = w[i];
+    };
+    auto get = [&]() {
+        uint32_t w[XW];
+#pragma unroll
+        for (int i = 0; i < XW; ++i) w[i] = stash[256 * i];
+        return load_xyzz<Fe>(w);
+    };
+    Xyzz<Fe> acc = xyzz_inf<Fe>();
+    bool stashed = false;
+    uint32_t s_slot = 0;
+    for (uint32_t pos = p0; pos < end; ++pos) {
+        if (pos == bend) {  // entry pos starts the next non-empty bucket
+            if (stashed) emit(s_slot, get());  // only with buckets shorter than ch
+            put(acc);
+            s_slot = slot;
+            stashed = true;
+            acc = xyzz_inf<Fe>();
+            slot = nslot;
+            bend = nbend;
+            if (bend < end) {  // a third bucket in this lane's range
+                nslot = bucket_of(bend, slot + 1);
+                nbend = starts[nslot + 1];
+            }
+        }
+        const uint32_t e = sorted[pos];
+        const uint32_t* xp;
+        const uint32_t* yp;
+        point_address<Tr::WORDS>(e, n_real, bases, endo_x, &xp, &yp);
+        Affine<Fe> p{Tr::load(xp), Tr::load(yp)};
+        if (e >> 31) p.y = Tr::zero() - p.y;
+        acc = xyzz_madd(acc, p);
+    }
+    if (stashed) emit(s_slot, get());
+    emit(slot, acc);
+}
+
 // G1 accumulation with the field in nine 29-bit limbs (field29.hpp): the same work split and the
 // same point law (madd-2008-s with every special case) as msm_accumulate_kernel<Fq>; the item sums
 // are written in Fq29's memory form for the Fq29 instantiation of the later stages. Bases are read in R = 2^256 form and taken as 32x their
@@ -910,6 +1005,18 @@ __global__ __launch_bounds__(64) void msm_combine29_kernel(const uint32_t* __res
 
 template <class Fe>
 void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
+    const size_t lanes = ((size_t)p.W * p.nv + p.CH - 1) / p.CH;  // lane-cut: one lane per CH sorted entries
+    if (p.lane_cut() && !std::is_same<Fe, Fq29>::value) {
+        if constexpr (std::is_same<Fe, Fq2_29>::value)
+            hipLaunchKernelGGL((msm_accumulate_lanes_kernel<Fq2, Fq2_29>), dim3(ceil_div(lanes, 256)), dim3(256), 0, s,
+                               bases, (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
+                               p.W * p.B, p.CH, p.item_info, p.item_pts);
+        else if constexpr (!std::is_same<Fe, Fq29>::value)
+            hipLaunchKernelGGL(msm_accumulate_lanes_kernel<Fe>, dim3(ceil_div(lanes, 256)), dim3(256), 0, s, bases,
+                               (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts, p.W * p.B,
+                               p.CH, p.item_info, p.item_pts);
+        return;
+    }
     if constexpr (std::is_same<Fe, Fq2_29>::value) {
         hipLaunchKernelGGL((msm_accumulate_kernel<Fq2, Fq2_29>), dim3(ceil_div(p.max_items, 256)), dim3(256), 0, s,
                            bases, (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts,
@@ -921,8 +1028,6 @@ void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s) {
                                p.W * p.B, p.CH, p.item_info, p.item_pts);
             return;
         }
-        // one lane per CH sorted entries (at most W nv of them)
-        const size_t lanes = ((size_t)p.W * p.nv + p.CH - 1) / p.CH;
         hipLaunchKernelGGL(msm_accumulate29_kernel<Fq29>, dim3(ceil_div(lanes, 256)), dim3(256), 0, s, bases,
                            (const uint32_t*)p.endo_x, p.n_real(), p.sorted, p.starts, p.istarts, p.W * p.B, p.CH,
                            p.item_info, p.item_pts);
