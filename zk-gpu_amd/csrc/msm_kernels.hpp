@@ -725,8 +725,17 @@ struct LaneOps<Fq2_29> : LaneOps2<Fq29, Fq2_29> {
 };
 
 template <>
-struct LaneOps<Fq2P> {  // lane shuffles of the G2 tail stages: each lane moves its own component
+struct LaneOps<Fq2P> {
+    // lane shuffles of the G2 tail stages: each lane moves its own component
     static ZKG_DEV Fq2P shfl(const Fq2P& x, int src) { return Fq2P{LaneOps<Fq29>::shfl(x.v, src)}; }
+    // products spread over lane pairs: pair k (lanes 2k, 2k + 1) computes product k, each lane its
+    // own component (two Fq products), and every lane fetches its component of product k from
+    // the lane of its parity in pair k with one shuffle
+    static constexpr int SPREAD = 2;
+    static ZKG_DEV Fq2P partial(uint32_t, const Fq2P& a, const Fq2P& b) { return a * b; }
+    static ZKG_DEV Fq2P gather(const Fq2P& m, int k) {
+        return Fq2P{LaneOps<Fq29>::shfl(m.v, 2 * k + (int)(__lane_id() & 1u))};
+    }
 };
 
 template <>
@@ -818,21 +827,15 @@ struct CombineField {
 
 template <>
 struct CombineField<Fq2_29> {
-    using type = Fq2;
-    static ZKG_DEV Fq one(const Fq29& x) {
-        F29 t;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) t.l[i] = x.l[i];
-        return f29_to_fq(t);
-    }
-    static ZKG_DEV Fq2 conv(const Fq2_29& x) { return Fq2{one(x.a), one(x.b)}; }
+    // G2 (29-bit stages): the chain on lane pairs (Fq2P, the same memory form), each Fq2 product
+    // by one pair. Measured at G2 2^18 against three lanes per product in the 32-bit Fq2 (0.94
+    // ms) and in Fq2_29 (1.16 ms, Karatsuba's partial products gathered by shuffles).
+    using type = Fq2P;
 };
 
 template <class Fe>
 ZKG_DEV Xyzz<typename CombineField<Fe>::type> load_combine(const uint32_t* p) {
-    typedef CombineField<Fe> C;
-    const Xyzz<Fe> v = load_xyzz<Fe>(p);
-    return Xyzz<typename C::type>{C::conv(v.x), C::conv(v.y), C::conv(v.zz), C::conv(v.zzz)};
+    return load_xyzz<typename CombineField<Fe>::type>(p);
 }
 
 template <class Fe>
@@ -851,7 +854,7 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
             if (!xyzz_is_inf(acc)) acc = xyzz_dbl_lanes(acc, lane);
         acc = xyzz_add_lanes(acc, load_combine<Fe>(wsums + (size_t)XW * (w + zero)), lane);
     }
-    if (lane == 0) store_xyzz_result(result, acc);
+    if (lane < FeLanes<Fc>::value) store_xyzz_result(result, acc);  // Fq2P: each lane of the pair its component
 }
 
 // ---------------------------------------------------------------- G1 combine on lazy 29-bit limbs
