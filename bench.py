@@ -33,12 +33,15 @@ R = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 R_WORDS = np.array([(R >> (32 * i)) & 0xFFFFFFFF for i in range(8)], dtype=np.uint32)
 HBM_PEAK_GBPS = 8000.0
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz, full-rate 32-bit integer ops
-# rocprofv3 measurements of ntt_pass_kernel<11,512> on NTT 2^24 (profiles/ntt_pass_2^24_hbm_traffic.md,
-# profiles/valu_rate_msm_accumulate_and_ntt_pass.md): HBM bytes per launch (FETCH_SIZE scaled by the
-# copy-kernel calibration factor 2.0, WRITE_SIZE by 1.0) and VALU lane operations per launch.
+# rocprofv3 measurements of ntt_pass_kernel<11,512> on NTT 2^24 (profiles/ntt_pass_2^24_per_pass.md,
+# profiles/valu_rate_msm_accumulate_and_ntt_pass.md): HBM bytes per launch, the mean of the three
+# passes (FETCH_SIZE scaled by the copy-kernel calibration factor 2.0, WRITE_SIZE by 1.0), and VALU
+# lane operations per launch.
 NTT24_KERNEL = "ntt_pass_kernel<11,512>"  # the default tile at 2^24: 2^11 elements, 512 lanes
-NTT24_TRAFFIC_BYTES_PER_LAUNCH = 1.536e9
+NTT24_TRAFFIC_BYTES_PER_LAUNCH = 1.537e9
 NTT24_VALU_LANE_OPS_PER_LAUNCH = 2.044e10
+# The same for msm_accumulate29_kernel on the headline plan (G1 2^20, c = 15, 589824 lanes).
+MSM_ACC_VALU_LANE_OPS_PER_LAUNCH = 4.762e10
 
 
 def log(msg):
@@ -735,6 +738,10 @@ def main():
         "frac": round(acc_products / (acc_ms * 1e-3) / 1e9 / mul29_gps, 4),
         "call_G_products_per_s": round(call_products / (g1_ms * 1e-3) / 1e9, 1),
         "call_frac": round(call_products / (g1_ms * 1e-3) / 1e9 / mul29_gps, 4),
+        "valu": {"lane_ops_per_launch": MSM_ACC_VALU_LANE_OPS_PER_LAUNCH,
+                 "achieved": round(MSM_ACC_VALU_LANE_OPS_PER_LAUNCH / (acc_ms * 1e-3) / 1e12, 2),
+                 "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2), "unit": "T lane-ops/s",
+                 "frac": round(MSM_ACC_VALU_LANE_OPS_PER_LAUNCH / (acc_ms * 1e-3) / VALU_PEAK_LANE_OPS, 4)},
         "note": "peak = the measured rate of independent 29-bit Montgomery products (the accumulation's own "
                 "arithmetic) over the whole GPU; accumulate_ms from Msm::profile",
     }
