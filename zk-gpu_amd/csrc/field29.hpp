@@ -126,7 +126,7 @@ ZKG_DEV F29 f29_mul(const F29& a, const F29& b) {
         for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
 #pragma unroll
         for (int j = 0; j < k; ++j) acc += (uint64_t)m[j] * M::P(k - j);
-        m[k] = ((uint32_t)acc * M::PINV) & f29::MASK;
+        m[k] = mul_lo_full_rate((uint32_t)acc, M::PINV) & f29::MASK;
         acc += (uint64_t)m[k] * M::P(0);  // clears the low 29 bits
         acc >>= 29;
     }
@@ -480,13 +480,111 @@ struct FeTraits<Fq2P> {
 
 ZKG_DEV Fq2P fe_select(bool c, const Fq2P& x, const Fq2P& y) { return Fq2P{fe_select(c, x.v, y.v)}; }
 
-// Lanes that hold one field element: 2 for Fq2P, else 1.
+// ---------------------------------------------------------------- G1 points on a pair of lanes
+// An Fq29 value held by both lanes of a pair (the same memory form as Fq29). Field operations act
+// on both copies; the point formulas (xyzz_add and xyzz_dbl below) split each level of
+// independent products between the two lanes and swap the results, so a point addition costs a
+// lane the latency of 7 products instead of 14 and a doubling 5 instead of 9. Used by the G1
+// bucket reduction and window sums, whose serial chains of additions on too few lanes set their
+// time. Both lanes of a pair must execute every operation together (branches on pair-uniform
+// conditions only: both lanes hold the same values).
+struct Fq29Pair {
+    Fq29 v;
+};
+
+ZKG_DEV Fq29Pair operator+(const Fq29Pair& x, const Fq29Pair& y) { return Fq29Pair{x.v + y.v}; }
+ZKG_DEV Fq29Pair operator-(const Fq29Pair& x, const Fq29Pair& y) { return Fq29Pair{x.v - y.v}; }
+ZKG_DEV Fq29Pair operator*(const Fq29Pair& x, const Fq29Pair& y) { return Fq29Pair{x.v * y.v}; }
+
+template <>
+struct FeTraits<Fq29Pair> {
+    static constexpr int WORDS = 8;
+    static ZKG_DEV Fq29Pair zero() { return Fq29Pair{FeTraits<Fq29>::zero()}; }
+    static ZKG_DEV Fq29Pair one() { return Fq29Pair{FeTraits<Fq29>::one()}; }
+    static ZKG_DEV bool is_zero(const Fq29Pair& x) { return fq29_is_zero(x.v); }
+    static ZKG_DEV Fq29Pair sqr(const Fq29Pair& x) { return Fq29Pair{x.v * x.v}; }
+    static ZKG_DEV Fq29Pair load(const uint32_t* p) { return Fq29Pair{FeTraits<Fq29>::load(p)}; }
+    // both lanes hold the value: the even lane writes it
+    static ZKG_DEV void store(uint32_t* p, const Fq29Pair& x) {
+        if (!pair_odd()) FeTraits<Fq29>::store(p, x.v);
+    }
+    static ZKG_DEV void store_result(uint32_t* p, const Fq29Pair& x) {
+        if (!pair_odd()) FeTraits<Fq29>::store_result(p, x.v);
+    }
+};
+
+ZKG_DEV Fq29Pair fe_select(bool c, const Fq29Pair& x, const Fq29Pair& y) { return Fq29Pair{fe_select(c, x.v, y.v)}; }
+
+// (even lane's m, odd lane's m) of a pair, on both lanes
+ZKG_DEV void pair_swap(const Fq29& m, bool odd, Fq29* even_val, Fq29* odd_val) {
+    const Fq29 o = pair_other(m);
+    *even_val = fe_select(odd, o, m);
+    *odd_val = fe_select(odd, m, o);
+}
+
+// 2P (dbl-2008-s-1, the formulas of the generic xyzz_dbl) with each level's products split
+// between the lanes of a pair. Infinity (ZZ = ZZZ = 0) maps to infinity.
+template <>
+ZKG_DEV Xyzz<Fq29Pair> xyzz_dbl<Fq29Pair>(const Xyzz<Fq29Pair>& p) {
+    const bool odd = pair_odd();
+    const Fq29 x = p.x.v, y = p.y.v;
+    const Fq29 u = y + y;
+    Fq29 v, x2, s, m2, w, msx, wy, zzz3;
+    // V = U^2 | X^2
+    const Fq29 t0 = fe_select(odd, x, u);
+    pair_swap(t0 * t0, odd, &v, &x2);
+    const Fq29 mm = (x2 + x2) + x2;
+    // S = X V | M^2, then ZZ3 = V ZZ on both lanes (each lane has one product slot per level)
+    pair_swap(fe_select(odd, mm, x) * fe_select(odd, mm, v), odd, &s, &m2);
+    const Fq29 x3 = m2 - (s + s);
+    const Fq29 zz3 = v * p.zz.v;
+    // W = U V | M (S - X3)
+    pair_swap(fe_select(odd, mm, u) * fe_select(odd, s - x3, v), odd, &w, &msx);
+    // W Y | W ZZZ
+    pair_swap(w * fe_select(odd, p.zzz.v, y), odd, &wy, &zzz3);
+    return Xyzz<Fq29Pair>{{x3}, {msx - wy}, {zz3}, {zzz3}};
+}
+
+// P + Q (add-2008-s, the formulas of the generic xyzz_add, every special case) with each level's
+// products split between the lanes of a pair.
+template <>
+ZKG_DEV Xyzz<Fq29Pair> xyzz_add<Fq29Pair>(const Xyzz<Fq29Pair>& p, const Xyzz<Fq29Pair>& q) {
+    if (xyzz_is_inf(q)) return p;
+    if (xyzz_is_inf(p)) return q;
+    const bool odd = pair_odd();
+    Fq29 u1, u2, s1, s2, zz12, zzz12, ppsq, r2, ppp, qq, ye, yo, zz3, zzz3;
+    // U1 = X1 ZZ2 | U2 = X2 ZZ1;  S1 = Y1 ZZZ2 | S2 = Y2 ZZZ1;  ZZ1 ZZ2 | ZZZ1 ZZZ2
+    pair_swap(fe_select(odd, q.x.v, p.x.v) * fe_select(odd, p.zz.v, q.zz.v), odd, &u1, &u2);
+    pair_swap(fe_select(odd, q.y.v, p.y.v) * fe_select(odd, p.zzz.v, q.zzz.v), odd, &s1, &s2);
+    pair_swap(fe_select(odd, p.zzz.v, p.zz.v) * fe_select(odd, q.zzz.v, q.zz.v), odd, &zz12, &zzz12);
+    const Fq29 pp = u2 - u1, r = s2 - s1;
+    if (fq29_is_zero(pp)) {
+        if (fq29_is_zero(r)) return xyzz_dbl(p);
+        return xyzz_inf<Fq29Pair>();
+    }
+    // PP = P^2 | R^2
+    const Fq29 t = fe_select(odd, r, pp);
+    pair_swap(t * t, odd, &ppsq, &r2);
+    // PPP = P PP | Q = U1 PP
+    pair_swap(fe_select(odd, u1, pp) * ppsq, odd, &ppp, &qq);
+    const Fq29 x3 = (r2 - ppp) - (qq + qq);
+    // R (Q - X3) | S1 PPP;  ZZ3 = ZZ1 ZZ2 PP | ZZZ3 = ZZZ1 ZZZ2 PPP
+    pair_swap(fe_select(odd, s1, r) * fe_select(odd, ppp, qq - x3), odd, &ye, &yo);
+    pair_swap(fe_select(odd, zzz12, zz12) * fe_select(odd, ppp, ppsq), odd, &zz3, &zzz3);
+    return Xyzz<Fq29Pair>{{x3}, {ye - yo}, {zz3}, {zzz3}};
+}
+
+// Lanes that hold one field element: 2 for Fq2P and Fq29Pair, else 1.
 template <class Fe>
 struct FeLanes {
     static constexpr uint32_t value = 1;
 };
 template <>
 struct FeLanes<Fq2P> {
+    static constexpr uint32_t value = 2;
+};
+template <>
+struct FeLanes<Fq29Pair> {
     static constexpr uint32_t value = 2;
 };
 

@@ -840,7 +840,10 @@ bool msm_glv_enabled(size_t n) {
 
 // Points per counting-sort block when none is given: at least one bucket range (B) and 8192, and
 // nv / 64 (fewer blocks means a smaller per-block cursor table to build, scan and read; measured
-// on MI355X at 2^16 .. 2^22 within 2% of the best block size, which was 2B .. 8B).
+// on MI355X at 2^16 .. 2^22 within 2% of the best block size, which was 2B .. 8B). Sizing the
+// blocks so that all W windows' blocks fit the GPU in one round of workgroups measured slower
+// (G1 2^20, c = 15: 504 blocks of 37888 points 0.326 ms against 576 of 32768 points 0.288 ms):
+// the placement's time follows the points per block.
 uint32_t msm_default_sort_points(size_t nv, uint32_t c) {
     const size_t B = (size_t)1 << (c - 1);
     return (uint32_t)std::max<size_t>(std::max<size_t>(B, 8192), std::min<size_t>(nv / 64, (size_t)1 << 20));

@@ -506,7 +506,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
 
 // Round r (stride s = G^r): in a bucket with more than FINAL_GROUP partial sums left, item
 // k = 0 mod s*G absorbs items k + s .. k + (G-1)s. G = 4 keeps each round's chain at three
-// additions; typical buckets (up to FINAL_GROUP items) skip the rounds.
+// additions; typical buckets (up to FINAL_GROUP items) skip the rounds. The grid is bounded
+// (MERGE_BLOCKS workgroups striding over the items), so a round that no bucket needs costs one
+// small launch: at one lane per item slot, the idle rounds of a G1 2^20 plan took 0.09 ms.
 template <class Fe>
 __global__ __launch_bounds__(256) void msm_merge_kernel(uint32_t* __restrict__ item_pts,
                                                        const uint2* __restrict__ item_info,
@@ -516,19 +518,21 @@ __global__ __launch_bounds__(256) void msm_merge_kernel(uint32_t* __restrict__ i
     // no bucket needs this round (the scan recorded the most items of any bucket): exit before
     // reading any item
     if ((max_items[0] + stride - 1) / stride <= FINAL_GROUP) return;
-    const uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / FeLanes<Fe>::value;  // pair-uniform
-    if (item >= istarts[S]) return;
-    const uint2 info = item_info[item];
-    const uint32_t k = info.x, total = info.y;
-    if ((total + stride - 1) / stride <= FINAL_GROUP) return;
-    if (k % (stride * SUM_GROUP) != 0) return;
+    constexpr uint32_t LN = FeLanes<Fe>::value;  // the lanes of a pair take the same item
+    const uint32_t nitems = istarts[S], step = gridDim.x * blockDim.x / LN;
     const size_t xw = 4 * Tr::WORDS;
-    Xyzz<Fe> acc = load_xyzz<Fe>(item_pts + xw * item);
-    for (uint32_t j = 1; j < SUM_GROUP; ++j) {
-        if (k + j * stride >= total) break;
-        acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (item + j * stride)));
+    for (uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / LN; item < nitems; item += step) {
+        const uint2 info = item_info[item];
+        const uint32_t k = info.x, total = info.y;
+        if ((total + stride - 1) / stride <= FINAL_GROUP) continue;
+        if (k % (stride * SUM_GROUP) != 0) continue;
+        Xyzz<Fe> acc = load_xyzz<Fe>(item_pts + xw * item);
+        for (uint32_t j = 1; j < SUM_GROUP; ++j) {
+            if (k + j * stride >= total) break;
+            acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (item + j * stride)));
+        }
+        store_xyzz(item_pts + xw * item, acc);
     }
-    store_xyzz(item_pts + xw * item, acc);
 }
 
 template <class Fe>
@@ -590,9 +594,14 @@ struct ReduceWaves<Fq2P> {
 // the lanes of a workgroup (all in one window) then add their results in an LDS tree, leaving
 // one partial sum per workgroup. Each loop has a single point-addition site (operands chosen by
 // selects), which keeps the fully inlined kernel small.
+// The grid holds only workgroups with buckets: windows w < nw (wb + 1 digit bits, 2^wb buckets)
+// take gh workgroups each, the others (2^(wb-1) buckets, half of B with the GLV split's balanced
+// widths) gl; window w's partials sit at w * P. Workgroups of bucket ranges beyond a window's
+// digits would be idle, and placed round-robin they doubled up busy ones on a CU.
 template <class Fe>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ReduceWaves<Fe>::value))) void msm_reduce_kernel(const uint32_t* __restrict__ buckets, uint32_t W,
                                                         uint32_t B, uint32_t L, uint32_t c, uint32_t wb, uint32_t nw,
+                                                        uint32_t gh, uint32_t gl, uint32_t P,
                                                         uint32_t* __restrict__ partials) {
     typedef FeTraits<Fe> Tr;
     constexpr int XW = 4 * Tr::WORDS;
@@ -602,9 +611,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ReduceWaves
     __shared__ uint32_t lds[128 * XP];
     constexpr uint32_t LN = FeLanes<Fe>::value;      // lanes per element
     const uint32_t tid = threadIdx.x / LN, nthr = blockDim.x / LN;  // logical threads
-    const uint32_t T = B / L;  // logical threads per window, a multiple of nthr
-    const uint32_t t = blockIdx.x * nthr + tid;
-    const uint32_t w = t / T, j = t % T, b0 = j * L;
+    const uint32_t hi_wgs = nw * gh;
+    const uint32_t w = blockIdx.x < hi_wgs ? blockIdx.x / gh : nw + (blockIdx.x - hi_wgs) / gl;
+    const uint32_t g = blockIdx.x < hi_wgs ? blockIdx.x % gh : (blockIdx.x - hi_wgs) % gl;
+    const uint32_t j = g * nthr + tid, b0 = j * L;
     Xyzz<Fe> r = xyzz_inf<Fe>();
     if (b0 < (1u << (wb + (w < nw ? 1u : 0u) - 1))) {  // else beyond this window's digit range: empty
         const uint32_t* src = buckets + (size_t)XW * ((size_t)w * B + b0);
@@ -630,24 +640,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ReduceWaves
         if (tid < off) r = xyzz_add(r, load_xyzz<Fe>(lds + XP * tid));
         __syncthreads();
     }
-    if (tid == 0) store_xyzz(partials + (size_t)XW * blockIdx.x, r);
+    if (tid == 0) store_xyzz(partials + (size_t)XW * ((size_t)w * P + g), r);
 }
 
-// One wave per window adds the window's P workgroup partials (P <= 64 at c <= 16 needs no
-// serial part; larger windows add P / 64 per lane first), then an LDS tree of `top` levels.
+// One wave per window adds the window's workgroup partials (gh for windows w < nw, else gl; at
+// most 64 at c <= 16 needs no serial part, more add several per lane first), then an LDS tree of
+// `top` levels. Window w's partials start at w * P.
 template <class Fe>
 __global__ __launch_bounds__(64) void msm_window_sum_kernel(const uint32_t* __restrict__ partials, uint32_t P,
-                                                           uint32_t top, uint32_t* __restrict__ wsums) {
+                                                           uint32_t nw, uint32_t gh, uint32_t gl, uint32_t top,
+                                                           uint32_t* __restrict__ wsums) {
     typedef FeTraits<Fe> Tr;
     constexpr int XW = 4 * Tr::WORDS;
     constexpr int XP = XW + 4;  // padded LDS slots, as in msm_reduce_kernel
     __shared__ uint32_t lds[32 * XP];
     constexpr uint32_t LN = FeLanes<Fe>::value;                       // lanes per element
     const uint32_t tid = threadIdx.x / LN, nthr = blockDim.x / LN;  // logical threads
-    const uint32_t w = blockIdx.x;
+    const uint32_t w = blockIdx.x, pw = w < nw ? gh : gl;
     const uint32_t* src = partials + (size_t)XW * w * P;
-    Xyzz<Fe> acc = tid < P ? load_xyzz<Fe>(src + XW * tid) : xyzz_inf<Fe>();
-    for (uint32_t k = tid + nthr; k < P; k += nthr) acc = xyzz_add(acc, load_xyzz<Fe>(src + XW * k));
+    Xyzz<Fe> acc = tid < pw ? load_xyzz<Fe>(src + XW * tid) : xyzz_inf<Fe>();
+    for (uint32_t k = tid + nthr; k < pw; k += nthr) acc = xyzz_add(acc, load_xyzz<Fe>(src + XW * k));
     for (uint32_t off = top; off > 0; off >>= 1) {
         if (tid >= off && tid < 2 * off) store_xyzz(lds + XP * (tid - off), acc);
         __syncthreads();
@@ -1053,11 +1065,26 @@ struct TailField<Fq2_29> {
     using type = Fq2P;
 };
 
+// The field of the bucket reduction and window sums: the tail field, except that G1 (Fq29) runs
+// them on lane pairs (Fq29Pair: each point operation's products split between the two lanes).
+// Their serial chains of additions otherwise leave most SIMDs idle: the G1 2^20 reduction had one
+// wave per two SIMDs. Measured on MI355X at G1 2^20, c = 15 (kernel trace): 0.296 ms on one lane
+// per chain, 0.279 ms on pairs over the full bucket range, 0.20 ms with the grid below.
+template <class Fe>
+struct ReduceField {
+    using type = typename TailField<Fe>::type;
+};
+template <>
+struct ReduceField<Fq29> {
+    using type = Fq29Pair;
+};
+
 template <class Fe>
 void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
     using Ft = typename TailField<Fe>::type;
     constexpr uint32_t LN = FeLanes<Ft>::value;
-    hipLaunchKernelGGL(msm_merge_kernel<Ft>, dim3(ceil_div(p.max_items * LN, 256)), dim3(256), 0, s, p.item_pts,
+    const size_t blocks = std::min<size_t>(ceil_div(p.max_items * LN, 256), MERGE_BLOCKS);
+    hipLaunchKernelGGL(msm_merge_kernel<Ft>, dim3(blocks), dim3(256), 0, s, p.item_pts,
                        p.item_info, p.istarts, p.W * p.B, stride, (const uint32_t*)p.max_items_dev);
 }
 
@@ -1081,23 +1108,40 @@ constexpr uint32_t reduce_block() {
     return 256 / FeLanes<Ft>::value;
 }
 
+// Workgroups of the bucket reduction: rb logical threads each (a workgroup never spans two
+// windows); windows below nw (2^wb buckets) take gh of them, the rest (2^(wb-1) buckets) gl, and
+// each window's partials are P = T / rb apart.
+template <class Ft>
+struct ReduceGrid {
+    uint32_t rb, gh, gl, P;
+    explicit ReduceGrid(const MsmPlan& p) {
+        rb = std::min<uint32_t>(reduce_block<Ft>(), p.T);
+        P = p.T / rb;
+        const uint32_t bh = std::min<uint32_t>(p.B, 1u << p.wb), bl = std::min<uint32_t>(p.B, 1u << (p.wb - 1));
+        gh = ceil_div(ceil_div(bh, p.L), rb);
+        gl = ceil_div(ceil_div(bl, p.L), rb);
+    }
+};
+
 template <class Fe>
 void launch_reduce(const MsmPlan& p, hipStream_t s) {
-    using Ft = typename TailField<Fe>::type;
+    using Ft = typename ReduceField<Fe>::type;
     constexpr uint32_t LN = FeLanes<Ft>::value;
-    const uint32_t rb = std::min<uint32_t>(reduce_block<Ft>(), p.T);  // a workgroup never spans two windows
-    hipLaunchKernelGGL(msm_reduce_kernel<Ft>, dim3((size_t)p.W * p.T / rb), dim3(rb * LN), 0, s, p.buckets, p.W, p.B,
-                       p.L, p.c, p.wb, p.nw, p.partials);
+    const ReduceGrid<Ft> g(p);
+    const size_t wgs = (size_t)p.nw * g.gh + (size_t)(p.W - p.nw) * g.gl;
+    hipLaunchKernelGGL(msm_reduce_kernel<Ft>, dim3(wgs), dim3(g.rb * LN), 0, s, p.buckets, p.W, p.B, p.L, p.c,
+                       p.wb, p.nw, g.gh, g.gl, g.P, p.partials);
 }
 
 template <class Fe>
 void launch_tree(const MsmPlan& p, hipStream_t s) {
-    using Ft = typename TailField<Fe>::type;
+    using Ft = typename ReduceField<Fe>::type;
     constexpr uint32_t LN = FeLanes<Ft>::value;
-    const uint32_t P = p.T / std::min<uint32_t>(reduce_block<Ft>(), p.T);  // workgroup partials per window
+    const ReduceGrid<Ft> g(p);
     uint32_t top = 1;
-    while (top < std::min<uint32_t>(P, 64 / LN)) top <<= 1;
-    hipLaunchKernelGGL(msm_window_sum_kernel<Ft>, dim3(p.W), dim3(64), 0, s, p.partials, P, top / 2, p.wsums);
+    while (top < std::min<uint32_t>(std::max(g.gh, g.gl), 64 / LN)) top <<= 1;
+    hipLaunchKernelGGL(msm_window_sum_kernel<Ft>, dim3(p.W), dim3(64), 0, s, p.partials, g.P, p.nw, g.gh, g.gl,
+                       top / 2, p.wsums);
 }
 
 template <class Fe>

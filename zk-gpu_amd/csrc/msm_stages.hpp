@@ -8,6 +8,7 @@ namespace zkg {
 
 static constexpr uint32_t SUM_GROUP = 4;    // fan-in of the heavy-bucket merge rounds
 static constexpr uint32_t FINAL_GROUP = 16;  // partial sums per bucket that finalize adds directly
+static constexpr uint32_t MERGE_BLOCKS = 256;  // workgroups of a merge round (one per CU; they stride)
 
 template <class Fe> void launch_accumulate(const MsmPlan& p, const uint32_t* bases, hipStream_t s);
 template <class Fe> void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s);
