@@ -376,3 +376,20 @@ def test_msm_generator_progression_2_18(gpu):
     for c, items in ((Msm.default_window(n), None), (15, 8)):
         m = Msm.for_group(gpu, Group.G1, n, c, items)
         assert np.array_equal(affine(gpu, m.run(gpu, bb, sb), 1), want), (c, items)
+
+
+def test_result_buffer_keeps_its_plan(gpu):
+    """The result buffer `run` returns is plan-owned memory: it keeps the plan alive, so the
+    result of a plan that is referenced nowhere else (a temporary) can still be read."""
+    import gc
+
+    from zkgpu import Group, Msm
+
+    n = 700
+    rng = np.random.default_rng(5500)
+    ks, sc = uniform_fr_words(rng, n), uniform_fr_words(rng, n)
+    bases = O.multiples(1, O.generator_words(1), ks)
+    bb, sb = Msm.upload_bases(gpu, bases), gpu.buffer(sc)
+    res = Msm.for_group(gpu, Group.G1, n, 10).run(gpu, bb, sb)
+    gc.collect()
+    assert np.array_equal(affine(gpu, res, 1), O.msm(1, bases, sc))

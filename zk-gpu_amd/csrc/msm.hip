@@ -463,6 +463,194 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_place_kernel(const uint32_t*
     }
 }
 
+// ---- two-pass placement (the default for c <= 16 where a packed entry fits 32 bits)
+// The one-pass placement above writes each 4-byte entry to its bucket's cursor, and a block holds
+// about two points per bucket, so its writes land all over the window's range: rocprof measured
+// 716 MB written to HBM for the 75 MB of entries of a G1 2^20 MSM (c = 15), 0.28 ms. Two passes
+// keep the writes local: (1) each block writes its points as runs per partition (2^plog
+// partitions per window, each a range of 2^sh buckets of the window's own bucket range) of a
+// temporary array, one run per (block, partition), reserved with one global atomic; (2) one
+// workgroup per (window, partition) counts the partition's buckets and places its entries, so
+// its scattered writes stay within a 32 KB range of the final list (at G1 2^20) that one CU
+// writes, small enough for the L2 to merge. A packed entry holds the point index (31 - kb bits),
+// the bucket within its partition (the kb bits below the sign; sh <= kb) and the sign (bit 31).
+// Loops keep four entries per lane in flight. The order of entries within a bucket depends on
+// the atomics, as in the one-pass placement; sums do not.
+// The partition tables (in the plan's `table`): bcount[w][blk][p] (a block's entries per
+// partition), pbase[w][p] (exclusive scan of the partition totals, pbase[M] = all entries) and
+// pcur[w][p] (the next free run of each partition), M = W 2^plog.
+static constexpr uint32_t PART_MAX = 256;  // partitions per window, and buckets per partition, at most
+static constexpr uint32_t BUCKET_THREADS = 256;
+
+// Bucket bits of partition ranges in window w: the window's bucket range 2^cw (cw = its digit
+// width - 1, at most cb) over 2^plog partitions.
+ZKG_DEV uint32_t part_shift(uint32_t w, uint32_t wb, uint32_t nw, uint32_t cb, uint32_t plog) {
+    const uint32_t cw = min(cb, wb + (w < nw ? 1u : 0u) - 1);
+    return cw > plog ? cw - plog : 0u;
+}
+
+// A block's entries per partition -> bcount, and into the partition totals (zeroed beforehand).
+__global__ __launch_bounds__(SORT_THREADS) void msm_part_hist_kernel(const uint32_t* __restrict__ digits, uint32_t n,
+                                                                    uint32_t P, uint32_t nblk, uint32_t wb,
+                                                                    uint32_t nw, uint32_t cb, uint32_t plog,
+                                                                    uint32_t* __restrict__ bcount,
+                                                                    uint32_t* __restrict__ ptotal) {
+    __shared__ uint32_t hist[PART_MAX];
+    const uint32_t blk = blockIdx.x, w = blockIdx.y, NP = 1u << plog, sh = part_shift(w, wb, nw, cb, plog);
+    for (uint32_t p = threadIdx.x; p < NP; p += SORT_THREADS) hist[p] = 0;
+    __syncthreads();
+    const uint32_t lo = blk * P, hi = min(lo + P, n);
+    const uint32_t* d = digits + (size_t)w * n;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * SORT_THREADS) {
+        uint32_t e[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[k] = i + k * SORT_THREADS < hi ? d[i + k * SORT_THREADS] : NO_DIGIT;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (e[k] != NO_DIGIT) atomicAdd(&hist[(e[k] & 0x7fffffffu) >> sh], 1u);
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < NP; p += SORT_THREADS) {
+        const uint32_t h = hist[p];
+        bcount[((size_t)w * nblk + blk) * NP + p] = h;
+        if (h) atomicAdd(&ptotal[w * NP + p], h);
+    }
+}
+
+// One workgroup: pbase[0 .. M) = exclusive scan of pbase (the partition totals) in place,
+// pbase[M] = the total, pcur = pbase. Four consecutive entries per lane per round of 4096.
+__global__ __launch_bounds__(1024) void msm_part_scan_kernel(uint32_t* __restrict__ pbase, uint32_t M,
+                                                            uint32_t* __restrict__ pcur) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < M; base += 4096) {
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t idx = base + 4 * threadIdx.x + k;
+            v[k] = idx < M ? pbase[idx] : 0u;
+            sum += v[k];
+        }
+        uint32_t x = sum;  // inclusive scan over the wave
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        if (wv == 0) {  // inclusive scan of the 16 wave totals
+            uint32_t t = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+            for (uint32_t off = 1; off < 16; off <<= 1) {
+                const uint32_t y = __shfl_up(t, off);
+                if (lane >= off) t += y;
+            }
+            if (lane < 16) wsum[lane] = t;
+        }
+        __syncthreads();
+        uint32_t run = carry + (wv ? wsum[wv - 1] : 0u) + x - sum;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t idx = base + 4 * threadIdx.x + k;
+            if (idx < M) {
+                pbase[idx] = run;
+                pcur[idx] = run;
+            }
+            run += v[k];
+        }
+        carry += wsum[15];
+        __syncthreads();  // wsum is rewritten next round
+    }
+    if (threadIdx.x == 0) pbase[M] = carry;
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void msm_part_place_kernel(const uint32_t* __restrict__ digits, uint32_t n,
+                                                                     uint32_t P, uint32_t nblk, uint32_t wb,
+                                                                     uint32_t nw, uint32_t cb, uint32_t plog,
+                                                                     uint32_t kb, const uint32_t* __restrict__ bcount,
+                                                                     uint32_t* __restrict__ pcur,
+                                                                     uint32_t* __restrict__ temp) {
+    __shared__ uint32_t cursor[PART_MAX];
+    const uint32_t blk = blockIdx.x, w = blockIdx.y, NP = 1u << plog, sh = part_shift(w, wb, nw, cb, plog);
+    for (uint32_t p = threadIdx.x; p < NP; p += SORT_THREADS) {  // reserve this block's run of each partition
+        const uint32_t h = bcount[((size_t)w * nblk + blk) * NP + p];
+        cursor[p] = h ? atomicAdd(&pcur[w * NP + p], h) : 0u;
+    }
+    __syncthreads();
+    const uint32_t lo = blk * P, hi = min(lo + P, n), bmask = (1u << sh) - 1;
+    const uint32_t* d = digits + (size_t)w * n;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * SORT_THREADS) {
+        uint32_t e[4], pos[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[k] = i + k * SORT_THREADS < hi ? d[i + k * SORT_THREADS] : NO_DIGIT;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (e[k] != NO_DIGIT) pos[k] = atomicAdd(&cursor[(e[k] & 0x7fffffffu) >> sh], 1u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (e[k] != NO_DIGIT)
+                temp[pos[k]] = (e[k] & 0x80000000u) | ((e[k] & bmask) << (31 - kb)) | (i + k * SORT_THREADS);
+    }
+}
+
+// Points per bucket of partition blockIdx.x of window blockIdx.y -> counts; the workgroup also
+// zeroes its share of the window's buckets beyond its digit range (which no partition covers).
+__global__ __launch_bounds__(BUCKET_THREADS) void msm_bucket_count_kernel(const uint32_t* __restrict__ temp,
+                                                                         const uint32_t* __restrict__ pbase,
+                                                                         uint32_t wb, uint32_t nw, uint32_t cb,
+                                                                         uint32_t plog, uint32_t kb, uint32_t B,
+                                                                         uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[PART_MAX];
+    const uint32_t p = blockIdx.x, w = blockIdx.y, NP = 1u << plog, sh = part_shift(w, wb, nw, cb, plog);
+    const uint32_t nb = 1u << sh, bmask = nb - 1;
+    const uint32_t lo = pbase[w * NP + p], hi = pbase[w * NP + p + 1];
+    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS) hist[b] = 0;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * BUCKET_THREADS) {
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = i + k * BUCKET_THREADS < hi ? temp[i + k * BUCKET_THREADS] : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * BUCKET_THREADS < hi) atomicAdd(&hist[(x[k] >> (31 - kb)) & bmask], 1u);
+    }
+    __syncthreads();
+    uint32_t* cw = counts + (size_t)w * B;
+    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS) cw[((size_t)p << sh) + b] = hist[b];
+    const uint32_t covered = NP << sh, share = (B - covered) >> plog;  // uncovered buckets per partition
+    for (uint32_t b = threadIdx.x; b < share; b += BUCKET_THREADS) cw[covered + p * share + b] = 0;
+}
+
+// Places partition blockIdx.x of window blockIdx.y: entries go to their buckets' cursors (from the
+// scan's starts) as point index | sign << 31.
+__global__ __launch_bounds__(BUCKET_THREADS) void msm_bucket_place_kernel(const uint32_t* __restrict__ temp,
+                                                                         const uint32_t* __restrict__ pbase,
+                                                                         uint32_t wb, uint32_t nw, uint32_t cb,
+                                                                         uint32_t plog, uint32_t kb, uint32_t B,
+                                                                         const uint32_t* __restrict__ starts,
+                                                                         uint32_t* __restrict__ sorted) {
+    __shared__ uint32_t cursor[PART_MAX];
+    const uint32_t p = blockIdx.x, w = blockIdx.y, NP = 1u << plog, sh = part_shift(w, wb, nw, cb, plog);
+    const uint32_t nb = 1u << sh, bmask = nb - 1, fmask = ((1u << kb) - 1) << (31 - kb);
+    const uint32_t lo = pbase[w * NP + p], hi = pbase[w * NP + p + 1];
+    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS)
+        cursor[b] = starts[(size_t)w * B + ((size_t)p << sh) + b];
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * BUCKET_THREADS) {
+        uint32_t x[4], pos[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = i + k * BUCKET_THREADS < hi ? temp[i + k * BUCKET_THREADS] : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * BUCKET_THREADS < hi) pos[k] = atomicAdd(&cursor[(x[k] >> (31 - kb)) & bmask], 1u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * BUCKET_THREADS < hi) sorted[pos[k]] = x[k] & ~fmask;
+    }
+}
+
 // ---------------------------------------------------------------- scan of counts and items
 // Three kernels: per-block exclusive scans (4096 slots per block), a single-block scan of the
 // block totals, and the add-back of block offsets. Both scans (points and items) run together.
@@ -663,6 +851,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     glv = sh.glv;
     f29 = msm_f29_enabled();
     lanes = msm_lanes_enabled((uint64_t)sh.W * sh.nv);
+    part_bits = msm_part_bits(sh, &part_log);
     nv = sh.nv;
     W = sh.W;
     wb = sh.wb;
@@ -791,6 +980,7 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, const MsmShape& 
     glv = sh.glv;
     f29 = msm_f29_enabled();
     lanes = msm_lanes_enabled((uint64_t)sh.W * sh.nv);
+    part_bits = msm_part_bits(sh, &part_log);
     nv = sh.nv;
     W = sh.W;
     wb = sh.wb;
@@ -858,6 +1048,25 @@ uint32_t msm_default_sort_points(size_t nv, uint32_t c) {
 // time into the window sums).
 uint32_t msm_reduce_chunk(int group, uint32_t c, uint32_t windows) {
     return group == 2 && ((uint64_t)windows << (c - 1)) < (1u << 18) ? 2 : 4;
+}
+
+uint32_t msm_part_bits(const MsmShape& sh, uint32_t* part_log) {
+    *part_log = 0;
+    if (!sh.lds_sort) return 0;
+    const char* e = std::getenv("ZKGPU_MSM_PARTSORT");
+    if (e && e[0] == '0') return 0;
+    uint32_t cb = 0;  // bucket bits: B = 2^cb
+    while ((1u << cb) < sh.B) ++cb;
+    // 2^8 partitions per window (fewer for small windows: at least four buckets each)
+    const uint32_t plog = std::min<uint32_t>(8, cb - 2);
+    const uint32_t kb = cb - plog;  // bucket bits within a partition
+    if ((uint64_t)sh.nv > (1ull << (31 - kb))) return 0;  // the point index needs 31 - kb bits
+    if ((1u << plog) > PART_MAX || (1u << kb) > PART_MAX) return 0;
+    // bcount, pbase (M + 1) and pcur must fit the plan's table of W nblk B words
+    const uint64_t M = (uint64_t)sh.W << plog;
+    if (M * sh.nblk + 2 * M + 1 > (uint64_t)sh.W * sh.nblk * sh.B) return 0;
+    *part_log = plog;
+    return kb;
 }
 
 bool msm_lanes_enabled(uint64_t entries) {
@@ -944,20 +1153,41 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         dscal = p.split;
     }
     const unsigned pg = ceil_div(p.nv, 256);
+    // two-pass placement: the digits go to `sorted`'s buffer, the partition runs to `digits`'s,
+    // and the final list to `sorted` (both W * nv words)
+    const uint32_t kb = p.part_bits, plog = p.part_log, NP = kb ? 1u << plog : 0;
+    const uint32_t cb = p.c - 1;  // bucket bits of a full window
+    uint32_t* dig = kb ? p.sorted : p.digits;
     if (p.lds_sort) {
         if (p.pre)
             hipLaunchKernelGGL(msm_digits_split_store_kernel, dim3(ceil_div(2 * (size_t)p.n, 256)), dim3(256), 0, s,
-                               dscal, p.n, p.W, p.wb, p.nw, p.digits);
+                               dscal, p.n, p.W, p.wb, p.nw, dig);
         else if (p.glv)
             hipLaunchKernelGGL(msm_digits_store_kernel<true>, dim3(pg), dim3(256), 0, s, dscal, p.nv, p.W, p.wb,
-                               p.nw, p.digits);
+                               p.nw, dig);
         else
             hipLaunchKernelGGL(msm_digits_store_kernel<false>, dim3(pg), dim3(256), 0, s, dscal, p.nv, p.W, p.wb,
-                               p.nw, p.digits);
-        hipLaunchKernelGGL(msm_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.nv, p.B,
-                           p.sort_pts, p.nblk, p.table);
-        hipLaunchKernelGGL(msm_colscan_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.table, p.W, p.B, p.nblk,
-                           p.counts);
+                               p.nw, dig);
+        if (kb) {
+            const uint32_t M = p.W * NP;  // partitions in all windows
+            uint32_t* bcount = p.table;
+            uint32_t* pbase = bcount + (size_t)M * p.nblk;
+            uint32_t* pcur = pbase + M + 1;
+            fill_u32(pbase, 0, M, s);
+            hipLaunchKernelGGL(msm_part_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 0, s, dig, p.nv,
+                               p.sort_pts, p.nblk, p.wb, p.nw, cb, plog, bcount, pbase);
+            hipLaunchKernelGGL(msm_part_scan_kernel, dim3(1), dim3(1024), 0, s, pbase, M, pcur);
+            hipLaunchKernelGGL(msm_part_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 0, s, dig, p.nv,
+                               p.sort_pts, p.nblk, p.wb, p.nw, cb, plog, kb, (const uint32_t*)bcount, pcur, p.digits);
+            hipLaunchKernelGGL(msm_bucket_count_kernel, dim3(NP, p.W), dim3(BUCKET_THREADS), 0, s,
+                               (const uint32_t*)p.digits, (const uint32_t*)pbase, p.wb, p.nw, cb, plog, kb, p.B,
+                               p.counts);
+        } else {
+            hipLaunchKernelGGL(msm_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, dig, p.nv, p.B,
+                               p.sort_pts, p.nblk, p.table);
+            hipLaunchKernelGGL(msm_colscan_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.table, p.W, p.B,
+                               p.nblk, p.counts);
+        }
     } else {
         fill_u32(p.counts, 0, S, s);
         fill_u32(p.cursors, 0, S, s);
@@ -974,7 +1204,12 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, p.scan_tmp, nb, S, p.starts, p.istarts);
     hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.scan_tmp, S, p.starts, p.istarts);
     stage("scan");
-    if (p.lds_sort) {
+    if (p.lds_sort && kb) {
+        const uint32_t* pbase = p.table + (size_t)p.W * NP * p.nblk;
+        hipLaunchKernelGGL(msm_bucket_place_kernel, dim3(NP, p.W), dim3(BUCKET_THREADS), 0, s,
+                           (const uint32_t*)p.digits, pbase, p.wb, p.nw, cb, plog, kb, p.B,
+                           (const uint32_t*)p.starts, p.sorted);
+    } else if (p.lds_sort) {
         hipLaunchKernelGGL(msm_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.nv, p.B,
                            p.sort_pts, p.nblk, p.table, p.starts, p.sorted);
     } else if (p.glv) {

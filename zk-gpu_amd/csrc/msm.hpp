@@ -51,6 +51,10 @@ bool msm_f29_enabled();
 // Whether a plan sorting `entries` (W nv) entries cuts the G1 accumulation's lanes at multiples of
 // its item size (see MsmPlan::lanes).
 bool msm_lanes_enabled(uint64_t entries);
+// Bucket bits within a partition of the two-pass placement for a plan of this shape, or 0 where it
+// does not apply (c > 16, or a packed entry would not hold the point index; ZKGPU_MSM_PARTSORT=0
+// turns it off for comparisons).
+uint32_t msm_part_bits(const MsmShape& sh, uint32_t* part_log);
 // Counting-sort block size (points) a plan over nv virtual points with window c uses by default.
 uint32_t msm_default_sort_points(size_t nv, uint32_t c);
 // Buckets per lane of the bucket-reduction stage (windows of 2^(c-1) buckets each).
@@ -133,6 +137,9 @@ struct MsmPlan {
     // ms, 2^22 c = 16 5.98 -> 5.81 ms; G1 2^16 0.21 -> 0.22 ms); ZKGPU_MSM_LANES=1 or 0 forces it
     // on or off.
     bool lanes = false;
+    // Two-pass placement (msm.hip): bits of the bucket-within-partition field of a packed entry, or
+    // 0 for the one-pass placement; 2^part_log partitions per window.
+    uint32_t part_bits = 0, part_log = 0;
     ~MsmPlan();
     // Queue the MSM on `stream`. With `stage_ms` set, wait around every stage and record
     // (stage name, milliseconds) as the reference's Msm::profile does.

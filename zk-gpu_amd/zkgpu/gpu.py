@@ -20,10 +20,11 @@ ZKG_ERR_DEVICE = 3
 class Buffer:
     """A device buffer (the reference's wgpu::Buffer). Owned buffers are freed with the handle."""
 
-    def __init__(self, gpu: "Gpu", handle: int, owned: bool = True):
+    def __init__(self, gpu: "Gpu", handle: int, owned: bool = True, keep=None):
         self._gpu = gpu  # keeps the device open while the buffer lives
         self._h = ctypes.c_void_p(handle)
         self._owned = owned
+        self._keep = keep  # the owner of memory this buffer does not own (e.g. an MSM plan)
 
     @property
     def handle(self):

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes
+import weakref
 from typing import Optional, Tuple
 
 import numpy as np
@@ -124,21 +125,27 @@ class Msm:
     def window(self) -> int:
         return int(_lib.load().zkg_msm_window(self._h))
 
+    def _result_for(self, gpu, handle: int) -> Buffer:
+        # The returned buffer refers to plan-owned memory, so it holds the plan (which would free
+        # that memory) alive; the plan keeps only a weak reference back, so dropping both frees them
+        # at once (the reference's borrow of the plan, which Python cannot express).
+        buf = self._result() if self._result is not None else None
+        if buf is None or buf.handle.value != handle:
+            buf = Buffer(gpu, handle, owned=False, keep=self)
+            self._result = weakref.ref(buf)
+        return buf
+
     def result_buffer(self, gpu) -> Buffer:
         """The plan-owned result buffer (written by `run` or by a batch that ran this plan)."""
         h = ctypes.c_void_p()
         check(_lib.load().zkg_msm_result(self._h, ctypes.byref(h)))
-        if self._result is None or self._result.handle.value != h.value:
-            self._result = Buffer(gpu, h.value, owned=False)
-        return self._result
+        return self._result_for(gpu, h.value)
 
     def run(self, gpu, bases: Buffer, scalars: Buffer) -> Buffer:
         """Queue sum(scalars[i] * bases[i]); returns the plan-owned XYZZ result buffer."""
         h = ctypes.c_void_p()
         check(_lib.load().zkg_msm_run(self._h, bases.handle, scalars.handle, ctypes.byref(h)))
-        if self._result is None or self._result.handle.value != h.value:
-            self._result = Buffer(gpu, h.value, owned=False)
-        return self._result
+        return self._result_for(gpu, h.value)
 
     async def run_yielding(self, gpu, bases: Buffer, scalars: Buffer) -> Buffer:
         """Msm::run_yielding (src/msm.rs:300-318): waits for the GPU after the heavy stages so a
@@ -149,9 +156,7 @@ class Msm:
         lib = _lib.load()
         await asyncio.to_thread(lambda: check(lib.zkg_msm_run_yielding(self._h, bases.handle, scalars.handle,
                                                                        ctypes.byref(h))))
-        if self._result is None or self._result.handle.value != h.value:
-            self._result = Buffer(gpu, h.value, owned=False)
-        return self._result
+        return self._result_for(gpu, h.value)
 
     def profile(self, gpu, bases: Buffer, scalars: Buffer):
         """Run once with a wait around every stage; [(stage, milliseconds)] (src/msm.rs:321-342)."""
