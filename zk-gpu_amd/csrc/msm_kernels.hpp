@@ -562,9 +562,16 @@ __global__ __launch_bounds__(256) void msm_finalize_kernel(const uint32_t* __res
         const uint32_t share = (m + (1u << split) - 1) >> split;  // terms per part
         const uint32_t lo = min(m, part * share), hi = min(m, lo + share);
         if (lo < hi) {
+            // item k + 1 is read while item k is added (half the kernel's wave cycles waited on the
+            // item reads with one read per addition)
             acc = load_xyzz<Fe>(item_pts + xw * (first + (size_t)lo * stride));
-            for (uint32_t k = lo + 1; k < hi; ++k)
-                acc = xyzz_add(acc, load_xyzz<Fe>(item_pts + xw * (first + (size_t)k * stride)));
+            Xyzz<Fe> next = acc;
+            if (lo + 1 < hi) next = load_xyzz<Fe>(item_pts + xw * (first + (size_t)(lo + 1) * stride));
+            for (uint32_t k = lo + 1; k < hi; ++k) {
+                const Xyzz<Fe> cur = next;
+                if (k + 1 < hi) next = load_xyzz<Fe>(item_pts + xw * (first + (size_t)(k + 1) * stride));
+                acc = xyzz_add(acc, cur);
+            }
         }
     }
     for (uint32_t r = 0; r < split; ++r) {  // uniform over the grid
