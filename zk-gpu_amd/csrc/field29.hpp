@@ -574,7 +574,100 @@ ZKG_DEV Xyzz<Fq29Pair> xyzz_add<Fq29Pair>(const Xyzz<Fq29Pair>& p, const Xyzz<Fq
     return Xyzz<Fq29Pair>{{x3}, {ye - yo}, {zz3}, {zzz3}};
 }
 
-// Lanes that hold one field element: 2 for Fq2P and Fq29Pair, else 1.
+// ---------------------------------------------------------------- G2 points on four lanes
+// An Fq2P value (one Fq component per lane of a pair) held by both pairs of a group of four lanes;
+// the G2 point formulas below split each level of independent Fq2 products between the two pairs,
+// as Fq29Pair does for G1, halving the serial chains of the G2 bucket reduction and window sums.
+// The pairs of a group execute every operation together (branches on group-uniform conditions).
+struct Fq2Q {
+    Fq2P v;
+};
+
+ZKG_DEV bool quad_odd() { return (__lane_id() & 2u) != 0; }
+
+// the same component from the other pair of the group
+ZKG_DEV Fq2P quad_other(const Fq2P& x) {
+    Fq29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = __shfl_xor(x.v.l[i], 2);
+    return Fq2P{r};
+}
+
+ZKG_DEV Fq2Q operator+(const Fq2Q& x, const Fq2Q& y) { return Fq2Q{x.v + y.v}; }
+ZKG_DEV Fq2Q operator-(const Fq2Q& x, const Fq2Q& y) { return Fq2Q{x.v - y.v}; }
+ZKG_DEV Fq2Q operator*(const Fq2Q& x, const Fq2Q& y) { return Fq2Q{x.v * y.v}; }
+
+template <>
+struct FeTraits<Fq2Q> {
+    static constexpr int WORDS = 16;
+    static ZKG_DEV Fq2Q zero() { return Fq2Q{FeTraits<Fq2P>::zero()}; }
+    static ZKG_DEV Fq2Q one() { return Fq2Q{FeTraits<Fq2P>::one()}; }
+    static ZKG_DEV bool is_zero(const Fq2Q& x) { return FeTraits<Fq2P>::is_zero(x.v); }
+    static ZKG_DEV Fq2Q sqr(const Fq2Q& x) { return Fq2Q{FeTraits<Fq2P>::sqr(x.v)}; }
+    static ZKG_DEV Fq2Q load(const uint32_t* p) { return Fq2Q{FeTraits<Fq2P>::load(p)}; }
+    // both pairs hold the value: the first pair writes it
+    static ZKG_DEV void store(uint32_t* p, const Fq2Q& x) {
+        if (!quad_odd()) FeTraits<Fq2P>::store(p, x.v);
+    }
+    static ZKG_DEV void store_result(uint32_t* p, const Fq2Q& x) {
+        if (!quad_odd()) FeTraits<Fq2P>::store_result(p, x.v);
+    }
+};
+
+ZKG_DEV Fq2Q fe_select(bool c, const Fq2Q& x, const Fq2Q& y) { return Fq2Q{fe_select(c, x.v, y.v)}; }
+
+// (first pair's m, second pair's m) of a group, on all four lanes
+ZKG_DEV void quad_swap(const Fq2P& m, bool odd, Fq2P* even_val, Fq2P* odd_val) {
+    const Fq2P o = quad_other(m);
+    *even_val = fe_select(odd, o, m);
+    *odd_val = fe_select(odd, m, o);
+}
+
+// 2P (dbl-2008-s-1, the formulas of the generic xyzz_dbl), each level's products split between
+// the pairs of the group, as xyzz_dbl<Fq29Pair>.
+template <>
+ZKG_DEV Xyzz<Fq2Q> xyzz_dbl<Fq2Q>(const Xyzz<Fq2Q>& p) {
+    const bool odd = quad_odd();
+    const Fq2P x = p.x.v, y = p.y.v;
+    const Fq2P u = y + y;
+    Fq2P v, x2, s, m2, w, msx, wy, zzz3;
+    const Fq2P t0 = fe_select(odd, x, u);
+    quad_swap(FeTraits<Fq2P>::sqr(t0), odd, &v, &x2);  // V = U^2 | X^2
+    const Fq2P mm = (x2 + x2) + x2;
+    quad_swap(fe_select(odd, mm, x) * fe_select(odd, mm, v), odd, &s, &m2);  // S = X V | M^2
+    const Fq2P x3 = m2 - (s + s);
+    const Fq2P zz3 = v * p.zz.v;
+    quad_swap(fe_select(odd, mm, u) * fe_select(odd, s - x3, v), odd, &w, &msx);  // W = U V | M (S - X3)
+    quad_swap(w * fe_select(odd, p.zzz.v, y), odd, &wy, &zzz3);                    // W Y | W ZZZ
+    return Xyzz<Fq2Q>{{x3}, {msx - wy}, {zz3}, {zzz3}};
+}
+
+// P + Q (add-2008-s, the formulas of the generic xyzz_add, every special case), products split
+// between the pairs of the group, as xyzz_add<Fq29Pair>.
+template <>
+ZKG_DEV Xyzz<Fq2Q> xyzz_add<Fq2Q>(const Xyzz<Fq2Q>& p, const Xyzz<Fq2Q>& q) {
+    typedef FeTraits<Fq2P> T;
+    if (xyzz_is_inf(q)) return p;
+    if (xyzz_is_inf(p)) return q;
+    const bool odd = quad_odd();
+    Fq2P u1, u2, s1, s2, zz12, zzz12, ppsq, r2, ppp, qq, ye, yo, zz3, zzz3;
+    quad_swap(fe_select(odd, q.x.v, p.x.v) * fe_select(odd, p.zz.v, q.zz.v), odd, &u1, &u2);
+    quad_swap(fe_select(odd, q.y.v, p.y.v) * fe_select(odd, p.zzz.v, q.zzz.v), odd, &s1, &s2);
+    quad_swap(fe_select(odd, p.zzz.v, p.zz.v) * fe_select(odd, q.zzz.v, q.zz.v), odd, &zz12, &zzz12);
+    const Fq2P pp = u2 - u1, r = s2 - s1;
+    if (T::is_zero(pp)) {
+        if (T::is_zero(r)) return xyzz_dbl(p);
+        return xyzz_inf<Fq2Q>();
+    }
+    quad_swap(T::sqr(fe_select(odd, r, pp)), odd, &ppsq, &r2);           // PP = P^2 | R^2
+    quad_swap(fe_select(odd, u1, pp) * ppsq, odd, &ppp, &qq);            // PPP = P PP | Q = U1 PP
+    const Fq2P x3 = (r2 - ppp) - (qq + qq);
+    quad_swap(fe_select(odd, s1, r) * fe_select(odd, ppp, qq - x3), odd, &ye, &yo);  // R (Q - X3) | S1 PPP
+    quad_swap(fe_select(odd, zzz12, zz12) * fe_select(odd, ppp, ppsq), odd, &zz3, &zzz3);
+    return Xyzz<Fq2Q>{{x3}, {ye - yo}, {zz3}, {zzz3}};
+}
+
+// Lanes that hold one field element: 4 for Fq2Q, 2 for Fq2P and Fq29Pair, else 1.
 template <class Fe>
 struct FeLanes {
     static constexpr uint32_t value = 1;
@@ -586,6 +679,10 @@ struct FeLanes<Fq2P> {
 template <>
 struct FeLanes<Fq29Pair> {
     static constexpr uint32_t value = 2;
+};
+template <>
+struct FeLanes<Fq2Q> {
+    static constexpr uint32_t value = 4;
 };
 
 // The storage form of the Fq29 / Fq2_29 stages, from canonical 32-bit coordinates.

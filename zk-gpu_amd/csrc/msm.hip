@@ -1041,13 +1041,13 @@ uint32_t msm_default_sort_points(size_t nv, uint32_t c) {
 
 // Buckets per lane of the bucket reduction: its serial chain is 2L additions plus the workgroup
 // tree, against W B / L lanes in all. Measured on MI355X: L = 4 for G1 (2^20: 2.90 ms, 3.00 with
-// L = 2; over precomputed bases at c = 16, 131072 buckets, L = 2 and 4 tie) and for G2 with 2^18
-// buckets or more (2^22, c = 16: reduce 1.08 ms, 1.86 with L = 2); L = 2 for G2 with fewer
-// buckets (2^18, c = 13, 40960 buckets: 4.52 ms, 4.59 with L = 4; precomputed 2^18 and 2^20 at
-// c = 16, 131072 buckets: reduce 0.99 ms, 1.07-1.09 with L = 4, 1.80 with L = 1, which moves the
-// time into the window sums).
+// L = 2; over precomputed bases at c = 16, 131072 buckets, L = 2 and 4 tie). G2 runs the
+// reduction on four lanes per point, and L = 4 there too (2^18, c = 13: reduce 0.46 -> 0.35 ms
+// against L = 2, which put a second wave on some SIMDs; precomputed 2^18 at c = 16: 1.05 -> 0.65
+// ms).
 uint32_t msm_reduce_chunk(int group, uint32_t c, uint32_t windows) {
-    return group == 2 && ((uint64_t)windows << (c - 1)) < (1u << 18) ? 2 : 4;
+    (void)group, (void)c, (void)windows;
+    return 4;
 }
 
 uint32_t msm_part_bits(const MsmShape& sh, uint32_t* part_log) {

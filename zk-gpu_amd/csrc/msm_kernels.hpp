@@ -595,6 +595,10 @@ template <>
 struct ReduceWaves<Fq2P> {
     static constexpr int value = 2;
 };
+template <>
+struct ReduceWaves<Fq2Q> {
+    static constexpr int value = 2;
+};
 
 // Lane t of window w takes buckets b0 .. b0+L-1 (b0 = j*L) and computes
 // sum_k (k+1) bucket[b0+k] + b0 * sum_k bucket[b0+k]  =  sum_k (b0+k+1) bucket[b0+k];
@@ -755,6 +759,18 @@ struct LaneOps<Fq2P> {
     static ZKG_DEV Fq2P gather(const Fq2P& m, int k) {
         return Fq2P{LaneOps<Fq29>::shfl(m.v, 2 * k + (int)(__lane_id() & 1u))};
     }
+};
+
+template <>
+struct LaneOps<Fq29Pair> {
+    // lane shuffles between pairs (sources at the same position of their pair)
+    static ZKG_DEV Fq29Pair shfl(const Fq29Pair& x, int src) { return Fq29Pair{LaneOps<Fq29>::shfl(x.v, src)}; }
+};
+
+template <>
+struct LaneOps<Fq2Q> {
+    // lane shuffles between groups of four lanes (sources at the same position of their group)
+    static ZKG_DEV Fq2Q shfl(const Fq2Q& x, int src) { return Fq2Q{LaneOps<Fq2P>::shfl(x.v, src)}; }
 };
 
 template <>
@@ -940,6 +956,63 @@ ZKG_DEV Xyzz29 xyzz29_dbl_lanes(const Xyzz29& p, uint32_t lane) {
     return Xyzz29{x3, y3, zz3, f29_shfl(m, 2)};
 }
 
+// The doubling chain with fixed lane roles: between doublings lane 0 holds Y, lane 1 X and lane 2
+// ZZ and ZZZ (other fields, and lanes 3..63, hold values no one reads), and each product's lane
+// fetches only its own operands, in four rounds of lane shuffles per doubling instead of the
+// nine full broadcasts of xyzz29_dbl_lanes (the LDS round trips were a third of the chain's
+// cycles). The formulas, their order and hence the bounds are those of xyzz29_dbl_lanes:
+//   L1  lane 0: V = U^2 (U = 2Y)   lane 1: X^2
+//       round 1: lanes 0, 3 <- X^2 (lane 1); lanes 1, 2 <- V (lane 0)
+//   L2  lane 0: W = U V   lane 1: S = X V   lane 2: ZZ3 = V ZZ   lane 3: M^2 (M = 3 X^2)
+//       round 2: lane 0 <- S (lane 1); lanes 1, 2 <- W (lane 0)
+//       round 3: lane 0 <- M^2 (lane 3); lane 1 <- Y (lane 0)
+//   lane 0: X3 = M^2 - 2S, D = S - X3
+//   L3  lane 0: M D   lane 1: W Y   lane 2: ZZZ3 = W ZZZ
+//       round 4: lane 0 <- W Y (lane 1), Y3 = M D - W Y; lane 1 <- X3 (lane 0)
+struct DblRoles {
+    uint32_t lane;
+    int src1, src2, src3, src4;  // the lane each round reads from
+    explicit ZKG_DEV DblRoles(uint32_t l) : lane(l) {
+        src1 = (l == 1 || l == 2) ? 0 : 1;
+        src2 = l == 0 ? 1 : 0;
+        src3 = l == 0 ? 3 : 0;
+        src4 = l == 0 ? 1 : 0;
+    }
+};
+
+ZKG_DEV Xyzz29 xyzz29_dbl_roles(const Xyzz29& p, const DblRoles& ro) {
+    const uint32_t lane = ro.lane;
+    const F29 u = f29_add(p.y, p.y);                        // lane 0: < 10, limbs < 2^30
+    const F29 o1 = f29_sel(lane == 0, u, p.x);
+    const F29 m1 = f29_mul(o1, o1);                         // lane 0: V, lane 1: X^2 (10 * 10, 9 * 9) -> < 2
+    const F29 r1 = f29_shfl(m1, ro.src1);                   // lanes 0, 3: X^2; lanes 1, 2: V
+    const F29 mm = f29_carry(f29_add(f29_add(r1, r1), r1));  // lanes 0, 3: M = 3 X^2 < 6, normalised
+    const F29 a2 = f29_pick(lane, u, p.x, p.zz, mm);
+    const F29 b2 = f29_pick(lane, m1, r1, r1, mm);
+    const F29 m2 = f29_mul(a2, b2);                         // W, S, ZZ3, M^2 (10 * 2, 9 * 2, 2 * 2, 6 * 6) -> < 2
+    const F29 r2 = f29_shfl(m2, ro.src2);                   // lane 0: S; lanes 1, 2: W
+    const F29 q3 = f29_sel(lane == 3, m2, p.y);
+    const F29 r3 = f29_shfl(q3, ro.src3);                   // lane 0: M^2; lane 1: Y
+    const F29 x3 = f29_sub<6>(r3, f29_add(r2, r2));         // lane 0: 2S < 4 < 5: X3 < 8
+    const F29 d = f29_sub<10>(r2, x3);                      // lane 0: < 12
+    const F29 a3 = f29_sel(lane == 0, mm, r2);
+    const F29 b3 = f29_pick(lane, d, r3, p.zzz);
+    const F29 m3 = f29_mul(a3, b3);                         // M D, W Y, ZZZ3 (6 * 12, 2 * 5, 2 * 2) -> < 2
+    const F29 q4 = f29_sel(lane == 0, x3, m3);
+    const F29 r4 = f29_shfl(q4, ro.src4);                   // lane 0: W Y; lane 1: X3
+    Xyzz29 o;
+    o.y = f29_sub<3>(m3, r4);                               // lane 0: Y3 = M D - W Y < 5
+    o.x = r4;                                               // lane 1: X3
+    o.zz = m2;                                              // lane 2: ZZ3
+    o.zzz = m3;                                             // lane 2: ZZZ3
+    return o;
+}
+
+// The whole point on every lane from the roles' lanes (Y: lane 0, X: lane 1, ZZ, ZZZ: lane 2).
+ZKG_DEV Xyzz29 xyzz29_from_roles(const Xyzz29& p) {
+    return Xyzz29{f29_shfl(p.x, 1), f29_shfl(p.y, 0), f29_shfl(p.zz, 2), f29_shfl(p.zzz, 2)};
+}
+
 // P + Q (add-2008-s) with every special case; P finite (bounds as above), Q finite with all
 // coordinates below 2. *inf is set when the sum is infinity.
 ZKG_DEV Xyzz29 xyzz29_add_lanes(const Xyzz29& p, const Xyzz29& q, uint32_t lane, bool* inf) {
@@ -991,10 +1064,13 @@ __global__ __launch_bounds__(64) void msm_combine29_kernel(const uint32_t* __res
     };
     bool inf;
     Xyzz29 acc = load(W - 1, &inf);
+    const DblRoles roles(lane);
     for (int w = (int)W - 2; w >= 0; --w) {
         const uint32_t width = wb + ((uint32_t)w < nw ? 1u : 0u);  // bits between window w and w + 1
-        if (!inf)
-            for (uint32_t k = 0; k < width; ++k) acc = xyzz29_dbl_lanes(acc, lane);  // no 2-torsion
+        if (!inf) {  // no 2-torsion: a finite point stays finite
+            for (uint32_t k = 0; k < width; ++k) acc = xyzz29_dbl_roles(acc, roles);
+            acc = xyzz29_from_roles(acc);
+        }
         bool q_inf;
         const Xyzz29 q = load(w, &q_inf);
         if (q_inf) continue;
@@ -1072,8 +1148,9 @@ struct TailField<Fq2_29> {
     using type = Fq2P;
 };
 
-// The field of the bucket reduction and window sums: the tail field, except that G1 (Fq29) runs
-// them on lane pairs (Fq29Pair: each point operation's products split between the two lanes).
+// The field of the bucket reduction and window sums: G1 (Fq29) runs them on lane pairs (Fq29Pair:
+// each point operation's products split between the two lanes), G2 on groups of four lanes (Fq2Q:
+// split between two lane pairs that each hold one Fq2 element).
 // Their serial chains of additions otherwise leave most SIMDs idle: the G1 2^20 reduction had one
 // wave per two SIMDs. Measured on MI355X at G1 2^20, c = 15 (kernel trace): 0.296 ms on one lane
 // per chain, 0.279 ms on pairs over the full bucket range, 0.20 ms with the grid below.
@@ -1085,6 +1162,10 @@ template <>
 struct ReduceField<Fq29> {
     using type = Fq29Pair;
 };
+template <>
+struct ReduceField<Fq2_29> {
+    using type = Fq2Q;
+};
 
 template <class Fe>
 void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
@@ -1095,6 +1176,17 @@ void launch_merge(const MsmPlan& p, uint32_t stride, hipStream_t s) {
                        p.item_info, p.istarts, p.W * p.B, stride, (const uint32_t*)p.max_items_dev);
 }
 
+// The field of the finalize: the tail field, except G2 on four lanes per point (Fq2Q; plain G2
+// 2^18: 0.215 -> 0.18 ms). G1 on lane pairs measured slower (2^20: 0.12 -> 0.15 ms).
+template <class Fe>
+struct FinalizeField {
+    using type = typename TailField<Fe>::type;
+};
+template <>
+struct FinalizeField<Fq2_29> {
+    using type = Fq2Q;
+};
+
 template <class Fe>
 void launch_finalize(const MsmPlan& p, hipStream_t s) {
     const uint32_t S = p.W * p.B;
@@ -1102,9 +1194,12 @@ void launch_finalize(const MsmPlan& p, hipStream_t s) {
     // MI355X: G1 2^20 (147456 buckets, two lanes) 0.20 -> 0.16 ms, G2 2^20 0.55 -> 0.47 ms; G2 2^16
     // (12288 buckets) 0.39 -> 0.27 ms with two lanes, 0.14 ms with four; at 2^18 buckets and more
     // one lane each is faster (G1 2^22 0.18 against 0.20, G2 0.62 against 0.80 with two).
-    const uint32_t split = S < (1u << 15) ? 2u : S < (1u << 18) ? 1u : 0u;
-    using Ft = typename TailField<Fe>::type;
+    // G2 runs on four lanes per point (Fq2Q): its thresholds are halved (precomputed 2^18 at c = 16,
+    // 131072 buckets: 0.44 ms on pairs, 0.47 with two parts of four lanes per bucket)
+    using Ft = typename FinalizeField<Fe>::type;
     constexpr uint32_t LN = FeLanes<Ft>::value;
+    constexpr uint32_t hs = LN >= 4 ? 1u : 0u;
+    const uint32_t split = S < (1u << (15 - hs)) ? 2u : S < (1u << (18 - hs)) ? 1u : 0u;
     hipLaunchKernelGGL(msm_finalize_kernel<Ft>, dim3(ceil_div(((size_t)S << split) * LN, 256)), dim3(256), 0, s,
                        p.item_pts, p.istarts, (const uint2*)p.item_info, S, split, p.buckets);
 }
