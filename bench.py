@@ -649,7 +649,8 @@ def main():
     m2 = Msm.for_group(gpu, Group.G2, n2, Msm.default_window(n2))
     g2_affine = affine_words(gpu.read(m2.run(gpu, bb2, sb2), 64), 2)
     g2_ms, _ = time_calls(gpu, ev, lambda: m2.run(gpu, bb2, sb2), max(3, args.steps // 4), 1)
-    log(f"G2 2^18: {g2_ms:.3f} ms")
+    stages2 = m2.profile(gpu, bb2, sb2)
+    log(f"G2 2^18: {g2_ms:.3f} ms; stages {stages2}")
 
     # ---------------- the same MSMs over precomputed bases (P_i and 2^64 P_i, built once per key)
     precomputed = {}
@@ -717,6 +718,12 @@ def main():
     _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 2, threads, chain, 2, ctypes.byref(ms)))
     _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 2, threads, chain, 2, ctypes.byref(ms)))
     mul29_gps = threads * chain * 2 / (ms.value * 1e-3) / 1e9
+    # Fermat inversions in the same limbs (362 products each), checked x * x^-1 = 1 on the device
+    inv_chain = 8
+    _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 3, threads, inv_chain, 1, ctypes.byref(ms)))
+    _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 3, threads, inv_chain, 1, ctypes.byref(ms)))
+    inv29_ms = ms.value
+    inv29_mps = threads * inv_chain / (inv29_ms * 1e-3) / 1e6
 
     # Algorithmic Fq products of the G1 MSM 2^20 (madd 10, add 14, dbl 9 products per operation):
     # one mixed addition per nonzero signed digit of the actual GLV half-scalars, the bucket
@@ -757,7 +764,8 @@ def main():
     if args.workload == "msm":
         print(json.dumps({"g1_msm_2^20_ms": round(g1_ms, 4), "g1_msm_2^20_tuned_ms": round(g1_tuned_ms, 4),
                           "g2_msm_2^18_ms": round(g2_ms, 4),
-                          "stages": {k: round(v, 4) for k, v in stages}}), flush=True)
+                          "stages": {k: round(v, 4) for k, v in stages},
+                          "g2_stages": {k: round(v, 4) for k, v in stages2}}), flush=True)
         return
     if not args.no_cpu:
         log("CPU baseline (oracle) on the same inputs")
@@ -788,6 +796,7 @@ def main():
             "g1_msm_2^20_tuned": {"ms": round(g1_tuned_ms, 4), "c": tuned1["c"], "item_points": tuned1["item_points"],
                                   "sort_points": tuned1["sort_points"]},
             "g2_msm_2^18_ms": round(g2_ms, 4),
+            "g2_msm_2^18_stages_ms": {k: round(v, 4) for k, v in stages2},
             "precomputed_bases": precomputed,
             "ntt": {f"2^{k}": v for k, v in ntt.items()},
             "ntt_2^24_hbm_GBps": n24["hbm_GBps"],
@@ -797,6 +806,9 @@ def main():
             "serving": serve,
             "field_mul_Gmulmod_per_s": round(mulmod_gps, 1),
             "field_mul29_Gmulmod_per_s": round(mul29_gps, 1),
+            "field_inv29": {"M_inversions_per_s": round(inv29_mps, 2),
+                            "products_per_inversion": round(mul29_gps * 1e3 / inv29_mps, 1),
+                            "method": "Fermat x^(p-2), 253 squarings + 109 products, checked x * x^-1 = 1"},
             "g1_msm_2^20_roofline": msm_roof,
             "device": info,
         },

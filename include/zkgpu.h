@@ -374,7 +374,9 @@ int zkg_test_point_op(zkg_device* dev, int group, int op, const uint32_t* p, con
                       size_t count);
 /* Field-multiply throughput: `threads` lanes each run `chain` dependent Montgomery products,
  * `ways` independent chains per lane. field 0 = Fq and 1 = Fr in 8 x 32-bit limbs (CIOS), 2 = Fq in
- * the MSM's 9 x 29-bit limbs (product scanning, R = 2^261). Reports kernel milliseconds (HIP events). */
+ * the MSM's 9 x 29-bit limbs (product scanning, R = 2^261); field 3 = Fq inversions in those limbs
+ * (Fermat, 362 products each; `chain` dependent inversions per lane, `ways` ignored, each checked:
+ * a failed x * x^-1 = 1 returns an error). Reports kernel milliseconds (HIP events). */
 int zkg_bench_field_mul(zkg_device* dev, int field, uint32_t threads, uint32_t chain, uint32_t ways, double* ms);
 /* A plain device-to-device copy kernel of `bytes` (16 per lane), run `reps` times; reports the
  * mean milliseconds per launch. The known-size reference for calibrating HBM byte counters. */

@@ -99,3 +99,13 @@ def test_generator_multiples(gpu, group):
     ob = gpu.empty_buffer(16 * group * n)
     _lib.check(_lib.load().zkg_util_generator_multiples(gpu.handle, group, kb.handle, n, ob.handle))
     assert np.array_equal(gpu.read(ob, 16 * group * n), O.multiples(group, O.generator_words(group), ks))
+
+
+def test_inversion_benchmark_checks_its_inverses(gpu):
+    """The Fermat-inversion mode of the field benchmark hook (29-bit limbs, x^(p-2)) verifies
+    x * x^-1 = 1 for every inversion it times and fails the call otherwise."""
+    import ctypes
+
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().zkg_bench_field_mul(gpu.handle, 3, 256 * 8, 3, 1, ctypes.byref(ms)))
+    assert ms.value > 0

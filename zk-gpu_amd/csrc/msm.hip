@@ -469,18 +469,50 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_place_kernel(const uint32_t*
 // 716 MB written to HBM for the 75 MB of entries of a G1 2^20 MSM (c = 15), 0.28 ms. Two passes
 // keep the writes local: (1) each block writes its points as runs per partition (2^plog
 // partitions per window, each a range of 2^sh buckets of the window's own bucket range) of a
-// temporary array, one run per (block, partition), reserved with one global atomic; (2) one
-// workgroup per (window, partition) counts the partition's buckets and places its entries, so
-// its scattered writes stay within a 32 KB range of the final list (at G1 2^20) that one CU
-// writes, small enough for the L2 to merge. A packed entry holds the point index (31 - kb bits),
-// the bucket within its partition (the kb bits below the sign; sh <= kb) and the sign (bit 31).
-// Loops keep four entries per lane in flight. The order of entries within a bucket depends on
-// the atomics, as in the one-pass placement; sums do not.
+// temporary array, one run per (block, partition), reserved with one global atomic; (2) each
+// partition is cut into slices of SLICE entries, and one workgroup per slice counts its buckets
+// and places its entries, its run of each bucket reserved with one global atomic, so the
+// scattered writes of a workgroup stay within a 32 KB range of the final list (at G1 2^20) that
+// one CU writes, small enough for the L2 to merge. Slices keep the work of a workgroup bounded
+// when a few buckets hold most entries (witness-like scalars: a fifth of a window in one bucket).
+// A packed entry holds the point index (31 - kb bits), the bucket within its partition (the kb
+// bits below the sign; sh <= kb) and the sign (bit 31). Loops keep four entries per lane in
+// flight. The order of entries within a bucket depends on the atomics, as in the one-pass
+// placement; sums do not.
 // The partition tables (in the plan's `table`): bcount[w][blk][p] (a block's entries per
-// partition), pbase[w][p] (exclusive scan of the partition totals, pbase[M] = all entries) and
-// pcur[w][p] (the next free run of each partition), M = W 2^plog.
+// partition), pbase[q] (q = w 2^plog + p: exclusive scan of the partition totals, pbase[M] = all
+// entries), pcur[q] (the next free run of each partition) and sbase[q] (exclusive scan of the
+// slices per partition, sbase[M] = all slices), M = W 2^plog. counts and the plan's `cursors`
+// (the next free position of each bucket after its start) are zeroed before.
 static constexpr uint32_t PART_MAX = 256;  // partitions per window, and buckets per partition, at most
 static constexpr uint32_t BUCKET_THREADS = 256;
+static constexpr uint32_t SLICE = 8192;  // entries per workgroup of the bucket pass
+
+// LDS counters aggregated over the lanes of a wave that share a key: lanes whose key is `hot` (a
+// block-uniform guess of the most frequent key) or, else, the key of the wave's first active lane
+// use one atomic for all of them (ranks by popcount). Inputs with a hot bucket would otherwise
+// serialise one LDS atomic per lane on a single address. Other lanes use their own atomics.
+ZKG_DEV uint32_t lds_fetch_inc(uint32_t* cnt, uint32_t key, uint32_t hot) {
+    uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+    if (__ballot(key == hot)) k0 = hot;
+    const uint64_t same = __ballot(key == k0);
+    if (key != k0) return atomicAdd(&cnt[key], 1u);
+    const uint32_t lane = __lane_id(), first = (uint32_t)__builtin_ctzll(same);
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(&cnt[k0], (uint32_t)__builtin_popcountll(same));
+    base = __shfl(base, (int)first);
+    return base + (uint32_t)__builtin_popcountll(same & ((1ull << lane) - 1));
+}
+
+ZKG_DEV void lds_inc(uint32_t* cnt, uint32_t key) {
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+    const uint64_t same = __ballot(key == k0);
+    if (key != k0) {
+        atomicAdd(&cnt[key], 1u);
+        return;
+    }
+    if (__lane_id() == (uint32_t)__builtin_ctzll(same)) atomicAdd(&cnt[k0], (uint32_t)__builtin_popcountll(same));
+}
 
 // Bucket bits of partition ranges in window w: the window's bucket range 2^cw (cw = its digit
 // width - 1, at most cb) over 2^plog partitions.
@@ -507,7 +539,7 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_part_hist_kernel(const uint3
         for (int k = 0; k < 4; ++k) e[k] = i + k * SORT_THREADS < hi ? d[i + k * SORT_THREADS] : NO_DIGIT;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (e[k] != NO_DIGIT) atomicAdd(&hist[(e[k] & 0x7fffffffu) >> sh], 1u);
+            if (e[k] != NO_DIGIT) lds_inc(hist, (e[k] & 0x7fffffffu) >> sh);
     }
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < NP; p += SORT_THREADS) {
@@ -518,52 +550,74 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_part_hist_kernel(const uint3
 }
 
 // One workgroup: pbase[0 .. M) = exclusive scan of pbase (the partition totals) in place,
-// pbase[M] = the total, pcur = pbase. Four consecutive entries per lane per round of 4096.
+// pbase[M] = the total, pcur = pbase; sbase[0 .. M] = exclusive scan of ceil(total / SLICE). Four
+// consecutive entries per lane per round of 4096.
 __global__ __launch_bounds__(1024) void msm_part_scan_kernel(uint32_t* __restrict__ pbase, uint32_t M,
-                                                            uint32_t* __restrict__ pcur) {
-    __shared__ uint32_t wsum[16];
+                                                            uint32_t* __restrict__ pcur,
+                                                            uint32_t* __restrict__ sbase) {
+    __shared__ uint32_t wsum[2][16];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint32_t carry = 0;
+    uint32_t carry = 0, scarry = 0;
     for (uint32_t base = 0; base < M; base += 4096) {
-        uint32_t v[4], sum = 0;
+        uint32_t v[4], sum = 0, ssum = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t idx = base + 4 * threadIdx.x + k;
             v[k] = idx < M ? pbase[idx] : 0u;
             sum += v[k];
+            ssum += (v[k] + SLICE - 1) / SLICE;
         }
-        uint32_t x = sum;  // inclusive scan over the wave
+        uint32_t x = sum, y = ssum;  // inclusive scans over the wave
 #pragma unroll
         for (uint32_t off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off);
-            if (lane >= off) x += y;
+            const uint32_t a = __shfl_up(x, off), b = __shfl_up(y, off);
+            if (lane >= off) {
+                x += a;
+                y += b;
+            }
         }
-        if (lane == 63) wsum[wv] = x;
+        if (lane == 63) {
+            wsum[0][wv] = x;
+            wsum[1][wv] = y;
+        }
         __syncthreads();
-        if (wv == 0) {  // inclusive scan of the 16 wave totals
-            uint32_t t = lane < 16 ? wsum[lane] : 0u;
+        if (wv == 0) {  // inclusive scans of the 16 wave totals
+            uint32_t t = lane < 16 ? wsum[0][lane] : 0u, u = lane < 16 ? wsum[1][lane] : 0u;
 #pragma unroll
             for (uint32_t off = 1; off < 16; off <<= 1) {
-                const uint32_t y = __shfl_up(t, off);
-                if (lane >= off) t += y;
+                const uint32_t a = __shfl_up(t, off), b = __shfl_up(u, off);
+                if (lane >= off) {
+                    t += a;
+                    u += b;
+                }
             }
-            if (lane < 16) wsum[lane] = t;
+            if (lane < 16) {
+                wsum[0][lane] = t;
+                wsum[1][lane] = u;
+            }
         }
         __syncthreads();
-        uint32_t run = carry + (wv ? wsum[wv - 1] : 0u) + x - sum;
+        uint32_t run = carry + (wv ? wsum[0][wv - 1] : 0u) + x - sum;
+        uint32_t srun = scarry + (wv ? wsum[1][wv - 1] : 0u) + y - ssum;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t idx = base + 4 * threadIdx.x + k;
             if (idx < M) {
                 pbase[idx] = run;
                 pcur[idx] = run;
+                sbase[idx] = srun;
             }
             run += v[k];
+            srun += (v[k] + SLICE - 1) / SLICE;
         }
-        carry += wsum[15];
+        carry += wsum[0][15];
+        scarry += wsum[1][15];
         __syncthreads();  // wsum is rewritten next round
     }
-    if (threadIdx.x == 0) pbase[M] = carry;
+    if (threadIdx.x == 0) {
+        pbase[M] = carry;
+        sbase[M] = scarry;
+    }
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void msm_part_place_kernel(const uint32_t* __restrict__ digits, uint32_t n,
@@ -573,12 +627,17 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_part_place_kernel(const uint
                                                                      uint32_t* __restrict__ pcur,
                                                                      uint32_t* __restrict__ temp) {
     __shared__ uint32_t cursor[PART_MAX];
+    __shared__ uint32_t hot_s;
     const uint32_t blk = blockIdx.x, w = blockIdx.y, NP = 1u << plog, sh = part_shift(w, wb, nw, cb, plog);
+    if (threadIdx.x == 0) hot_s = 0;
+    __syncthreads();
     for (uint32_t p = threadIdx.x; p < NP; p += SORT_THREADS) {  // reserve this block's run of each partition
         const uint32_t h = bcount[((size_t)w * nblk + blk) * NP + p];
         cursor[p] = h ? atomicAdd(&pcur[w * NP + p], h) : 0u;
+        if (h) atomicMax(&hot_s, (h << 8) | p);  // the block's fullest partition (h < 2^24)
     }
     __syncthreads();
+    const uint32_t hot = hot_s & 0xffu;
     const uint32_t lo = blk * P, hi = min(lo + P, n), bmask = (1u << sh) - 1;
     const uint32_t* d = digits + (size_t)w * n;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * SORT_THREADS) {
@@ -587,7 +646,7 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_part_place_kernel(const uint
         for (int k = 0; k < 4; ++k) e[k] = i + k * SORT_THREADS < hi ? d[i + k * SORT_THREADS] : NO_DIGIT;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (e[k] != NO_DIGIT) pos[k] = atomicAdd(&cursor[(e[k] & 0x7fffffffu) >> sh], 1u);
+            if (e[k] != NO_DIGIT) pos[k] = lds_fetch_inc(cursor, (e[k] & 0x7fffffffu) >> sh, hot);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (e[k] != NO_DIGIT)
@@ -595,59 +654,104 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_part_place_kernel(const uint
     }
 }
 
-// Points per bucket of partition blockIdx.x of window blockIdx.y -> counts; the workgroup also
-// zeroes its share of the window's buckets beyond its digit range (which no partition covers).
+// The slice of workgroup blockIdx.x: its partition q (binary search of sbase) and entry range.
+struct PartSlice {
+    uint32_t q, lo, hi;
+    bool valid;
+};
+ZKG_DEV PartSlice part_slice(const uint32_t* __restrict__ pbase, const uint32_t* __restrict__ sbase, uint32_t M) {
+    PartSlice r;
+    const uint32_t g = blockIdx.x;
+    r.valid = g < sbase[M];
+    r.q = 0;
+    r.lo = r.hi = 0;
+    if (!r.valid) return r;
+    uint32_t lo = 0, hi = M;  // last q with sbase[q] <= g (partitions with no slice share their successor's sbase)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sbase[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    r.q = lo;
+    r.lo = pbase[lo] + (g - sbase[lo]) * SLICE;
+    r.hi = min(r.lo + SLICE, pbase[lo + 1]);
+    return r;
+}
+
+// Points per bucket of a slice, added into counts (zeroed beforehand).
 __global__ __launch_bounds__(BUCKET_THREADS) void msm_bucket_count_kernel(const uint32_t* __restrict__ temp,
                                                                          const uint32_t* __restrict__ pbase,
+                                                                         const uint32_t* __restrict__ sbase, uint32_t M,
                                                                          uint32_t wb, uint32_t nw, uint32_t cb,
                                                                          uint32_t plog, uint32_t kb, uint32_t B,
                                                                          uint32_t* __restrict__ counts) {
     __shared__ uint32_t hist[PART_MAX];
-    const uint32_t p = blockIdx.x, w = blockIdx.y, NP = 1u << plog, sh = part_shift(w, wb, nw, cb, plog);
+    const PartSlice sl = part_slice(pbase, sbase, M);
+    if (!sl.valid) return;
+    const uint32_t w = sl.q >> plog, p = sl.q & ((1u << plog) - 1), sh = part_shift(w, wb, nw, cb, plog);
     const uint32_t nb = 1u << sh, bmask = nb - 1;
-    const uint32_t lo = pbase[w * NP + p], hi = pbase[w * NP + p + 1];
     for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS) hist[b] = 0;
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * BUCKET_THREADS) {
+    for (uint32_t i = sl.lo + threadIdx.x; i < sl.hi; i += 4 * BUCKET_THREADS) {
         uint32_t x[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) x[k] = i + k * BUCKET_THREADS < hi ? temp[i + k * BUCKET_THREADS] : 0u;
+        for (int k = 0; k < 4; ++k) x[k] = i + k * BUCKET_THREADS < sl.hi ? temp[i + k * BUCKET_THREADS] : 0u;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (i + k * BUCKET_THREADS < hi) atomicAdd(&hist[(x[k] >> (31 - kb)) & bmask], 1u);
+            if (i + k * BUCKET_THREADS < sl.hi) lds_inc(hist, (x[k] >> (31 - kb)) & bmask);
     }
     __syncthreads();
-    uint32_t* cw = counts + (size_t)w * B;
-    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS) cw[((size_t)p << sh) + b] = hist[b];
-    const uint32_t covered = NP << sh, share = (B - covered) >> plog;  // uncovered buckets per partition
-    for (uint32_t b = threadIdx.x; b < share; b += BUCKET_THREADS) cw[covered + p * share + b] = 0;
+    uint32_t* cw = counts + (size_t)w * B + ((size_t)p << sh);
+    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS)
+        if (hist[b]) atomicAdd(&cw[b], hist[b]);
 }
 
-// Places partition blockIdx.x of window blockIdx.y: entries go to their buckets' cursors (from the
-// scan's starts) as point index | sign << 31.
+// Places a slice: its run of each bucket is reserved with one global atomic on the bucket's
+// cursor (positions after the bucket's start), then entries go there as point index | sign << 31.
 __global__ __launch_bounds__(BUCKET_THREADS) void msm_bucket_place_kernel(const uint32_t* __restrict__ temp,
                                                                          const uint32_t* __restrict__ pbase,
+                                                                         const uint32_t* __restrict__ sbase, uint32_t M,
                                                                          uint32_t wb, uint32_t nw, uint32_t cb,
                                                                          uint32_t plog, uint32_t kb, uint32_t B,
                                                                          const uint32_t* __restrict__ starts,
+                                                                         uint32_t* __restrict__ bcur,
                                                                          uint32_t* __restrict__ sorted) {
     __shared__ uint32_t cursor[PART_MAX];
-    const uint32_t p = blockIdx.x, w = blockIdx.y, NP = 1u << plog, sh = part_shift(w, wb, nw, cb, plog);
+    __shared__ uint32_t hot_s;
+    const PartSlice sl = part_slice(pbase, sbase, M);
+    if (!sl.valid) return;
+    const uint32_t w = sl.q >> plog, p = sl.q & ((1u << plog) - 1), sh = part_shift(w, wb, nw, cb, plog);
     const uint32_t nb = 1u << sh, bmask = nb - 1, fmask = ((1u << kb) - 1) << (31 - kb);
-    const uint32_t lo = pbase[w * NP + p], hi = pbase[w * NP + p + 1];
-    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS)
-        cursor[b] = starts[(size_t)w * B + ((size_t)p << sh) + b];
+    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS) cursor[b] = 0;
+    if (threadIdx.x == 0) hot_s = 0;
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * BUCKET_THREADS) {
+    for (uint32_t i = sl.lo + threadIdx.x; i < sl.hi; i += 4 * BUCKET_THREADS) {  // the slice's counts
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = i + k * BUCKET_THREADS < sl.hi ? temp[i + k * BUCKET_THREADS] : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * BUCKET_THREADS < sl.hi) lds_inc(cursor, (x[k] >> (31 - kb)) & bmask);
+    }
+    __syncthreads();
+    const size_t b0 = (size_t)w * B + ((size_t)p << sh);
+    for (uint32_t b = threadIdx.x; b < nb; b += BUCKET_THREADS) {
+        const uint32_t h = cursor[b];
+        if (h) atomicMax(&hot_s, (h << 8) | b);  // the slice's fullest bucket (h <= SLICE)
+        cursor[b] = h ? starts[b0 + b] + atomicAdd(&bcur[b0 + b], h) : 0u;
+    }
+    __syncthreads();
+    const uint32_t hot = hot_s & 0xffu;
+    for (uint32_t i = sl.lo + threadIdx.x; i < sl.hi; i += 4 * BUCKET_THREADS) {
         uint32_t x[4], pos[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) x[k] = i + k * BUCKET_THREADS < hi ? temp[i + k * BUCKET_THREADS] : 0u;
+        for (int k = 0; k < 4; ++k) x[k] = i + k * BUCKET_THREADS < sl.hi ? temp[i + k * BUCKET_THREADS] : 0u;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (i + k * BUCKET_THREADS < hi) pos[k] = atomicAdd(&cursor[(x[k] >> (31 - kb)) & bmask], 1u);
+            if (i + k * BUCKET_THREADS < sl.hi) pos[k] = lds_fetch_inc(cursor, (x[k] >> (31 - kb)) & bmask, hot);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (i + k * BUCKET_THREADS < hi) sorted[pos[k]] = x[k] & ~fmask;
+            if (i + k * BUCKET_THREADS < sl.hi) sorted[pos[k]] = x[k] & ~fmask;
     }
 }
 
@@ -1062,9 +1166,9 @@ uint32_t msm_part_bits(const MsmShape& sh, uint32_t* part_log) {
     const uint32_t kb = cb - plog;  // bucket bits within a partition
     if ((uint64_t)sh.nv > (1ull << (31 - kb))) return 0;  // the point index needs 31 - kb bits
     if ((1u << plog) > PART_MAX || (1u << kb) > PART_MAX) return 0;
-    // bcount, pbase (M + 1) and pcur must fit the plan's table of W nblk B words
+    // bcount, pbase (M + 1), pcur (M) and sbase (M + 1) must fit the plan's table of W nblk B words
     const uint64_t M = (uint64_t)sh.W << plog;
-    if (M * sh.nblk + 2 * M + 1 > (uint64_t)sh.W * sh.nblk * sh.B) return 0;
+    if (M * sh.nblk + 3 * M + 2 > (uint64_t)sh.W * sh.nblk * sh.B) return 0;
     *part_log = plog;
     return kb;
 }
@@ -1173,15 +1277,19 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
             uint32_t* bcount = p.table;
             uint32_t* pbase = bcount + (size_t)M * p.nblk;
             uint32_t* pcur = pbase + M + 1;
+            uint32_t* sbase = pcur + M;
+            const unsigned slices = ceil_div((size_t)p.W * p.nv, SLICE) + M;  // at most
             fill_u32(pbase, 0, M, s);
+            fill_u32(p.counts, 0, S, s);
+            fill_u32(p.cursors, 0, S, s);
             hipLaunchKernelGGL(msm_part_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 0, s, dig, p.nv,
                                p.sort_pts, p.nblk, p.wb, p.nw, cb, plog, bcount, pbase);
-            hipLaunchKernelGGL(msm_part_scan_kernel, dim3(1), dim3(1024), 0, s, pbase, M, pcur);
+            hipLaunchKernelGGL(msm_part_scan_kernel, dim3(1), dim3(1024), 0, s, pbase, M, pcur, sbase);
             hipLaunchKernelGGL(msm_part_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 0, s, dig, p.nv,
                                p.sort_pts, p.nblk, p.wb, p.nw, cb, plog, kb, (const uint32_t*)bcount, pcur, p.digits);
-            hipLaunchKernelGGL(msm_bucket_count_kernel, dim3(NP, p.W), dim3(BUCKET_THREADS), 0, s,
-                               (const uint32_t*)p.digits, (const uint32_t*)pbase, p.wb, p.nw, cb, plog, kb, p.B,
-                               p.counts);
+            hipLaunchKernelGGL(msm_bucket_count_kernel, dim3(slices), dim3(BUCKET_THREADS), 0, s,
+                               (const uint32_t*)p.digits, (const uint32_t*)pbase, (const uint32_t*)sbase, M, p.wb,
+                               p.nw, cb, plog, kb, p.B, p.counts);
         } else {
             hipLaunchKernelGGL(msm_hist_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, dig, p.nv, p.B,
                                p.sort_pts, p.nblk, p.table);
@@ -1205,10 +1313,13 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
     hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, s, p.scan_tmp, S, p.starts, p.istarts);
     stage("scan");
     if (p.lds_sort && kb) {
-        const uint32_t* pbase = p.table + (size_t)p.W * NP * p.nblk;
-        hipLaunchKernelGGL(msm_bucket_place_kernel, dim3(NP, p.W), dim3(BUCKET_THREADS), 0, s,
-                           (const uint32_t*)p.digits, pbase, p.wb, p.nw, cb, plog, kb, p.B,
-                           (const uint32_t*)p.starts, p.sorted);
+        const uint32_t M = p.W * NP;
+        const uint32_t* pbase = p.table + (size_t)M * p.nblk;
+        const uint32_t* sbase = pbase + 2 * M + 1;
+        const unsigned slices = ceil_div((size_t)p.W * p.nv, SLICE) + M;
+        hipLaunchKernelGGL(msm_bucket_place_kernel, dim3(slices), dim3(BUCKET_THREADS), 0, s,
+                           (const uint32_t*)p.digits, pbase, sbase, M, p.wb, p.nw, cb, plog, kb, p.B,
+                           (const uint32_t*)p.starts, p.cursors, p.sorted);
     } else if (p.lds_sort) {
         hipLaunchKernelGGL(msm_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.nv, p.B,
                            p.sort_pts, p.nblk, p.table, p.starts, p.sorted);
