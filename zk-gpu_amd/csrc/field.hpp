@@ -20,16 +20,6 @@ namespace zkg {
 #define ZKG_DEV __device__ __forceinline__
 #define ZKG_HD __host__ __device__ __forceinline__
 
-// Low 32 bits of a * c (c a constant) on the full-rate v_mad_u64_u32. For a product whose high
-// half is unused the compiler selects v_mul_lo_u32, which issues at a quarter of the VALU rate;
-// the Montgomery quotient digit of every field product is such a product. The multiply-add's
-// carry-out goes to VCC, declared clobbered.
-ZKG_DEV uint32_t mul_lo_full_rate(uint32_t a, uint32_t c) {
-    uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(r) : "v"(a), "s"(c) : "vcc");
-    return (uint32_t)r;
-}
-
 // Field parameters as constexpr word functions, so that unrolled loops fold them into literals.
 struct FqParams {  // p = 0x30644e72e131a029b85045b68181585d97816a916871ca8d3c208c16d87cfd47
     static constexpr uint32_t INV = 0xe4866389u;  // -p^-1 mod 2^32
@@ -187,7 +177,7 @@ ZKG_DEV Fp<P> operator*(const Fp<P>& a, const Fp<P>& b) {
         }
         const uint32_t top = hi[7] + c;  // word 8 of the running value; cannot overflow
         // t += m * p, t >>= 32   (lo[0] becomes 0 by the choice of m)
-        const uint32_t m = mul_lo_full_rate(t[0], P::INV);
+        const uint32_t m = t[0] * P::INV;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const uint64_t s = (uint64_t)m * P::mod(j) + t[j];

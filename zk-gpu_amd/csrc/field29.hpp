@@ -126,7 +126,7 @@ ZKG_DEV F29 f29_mul(const F29& a, const F29& b) {
         for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
 #pragma unroll
         for (int j = 0; j < k; ++j) acc += (uint64_t)m[j] * M::P(k - j);
-        m[k] = mul_lo_full_rate((uint32_t)acc, M::PINV) & f29::MASK;
+        m[k] = ((uint32_t)acc * M::PINV) & f29::MASK;
         acc += (uint64_t)m[k] * M::P(0);  // clears the low 29 bits
         acc >>= 29;
     }

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_place_kernel(const uint32_t*
 // entries), pcur[q] (the next free run of each partition) and sbase[q] (exclusive scan of the
 // slices per partition, sbase[M] = all slices), M = W 2^plog. counts and the plan's `cursors`
 // (the next free position of each bucket after its start) are zeroed before.
-static constexpr uint32_t PART_MAX = 256;  // partitions per window, and buckets per partition, at most
+static constexpr uint32_t PART_MAX = 1024;  // partitions per window, and buckets per partition, at most
 static constexpr uint32_t BUCKET_THREADS = 256;
 static constexpr uint32_t SLICE = 8192;  // entries per workgroup of the bucket pass
 
@@ -634,10 +634,10 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_part_place_kernel(const uint
     for (uint32_t p = threadIdx.x; p < NP; p += SORT_THREADS) {  // reserve this block's run of each partition
         const uint32_t h = bcount[((size_t)w * nblk + blk) * NP + p];
         cursor[p] = h ? atomicAdd(&pcur[w * NP + p], h) : 0u;
-        if (h) atomicMax(&hot_s, (h << 8) | p);  // the block's fullest partition (h < 2^24)
+        if (h) atomicMax(&hot_s, (h << 10) | p);  // the block's fullest partition (h <= 2^20)
     }
     __syncthreads();
-    const uint32_t hot = hot_s & 0xffu;
+    const uint32_t hot = hot_s & 0x3ffu;
     const uint32_t lo = blk * P, hi = min(lo + P, n), bmask = (1u << sh) - 1;
     const uint32_t* d = digits + (size_t)w * n;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += 4 * SORT_THREADS) {
@@ -1159,13 +1159,16 @@ uint32_t msm_part_bits(const MsmShape& sh, uint32_t* part_log) {
     if (!sh.lds_sort) return 0;
     const char* e = std::getenv("ZKGPU_MSM_PARTSORT");
     if (e && e[0] == '0') return 0;
-    uint32_t cb = 0;  // bucket bits: B = 2^cb
+    uint32_t cb = 0, lgnv = 0;  // bucket bits (B = 2^cb); point-index bits (nv <= 2^lgnv)
     while ((1u << cb) < sh.B) ++cb;
-    // 2^8 partitions per window (fewer for small windows: at least four buckets each)
-    const uint32_t plog = std::min<uint32_t>(8, cb - 2);
-    const uint32_t kb = cb - plog;  // bucket bits within a partition
-    if ((uint64_t)sh.nv > (1ull << (31 - kb))) return 0;  // the point index needs 31 - kb bits
-    if ((1u << plog) > PART_MAX || (1u << kb) > PART_MAX) return 0;
+    while ((1ull << lgnv) < sh.nv) ++lgnv;
+    // 2^8 partitions per window (at least four buckets each), more where the point index needs the
+    // bits of the bucket-within-partition field (G1 2^24 at c = 16: 2^9)
+    uint32_t plog = std::min<uint32_t>(8, cb - 2);
+    if (cb - plog + lgnv > 31) plog = cb + lgnv - 31;
+    if (plog > cb || (1u << plog) > PART_MAX) return 0;
+    const uint32_t kb = cb - plog;  // bucket bits within a partition, at most 8
+    if (kb > 8 || (uint64_t)sh.nv > (1ull << (31 - kb))) return 0;
     // bcount, pbase (M + 1), pcur (M) and sbase (M + 1) must fit the plan's table of W nblk B words
     const uint64_t M = (uint64_t)sh.W << plog;
     if (M * sh.nblk + 3 * M + 2 > (uint64_t)sh.W * sh.nblk * sh.B) return 0;
