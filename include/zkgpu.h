@@ -385,6 +385,11 @@ int zkg_bench_copy(zkg_device* dev, size_t bytes, uint32_t reps, double* ms);
  * each as |k| (127 bits, 4 words) with the sign in bit 127, such that s = k1 + k2 * lambda mod r
  * (lambda = 0xb3c4d79d41a917585bfc41088d8daaa78b17ea66b99c90dd). Synchronous; host arrays. */
 int zkg_test_glv_split(zkg_device* dev, const uint32_t* scalars, size_t n, uint32_t* out);
+/* The G2 MSM's GLS scalar split on the device (no counterpart in the reference, which has no such
+ * split): out[12 i + 3 t], out[12 i + 3 t + 1] = |k_t| (low, high word), out[12 i + 3 t + 2] = the
+ * sign of k_t (1 = negative), t = 0..3, with s_i = k0 + k1 lambda + k2 mu + k3 lambda mu mod r,
+ * mu = p mod r, and |k_t| < 2^64. Synchronous; host arrays. */
+int zkg_test_gls_split(zkg_device* dev, const uint32_t* scalars, size_t n, uint32_t* out);
 /* Fixed-base multiples out[i] = ks[i] * generator (plain affine, infinity = zeros), for building
  * large synthetic inputs on the device. ks: n scalars (8 words each). */
 int zkg_util_generator_multiples(zkg_device* dev, int group, const zkg_buffer* ks, size_t n, zkg_buffer* out);

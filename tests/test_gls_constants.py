@@ -1,0 +1,170 @@
+"""The G2 GLS split's constants, derived here from first principles and compared with the ones the
+kernels use (read from csrc/msm.hip): psi's coefficients xi^((p-1)/3), xi^((p-1)/2) with xi = 9 + u
+and its eigenvalue p mod r on G2 (checked by point multiplication), the reduced basis of the
+4-dimensional lattice (membership, determinant +-r, the Babai bound below 2^64), the rounding
+constants G_j, and the split itself (restated with the kernel's integer steps) on random and
+extreme scalars: s = k0 + k1 lambda + k2 mu + k3 lambda mu (mod r), |k_t| < 2^64."""
+import os
+import random
+import re
+from fractions import Fraction
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+P, R = O.P, O.R
+XBN = 4965661367192848881  # the BN parameter of the curve
+LAM = 0xB3C4D79D41A917585BFC41088D8DAAA78B17EA66B99C90DD
+BETA_G2 = P - 1 - 0x59E26BCEA0D48BACD4F263F1ACDB5C4F5763473177FFFFFE
+
+
+def _source_consts():
+    src = open(os.path.join(ROOT, "zk-gpu_amd", "csrc", "msm.hip")).read()
+    body = src[src.index("struct GlsConst {"):src.index("typedef unsigned __int128 u128;")]
+    x = int(re.search(r"uint64_t X = (0x[0-9a-f]+)ull", body).group(1), 16)
+
+    def table(name, rows, cols):
+        blk = body[body.index(f"constexpr uint32_t {name}(int"):]
+        blk = blk[blk.index("= {") + 3:]
+        words = [int(w, 16) for w in re.findall(r"0x([0-9a-f]+)u", blk)[:rows * cols]]
+        return [sum(words[r * cols + k] << (32 * k) for k in range(cols)) for r in range(rows)]
+
+    return x, table("g", 4, 7), table("gx", 2, 8), table("gy", 2, 8)
+
+
+def _f2pow(a, e):
+    res = (1, 0)
+    while e:
+        if e & 1:
+            res = O.py_f2_mul(res, a)
+        a = O.py_f2_mul(a, a)
+        e >>= 1
+    return res
+
+
+def _psi(pt, gx, gy):
+    conj = lambda a: (a[0], (-a[1]) % P)
+    return O.py_f2_mul(conj(pt[0]), gx), O.py_f2_mul(conj(pt[1]), gy)
+
+
+def test_parameter_and_psi_coefficients():
+    x, _, gxm, gym = _source_consts()
+    assert x == XBN
+    assert P == 36 * x**4 + 36 * x**3 + 24 * x**2 + 6 * x + 1
+    assert R == 36 * x**4 + 36 * x**3 + 18 * x**2 + 6 * x + 1
+    gx, gy = _f2pow((9, 1), (P - 1) // 3), _f2pow((9, 1), (P - 1) // 2)
+    mont = lambda v: v * 2**256 % P
+    assert gxm == [mont(gx[0]), mont(gx[1])]
+    assert gym == [mont(gy[0]), mont(gy[1])]
+
+
+def test_psi_and_phi_eigenvalues_on_g2():
+    gx, gy = _f2pow((9, 1), (P - 1) // 3), _f2pow((9, 1), (P - 1) // 2)
+    mu = P % R
+    for k in (1, 123456789, R - 5):
+        q = O.py_g2_mul(O.G2, k)
+        pq = _psi(q, gx, gy)
+        assert O.py_g2_on_curve(pq)
+        assert pq == O.py_g2_mul(q, mu)
+        phiq = ((q[0][0] * BETA_G2 % P, q[0][1] * BETA_G2 % P), q[1])
+        assert phiq == O.py_g2_mul(q, LAM)
+        # phi and psi commute
+        phipq = ((pq[0][0] * BETA_G2 % P, pq[0][1] * BETA_G2 % P), pq[1])
+        assert phipq == _psi(phiq, gx, gy) == O.py_g2_mul(q, LAM * mu % R)
+
+
+def _basis(x):
+    return [[2 * x, x, x + 1, -x], [-x, x, x, 2 * x + 1], [-x - 1, x, -x, -2 * x], [2 * x + 1, x + 1, -x, x]]
+
+
+def _det(m):
+    m = [[Fraction(v) for v in row] for row in m]
+    d = Fraction(1)
+    for i in range(4):
+        piv = next(r for r in range(i, 4) if m[r][i] != 0)
+        if piv != i:
+            m[i], m[piv] = m[piv], m[i]
+            d = -d
+        d *= m[i][i]
+        for r in range(i + 1, 4):
+            f = m[r][i] / m[i][i]
+            m[r] = [a - f * b for a, b in zip(m[r], m[i])]
+    return d
+
+
+def _first_row_of_inverse_times_r(b):
+    # A_j = r (B^-1)_{0j}: solve A B = (r, 0, 0, 0)
+    m = [[Fraction(b[j][i]) for j in range(4)] + [Fraction(R if i == 0 else 0)] for i in range(4)]
+    for i in range(4):
+        piv = next(r for r in range(i, 4) if m[r][i] != 0)
+        m[i], m[piv] = m[piv], m[i]
+        m[i] = [v / m[i][i] for v in m[i]]
+        for r in range(4):
+            if r != i and m[r][i] != 0:
+                f = m[r][i]
+                m[r] = [a - f * c for a, c in zip(m[r], m[i])]
+    a = [m[i][4] for i in range(4)]
+    assert all(v.denominator == 1 for v in a)
+    return [int(v) for v in a]
+
+
+def test_lattice_basis_and_rounding_constants():
+    x, g, _, _ = _source_consts()
+    mu = P % R
+    ev = [1, LAM, mu, LAM * mu % R]
+    b = _basis(x)
+    for row in b:
+        assert sum(c * e for c, e in zip(row, ev)) % R == 0
+    assert abs(_det(b)) == R  # a basis of the whole lattice, not a sublattice
+    a = _first_row_of_inverse_times_r(b)
+    assert [v < 0 for v in a] == [False, True, True, False]
+    assert g == [(abs(v) * 2**288 + R // 2) // R for v in a]
+    # Babai bound: |k_i| <= (1/2 + 2^-35) sum_j |b_j,i| < 2^64
+    for i in range(4):
+        col = sum(abs(b[j][i]) for j in range(4))
+        assert Fraction(col) * (Fraction(1, 2) + Fraction(1, 2**35)) < 2**64
+
+
+def _split(s, x, g):
+    """The kernel's steps: c_j from bits 288.. of s G_j + 2^287, coefficients mod 2^128."""
+    s %= R
+    m128 = (1 << 128) - 1
+    c = [((s * g[j] + 2**287) >> 288) & m128 for j in range(4)]
+    p = [(cj * x) & m128 for cj in c]
+    k = [(s - (2 * p[0] + p[1] + p[2] + c[2] + 2 * p[3] + c[3])) & m128,
+         ((p[1] + p[2]) - (p[0] + p[3] + c[3])) & m128,
+         ((p[1] + p[3]) - (p[0] + c[0] + p[2])) & m128,
+         ((p[0] + 2 * p[1] + c[1]) - (2 * p[2] + p[3])) & m128]
+    return [v - (1 << 128) if v >> 127 else v for v in k]
+
+
+def gls_check(s, k):
+    mu = P % R
+    assert (k[0] + k[1] * LAM + k[2] * mu + k[3] * LAM * mu - s) % R == 0
+    assert all(abs(v) < 2**64 for v in k)
+
+
+def test_split_restated_on_random_and_extreme_scalars():
+    x, g, _, _ = _source_consts()
+    rng = random.Random(77)
+    cases = [0, 1, 2, R - 1, R, R + 1, 2**256 - 1, 2**64, 2**128, 2**192, R // 2, R // 3]
+    cases += [rng.randrange(2**256) for _ in range(3000)]
+    # scalars whose Babai coordinates sit near half-integers (the rounding's worst case)
+    b = _basis(x)
+    a = _first_row_of_inverse_times_r(b)
+    for _ in range(300):
+        j = rng.randrange(4)
+        t = rng.randrange(1, abs(a[j]))
+        cases.append((R * (2 * t + 1) // (2 * abs(a[j]))) % R)
+    for s in cases:
+        gls_check(s, _split(s, x, g))
+
+
+def test_g2_plan_shapes_every_window():
+    """Every G2 window 9..20 at sizes on both sides of the GLS cutoff gets a plan: GLS plans exist
+    only where their digits go through the LDS sort (c <= 16), the host rejects any other."""
+    from zkgpu import Group, Msm
+
+    for n in (1, 1000, 1 << 18, (1 << 18) + 1):
+        for c in range(9, 21):
+            assert Msm.plan_check(Group.G2, n, c) > 0, (n, c)

@@ -200,15 +200,11 @@ ZKG_DEV void glv_store(uint32_t* dst, uint32_t k[5]) {
     *reinterpret_cast<uint4*>(dst) = make_uint4(k[0], k[1], k[2], (k[3] & 0x7fffffffu) | (neg << 31));
 }
 
-__global__ __launch_bounds__(256) void msm_glv_split_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
-                                                           uint32_t* __restrict__ split) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s[8];
+// Scalar i (any 256-bit value) reduced mod r (2^256 < 6r).
+ZKG_DEV void load_scalar_mod_r(const uint32_t* __restrict__ scalars, uint32_t i, uint32_t s[8]) {
     const uint4* p = reinterpret_cast<const uint4*>(scalars + 8 * (size_t)i);
     const uint4 a = p[0], b = p[1];
     s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
-    // any 256-bit scalar: reduce mod r first (2^256 < 6r)
     for (int rep = 0; rep < 5; ++rep) {
         uint32_t d[8], borrow = 0;
 #pragma unroll
@@ -217,6 +213,14 @@ __global__ __launch_bounds__(256) void msm_glv_split_kernel(const uint32_t* __re
 #pragma unroll
         for (int k = 0; k < 8; ++k) s[k] = d[k];
     }
+}
+
+__global__ __launch_bounds__(256) void msm_glv_split_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
+                                                           uint32_t* __restrict__ split) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s[8];
+    load_scalar_mod_r(scalars, i, s);
     uint32_t c1[3], c2[5];
     glv_mul_high<3>(s, GlvConst::g1c, c1);
     glv_mul_high<5>(s, GlvConst::g2c, c2);
@@ -252,6 +256,152 @@ __global__ __launch_bounds__(256) void msm_endo_kernel(const uint32_t* __restric
     for (int j = 0; j < 8; ++j) beta.w[j] = group == 1 ? GlvConst::beta_g1(j) : GlvConst::beta_g2(j);
     const Fq x = fp_load<FqParams>(bases + 16 * group * i + 8 * k);
     fp_store(endo_x + 8 * t, x * beta);
+}
+
+// ---------------------------------------------------------------- GLS split (G2)
+// On G2 a second endomorphism is cheap: psi(x, y) = (conj(x) gx, conj(y) gy) with
+// gx = xi^((p-1)/3), gy = xi^((p-1)/2), xi = 9 + u (the p-power Frobenius carried to the sextic
+// twist), maps Q to mu Q, mu = p mod r, and commutes with phi. So
+//   s Q = k0 Q + k1 phi(Q) + k2 psi(Q) + k3 phi(psi(Q))  whenever  k0 + k1 lambda + k2 mu + k3 lambda mu = s (mod r),
+// and a 4-dimensional split gives coefficients of 64 bits instead of the GLV split's 127. The
+// lattice of (a0..a3) with a0 + a1 lambda + a2 mu + a3 lambda mu = 0 (mod r) has the LLL-reduced
+// basis (x = 0x44e992b44a6909f1, the BN parameter of the curve)
+//   b0 = (2x, x, x + 1, -x),  b1 = (-x, x, x, 2x + 1),  b2 = (-x - 1, x, -x, -2x),  b3 = (2x + 1, x + 1, -x, x).
+// Babai rounding: (s, 0, 0, 0) = sum_j alpha_j b_j with alpha_j = s A_j / r, A = r times the first
+// row of the basis' inverse (|A_j| < 2^189; A_1, A_2 < 0). c_j = floor((s G_j + 2^287) / 2^288) with
+// G_j = round(|A_j| 2^288 / r) is within 1/2 + 2^-35 of |alpha_j| (s < 2^254), and with d_j =
+// sign(A_j) b_j, (k0..k3) = (s, 0, 0, 0) - sum_j c_j d_j:
+//   k0 = s - 2x c0 - x c1 - (x + 1) c2 - (2x + 1) c3      k1 = -x c0 + x c1 + x c2 - (x + 1) c3
+//   k2 = -(x + 1) c0 + x c1 - x c2 + x c3                 k3 = x c0 + (2x + 1) c1 - 2x c2 - x c3
+// |k_i| <= (1/2 + 2^-35) sum_j |b_j,i| <= 3x (1 + 2^-34) < 2^63.7, so the coefficients are exact
+// modulo 2^128 and each fits 64 bits.
+struct GlsConst {
+    static constexpr uint64_t X = 0x44e992b44a6909f1ull;
+    // G_j, 7 words each
+    static ZKG_HD constexpr uint32_t g(int j, int k) {
+        constexpr uint32_t t[4][7] = {
+            {0x7d1fff2eu, 0x95d51bb1u, 0x46f4bda9u, 0xfc7184aeu, 0x08e5da66u, 0xb0d92b93u, 0x9e80318au},
+            {0x353ccca1u, 0x32e42728u, 0x2dff2915u, 0xa3e5577fu, 0x55b4ca7bu, 0xb0d92b95u, 0x9e80318au},
+            {0xa3f693b5u, 0xfac4daffu, 0x071c4c43u, 0xa3e55782u, 0x55b4ca7bu, 0xb0d92b95u, 0x9e80318au},
+            {0x23038c2au, 0xcef3cd3fu, 0xc170977du, 0xa3e5577du, 0x55b4ca7bu, 0xb0d92b95u, 0x9e80318au}};
+        return t[j][k];
+    }
+    // psi's coefficients times 2^256 mod p (the bases are in Montgomery form): component c (0 = real,
+    // 1 = u) of gx = 0x2fb347984f7911f74c0bec3cf559b143b78cc310c2c3330c99e39557176f553d
+    // + 0x16c9e55061ebae204ba4cc8bd75a079432ae2a1d0b7c9dce1665d51c640fcba2 u and of
+    // gy = 0x063cf305489af5dcdc5ec698b6e2f9b9dbaae0eda9c95998dc54014671a0135a
+    // + 0x07c03cbcac41049a0704b5a7ec796f2b21807dc98fa25bd282d37f632623b0e3 u
+    static ZKG_HD constexpr uint32_t gx(int c, int k) {
+        constexpr uint32_t t[2][8] = {
+            {0x4563ab30u, 0xb5773b10u, 0xa9aa6454u, 0x347f91c8u, 0x242e0991u, 0x7a007127u, 0x118214ecu, 0x1956bcd8u},
+            {0xa0aa4757u, 0x6e849f1eu, 0x89f89141u, 0xaa1c7b6du, 0xfae0ca3au, 0xb6e713cdu, 0x4e82ebc3u, 0x26694fbbu}};
+        return t[c][k];
+    }
+    static ZKG_HD constexpr uint32_t gy(int c, int k) {
+        constexpr uint32_t t[2][8] = {
+            {0x2936b629u, 0xe4bbdd0cu, 0xe133bacbu, 0xbb30f162u, 0xf9645366u, 0x31a9d1b6u, 0xa500f8ddu, 0x253570beu},
+            {0x5ffe77c7u, 0xa1d77ce4u, 0x7826d1dbu, 0x07affd11u, 0xbb7edc6bu, 0x6d16bd27u, 0x85defeccu, 0x2c872002u}};
+        return t[c][k];
+    }
+};
+
+typedef unsigned __int128 u128;
+
+// floor((s G_j + 2^287) / 2^288) mod 2^128: words 9..12 of the 15-word sum
+ZKG_DEV u128 gls_coeff(const uint32_t s[8], int j) {
+    uint32_t prod[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) prod[k] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const uint64_t t = (uint64_t)s[i] * GlsConst::g(j, k) + prod[i + k] + carry;
+            prod[i + k] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[i + 7] = (uint32_t)carry;
+    }
+    uint32_t c = 0;
+    prod[8] = __builtin_addc(prod[8], 0x80000000u, 0u, &c);
+#pragma unroll
+    for (int k = 9; k < 13; ++k) prod[k] = __builtin_addc(prod[k], 0u, c, &c);
+    return ((u128)prod[12] << 96) | ((u128)prod[11] << 64) | ((u128)prod[10] << 32) | prod[9];
+}
+
+// The coefficients (k0, k1, k2, k3) of scalar i as magnitudes (< 2^64) and signs.
+ZKG_DEV void gls_split(const uint32_t* __restrict__ scalars, uint32_t i, uint64_t mag[4], uint32_t neg[4]) {
+    uint32_t s[8];
+    load_scalar_mod_r(scalars, i, s);
+    const u128 c0 = gls_coeff(s, 0), c1 = gls_coeff(s, 1), c2 = gls_coeff(s, 2), c3 = gls_coeff(s, 3);
+    const u128 x = GlsConst::X;
+    const u128 p0 = c0 * x, p1 = c1 * x, p2 = c2 * x, p3 = c3 * x;  // mod 2^128
+    const u128 sl = ((u128)s[3] << 96) | ((u128)s[2] << 64) | ((u128)s[1] << 32) | s[0];
+    u128 k[4];
+    k[0] = sl - (p0 + p0 + p1 + p2 + c2 + p3 + p3 + c3);
+    k[1] = (p1 + p2) - (p0 + p3 + c3);
+    k[2] = (p1 + p3) - (p0 + c0 + p2);
+    k[3] = (p0 + p1 + p1 + c1) - (p2 + p2 + p3);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        neg[t] = (uint32_t)(k[t] >> 127);
+        mag[t] = (uint64_t)(neg[t] ? (u128)0 - k[t] : k[t]);
+    }
+}
+
+// GLS plans: table[0..n) = the bases, table[n..2n) = psi of each, endo_x[0..2n) = beta x of every
+// table entry (the table and phi images of a plan over 4n virtual points, as a precomputed plan's).
+__global__ __launch_bounds__(256) void msm_gls_prepare_kernel(const uint32_t* __restrict__ bases, uint32_t n,
+                                                             uint32_t* __restrict__ table,
+                                                             uint32_t* __restrict__ endo_x) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    typedef FeTraits<Fq2> T2;
+    Fq2 gx, gy;
+    Fq beta;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        gx.a.w[j] = GlsConst::gx(0, j);
+        gx.b.w[j] = GlsConst::gx(1, j);
+        gy.a.w[j] = GlsConst::gy(0, j);
+        gy.b.w[j] = GlsConst::gy(1, j);
+        beta.w[j] = GlvConst::beta_g2(j);
+    }
+    const uint32_t* src = bases + 32 * (size_t)i;
+    const Fq2 x = T2::load(src), y = T2::load(src + 16);
+    const Fq zero = fp_zero<FqParams>();
+    const Fq2 px = Fq2{x.a, zero - x.b} * gx, py = Fq2{y.a, zero - y.b} * gy;  // infinity (0, 0) stays
+    uint32_t* q = table + 32 * (size_t)i;
+    uint32_t* pq = table + 32 * ((size_t)n + i);
+    T2::store(q, x);
+    T2::store(q + 16, y);
+    T2::store(pq, px);
+    T2::store(pq + 16, py);
+    T2::store(endo_x + 16 * (size_t)i, Fq2{x.a * beta, x.b * beta});
+    T2::store(endo_x + 16 * ((size_t)n + i), Fq2{px.a * beta, px.b * beta});
+}
+
+__global__ __launch_bounds__(256) void msm_gls_split_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
+                                                           uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t mag[4];
+    uint32_t neg[4];
+    gls_split(scalars, i, mag, neg);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        uint32_t* o = out + 12 * (size_t)i + 3 * t;
+        o[0] = (uint32_t)mag[t];
+        o[1] = (uint32_t)(mag[t] >> 32);
+        o[2] = neg[t];
+    }
+}
+
+void msm_gls_split(const uint32_t* scalars, uint32_t n, uint32_t* out, hipStream_t stream) {
+    if (!n) return;
+    hipLaunchKernelGGL(msm_gls_split_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, scalars, n, out);
+    ZKG_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- digits: count and scatter
@@ -404,6 +554,33 @@ __global__ __launch_bounds__(256) void msm_digits_split_store_kernel(const uint3
             uint32_t mag, neg;
             signed_digit<true>(v, width, mag, neg, carry);
             digits[(size_t)w * nv + v_idx] = mag ? ((mag - 1) | ((neg ^ sign) << 31)) : NO_DIGIT;
+        }
+    }
+}
+
+// GLS plans (bases table = Q_0..Q_{n-1}, psi(Q_0)..psi(Q_{n-1})): virtual points [0, n) Q_i with
+// k0, [n, 2n) psi(Q_i) with k2, [2n, 3n) phi(Q_i) with k1, [3n, 4n) phi(psi(Q_i)) with k3. Each
+// coefficient (|k| < 2^64) has W signed digits over 65 bits; the top window (bit 64 clear) takes no
+// carry out.
+__global__ __launch_bounds__(256) void msm_gls_digits_store_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
+                                                                  uint32_t W, uint32_t wb, uint32_t nw,
+                                                                  uint32_t* __restrict__ digits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t nv = 4 * (size_t)n;
+    uint64_t mag[4];
+    uint32_t neg[4];
+    gls_split(scalars, i, mag, neg);
+    const uint32_t v_idx[4] = {i, 2 * n + i, n + i, 3 * n + i};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        uint32_t carry = 0;
+        for (uint32_t w = 0; w < W; ++w) {
+            const uint32_t width = window_width(w, wb, nw);
+            const uint32_t v = (uint32_t)(mag[t] >> window_pos(w, wb, nw)) & ((1u << width) - 1);
+            uint32_t dm, dn;
+            signed_digit<true>(v + carry, width, dm, dn, carry);
+            digits[(size_t)w * nv + v_idx[t]] = dm ? ((dm - 1) | ((dn ^ neg[t]) << 31)) : NO_DIGIT;
         }
     }
 }
@@ -880,9 +1057,7 @@ __global__ __launch_bounds__(256) void scan_add_kernel(const uint32_t* __restric
 
 // ---------------------------------------------------------------- plan
 
-uint32_t msm_plan_item_points(size_t n, uint32_t c) {
-    return msm_auto_item_points(msm_glv_enabled(n) ? 2 * n : n, c);
-}
+uint32_t msm_plan_item_points(int group, size_t n, uint32_t c) { return msm_shape(group, n, c, 0).CH; }
 
 MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uint32_t sort_points,
                    bool precomputed) {
@@ -900,22 +1075,35 @@ MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uin
         if (!sh.glv || npts > 0x1fffffffu)
             throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases needs the GLV split and n < 2^29");
     }
+    // GLS where its windows are no more than the GLV split's (c = 11, 13, 14) and n <= 2^18: the
+    // accumulation gathers from a table of 2n points and 2n phi x-coordinates (384 bytes per base,
+    // twice the GLV plan's), and beyond 2^18 bases those gathers fall out of the caches. Measured
+    // (G2, uniform scalars, best c of each): 2^16 1.53 ms against 1.84 for the GLV plan, 2^18 3.08
+    // against 3.38; 2^19 at c = 14 5.63 against 5.02 and 2^20 accumulate 8.99 ms against 6.49.
+    sh.gls = group == 2 && sh.glv && !sh.pre && c <= 16 && npts <= ((size_t)1 << 18) && msm_gls_enabled() &&
+             2 * ((65 + c - 1) / c) <= (128 + c - 1) / c;
     const uint64_t n = npts;
-    const uint64_t nv = sh.pre ? 4 * n : sh.glv ? 2 * n : n;
-    const uint32_t sbits = sh.pre ? 64 : 128;  // bits per (half-)scalar the windows cover
+    const uint64_t nv = sh.pre || sh.gls ? 4 * n : sh.glv ? 2 * n : n;
+    // bits per (half-)scalar the windows cover: |k| < 2^63.7 for GLS, so 65 bits of signed digits
+    const uint32_t sbits = sh.pre ? 64 : sh.gls ? 65 : 128;
     sh.nv = (uint32_t)nv;
     sh.W = sh.glv ? (sbits + c - 1) / c : (255 + c - 1) / c;
     sh.wb = sh.glv ? sbits / sh.W : c;
     sh.nw = sh.glv ? sbits - sh.wb * sh.W : 0;
-    sh.B = 1u << (c - 1);
+    const uint32_t cw = sh.gls ? sh.wb + (sh.nw ? 1 : 0) : c;
+    sh.cw = cw;
+    sh.B = 1u << (cw - 1);
     // sorted entries are addressed by 32-bit positions (the slot scan's starts) and hold W of
     // them per virtual point
     if ((uint64_t)sh.W * nv > 0xffffffffull)
         throw Error(ZKG_ERR_INVALID, "an MSM of " + std::to_string(n) + " points at c = " + std::to_string(c) +
                                          " needs " + std::to_string((uint64_t)sh.W * nv) +
                                          " sorted entries; at most 2^32 - 1 fit one plan");
-    sh.CH = item_points ? item_points : msm_auto_item_points(nv, c);
-    sh.L = msm_reduce_chunk(group, c, sh.W);
+    // GLS: the item size of the GLV plan at the same c. Its buckets hold twice the points, but the
+    // rule's larger items halve the work items, and the G2 accumulation (two waves per SIMD) then
+    // no longer fills the GPU (G2 2^18, c = 13: accumulate 2.50 ms at 64 points per item, 1.91 at 32).
+    sh.CH = item_points ? item_points : msm_auto_item_points(sh.gls ? nv / 2 : nv, cw);
+    sh.L = msm_reduce_chunk(group, cw, sh.W);
     sh.T = sh.B / sh.L;
     sh.aw = group == 1 ? 16 : 32;
     sh.xw = 2 * sh.aw;
@@ -927,8 +1115,10 @@ MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uin
     for (uint64_t m = (nv + sh.CH - 2) / sh.CH + 1; m > FINAL_GROUP; m = (m + SUM_GROUP - 1) / SUM_GROUP)
         ++sh.merge_rounds;
     sh.scan_blocks = (S + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    sh.lds_sort = c <= 16;
-    sh.sort_pts = sort_points ? std::max<uint32_t>(sort_points, sh.B) : msm_default_sort_points(nv, c);
+    sh.lds_sort = cw <= 16;
+    // the GLS digits exist only as the stored digits of the LDS-sort path
+    if (sh.gls && !sh.lds_sort) throw Error(ZKG_ERR_INTERNAL, "GLS plan without the LDS sort");
+    sh.sort_pts = sort_points ? std::max<uint32_t>(sort_points, sh.B) : msm_default_sort_points(nv, cw);
     sh.nblk = (uint32_t)((nv + sh.sort_pts - 1) / sh.sort_pts);
     return sh;
 }
@@ -936,7 +1126,9 @@ MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uin
 // Bytes of the allocations MsmPlan::MsmPlan makes for this shape (n real points).
 size_t MsmShape::workspace_bytes(size_t n) const {
     const size_t S = (size_t)W * B;
-    const size_t words = (glv ? 8 * n + (pre ? 2 : 1) * n * xw / 2 : 0)  // split, endo_x
+    const size_t words = (glv && !gls ? 8 * n : 0)                            // split
+                         + (glv ? (pre || gls ? 2 : 1) * n * xw / 2 : 0)      // endo_x
+                         + (gls ? 2 * n * aw : 0)                             // gls_bases
                          + 4 * S + 2                                      // counts, cursors, starts, istarts
                          + 3 * scan_blocks + 1                            // scan_tmp
                          + (size_t)W * nv * (lds_sort ? 2 : 1)            // sorted (+ digits)
@@ -953,6 +1145,8 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
     n = (uint32_t)npts;
     c = win;
     glv = sh.glv;
+    gls = sh.gls;
+    cw = sh.cw;
     f29 = msm_f29_enabled();
     lanes = msm_lanes_enabled((uint64_t)sh.W * sh.nv);
     part_bits = msm_part_bits(sh, &part_log);
@@ -991,10 +1185,9 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, uint32_t item_po
         ZKG_HIP(hipMalloc(&istarts, 4 * (S + 1)));
         ZKG_HIP(hipMalloc(&scan_tmp, 4 * (3 * (size_t)nb + 1)));
         max_items_dev = scan_tmp + 3 * nb;
-        if (glv) {
-            ZKG_HIP(hipMalloc(&split, 32 * (size_t)n));
-            ZKG_HIP(hipMalloc(&endo_x, 2 * base_points() * aw));
-        }
+        if (glv && !gls) ZKG_HIP(hipMalloc(&split, 32 * (size_t)n));
+        if (glv) ZKG_HIP(hipMalloc(&endo_x, 2 * (gls ? 2 * (size_t)n : base_points()) * aw));
+        if (gls) ZKG_HIP(hipMalloc(&gls_bases, 4 * 2 * (size_t)n * aw));
         ZKG_HIP(hipMalloc(&sorted, 4 * (size_t)W * nv));
         if (lds_sort) {
             ZKG_HIP(hipMalloc(&digits, 4 * (size_t)W * nv));
@@ -1031,8 +1224,9 @@ static size_t carve(const MsmShape& sh, size_t n, uint8_t* base, MsmPlan* p) {
     uint32_t* starts = (uint32_t*)take(4 * (S + 1));
     uint32_t* istarts = (uint32_t*)take(4 * (S + 1));
     uint32_t* scan_tmp = (uint32_t*)take(4 * (3 * sh.scan_blocks + 1));
-    uint32_t* split = sh.glv ? (uint32_t*)take(32 * n) : nullptr;
-    uint32_t* endo_x = sh.glv ? (uint32_t*)take(2 * (sh.pre ? 2 * n : n) * sh.aw) : nullptr;
+    uint32_t* split = sh.glv && !sh.gls ? (uint32_t*)take(32 * n) : nullptr;
+    uint32_t* endo_x = sh.glv ? (uint32_t*)take(2 * (sh.pre || sh.gls ? 2 * n : n) * sh.aw) : nullptr;
+    uint32_t* gls_bases = sh.gls ? (uint32_t*)take(4 * 2 * n * sh.aw) : nullptr;
     uint32_t* sorted = (uint32_t*)take(4 * (size_t)sh.W * sh.nv);
     uint32_t* digits = sh.lds_sort ? (uint32_t*)take(4 * (size_t)sh.W * sh.nv) : nullptr;
     uint32_t* table = sh.lds_sort ? (uint32_t*)take(4 * (size_t)sh.W * sh.nblk * sh.B) : nullptr;
@@ -1050,6 +1244,7 @@ static size_t carve(const MsmShape& sh, size_t n, uint8_t* base, MsmPlan* p) {
         p->max_items_dev = scan_tmp + 3 * sh.scan_blocks;
         p->split = split;
         p->endo_x = endo_x;
+        p->gls_bases = gls_bases;
         p->sorted = sorted;
         p->digits = digits;
         p->table = table;
@@ -1082,6 +1277,8 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, const MsmShape& 
     n = (uint32_t)npts;
     c = win;
     glv = sh.glv;
+    gls = sh.gls;
+    cw = sh.cw;
     f29 = msm_f29_enabled();
     lanes = msm_lanes_enabled((uint64_t)sh.W * sh.nv);
     part_bits = msm_part_bits(sh, &part_log);
@@ -1113,11 +1310,11 @@ MsmPlan::MsmPlan(Device* d, int grp, size_t npts, uint32_t win, const MsmShape& 
 
 MsmPlan::~MsmPlan() {
     if (!owned) return;  // carved from a caller's region: nothing to free
-    for (uint32_t* p : {split, endo_x, counts, cursors, starts, istarts, scan_tmp, sorted, digits, table, item_pts,
-                        buckets, partials, wsums, result})
+    for (uint32_t* p : {split, endo_x, gls_bases, counts, cursors, starts, istarts, scan_tmp, sorted, digits, table,
+                        item_pts, buckets, partials, wsums, result})
         if (p) (void)hipFree(p);
     if (item_info) (void)hipFree(item_info);
-    split = endo_x = counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_pts = buckets =
+    split = endo_x = gls_bases = counts = cursors = starts = istarts = scan_tmp = sorted = digits = table = item_pts = buckets =
         partials = wsums = result = nullptr;
     max_items_dev = nullptr;  // points into scan_tmp
     item_info = nullptr;
@@ -1129,6 +1326,11 @@ bool msm_glv_enabled(size_t n) {
     // virtual point indices (2n of them) must fit the 31-bit index field of a sorted entry
     if (n > 0x3fffffffu) return false;
     const char* e = std::getenv("ZKGPU_MSM_GLV");
+    return !(e && e[0] == '0');
+}
+
+bool msm_gls_enabled() {
+    const char* e = std::getenv("ZKGPU_MSM_GLS");
     return !(e && e[0] == '0');
 }
 
@@ -1250,9 +1452,14 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         ZKG_HIP(hipStreamSynchronize(s));
         t0 = std::chrono::steady_clock::now();
     }
-    // GLV: split every scalar into two signed 127-bit halves and compute phi(P_i) = (beta x_i, y_i)
+    // GLS: the table of the bases and their psi images, and phi of both; the stages then read the
+    // table. GLV: split every scalar into two signed 127-bit halves and compute phi(P_i) = (beta x_i, y_i).
     const uint32_t* dscal = scalars;
-    if (p.glv) {
+    if (p.gls) {
+        hipLaunchKernelGGL(msm_gls_prepare_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, bases, p.n, p.gls_bases,
+                           p.endo_x);
+        bases = p.gls_bases;
+    } else if (p.glv) {
         hipLaunchKernelGGL(msm_glv_split_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, scalars, p.n, p.split);
         const size_t nbase = p.base_points();
         hipLaunchKernelGGL(msm_endo_kernel, dim3(ceil_div(nbase * p.group, 256)), dim3(256), 0, s, bases,
@@ -1263,10 +1470,13 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
     // two-pass placement: the digits go to `sorted`'s buffer, the partition runs to `digits`'s,
     // and the final list to `sorted` (both W * nv words)
     const uint32_t kb = p.part_bits, plog = p.part_log, NP = kb ? 1u << plog : 0;
-    const uint32_t cb = p.c - 1;  // bucket bits of a full window
+    const uint32_t cb = p.cw - 1;  // bucket bits of a full window
     uint32_t* dig = kb ? p.sorted : p.digits;
     if (p.lds_sort) {
-        if (p.pre)
+        if (p.gls)
+            hipLaunchKernelGGL(msm_gls_digits_store_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, scalars, p.n,
+                               p.W, p.wb, p.nw, dig);
+        else if (p.pre)
             hipLaunchKernelGGL(msm_digits_split_store_kernel, dim3(ceil_div(2 * (size_t)p.n, 256)), dim3(256), 0, s,
                                dscal, p.n, p.W, p.wb, p.nw, dig);
         else if (p.glv)
@@ -1303,10 +1513,10 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         fill_u32(p.counts, 0, S, s);
         fill_u32(p.cursors, 0, S, s);
         if (p.glv)
-            hipLaunchKernelGGL((msm_digits_kernel<false, true>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+            hipLaunchKernelGGL((msm_digits_kernel<false, true>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.cw, p.W,
                                p.wb, p.nw, p.counts, (const uint32_t*)nullptr, (uint32_t*)nullptr);
         else
-            hipLaunchKernelGGL((msm_digits_kernel<false, false>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+            hipLaunchKernelGGL((msm_digits_kernel<false, false>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.cw, p.W,
                                p.wb, p.nw, p.counts, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     }
     stage("count");
@@ -1327,10 +1537,10 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         hipLaunchKernelGGL(msm_place_kernel, dim3(p.nblk, p.W), dim3(SORT_THREADS), 4 * p.B, s, p.digits, p.nv, p.B,
                            p.sort_pts, p.nblk, p.table, p.starts, p.sorted);
     } else if (p.glv) {
-        hipLaunchKernelGGL((msm_digits_kernel<true, true>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+        hipLaunchKernelGGL((msm_digits_kernel<true, true>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.cw, p.W,
                            p.wb, p.nw, p.cursors, p.starts, p.sorted);
     } else {
-        hipLaunchKernelGGL((msm_digits_kernel<true, false>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.c, p.W,
+        hipLaunchKernelGGL((msm_digits_kernel<true, false>), dim3(pg), dim3(256), 0, s, dscal, p.nv, p.cw, p.W,
                            p.wb, p.nw, p.cursors, p.starts, p.sorted);
     }
     stage("scatter");

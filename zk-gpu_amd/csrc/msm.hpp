@@ -14,16 +14,18 @@ namespace zkg {
 
 uint32_t msm_default_window(size_t n);          // clamp(ilog2(max(n,1)) - 5, 11, 16)
 uint32_t msm_auto_item_points(size_t n, uint32_t c);
-// The work-item size a plan for (n, c) picks when none is given.
-uint32_t msm_plan_item_points(size_t n, uint32_t c);
+// The work-item size a plan for (group, n, c) picks when none is given.
+uint32_t msm_plan_item_points(int group, size_t n, uint32_t c);
 // Everything a plan for (group, n, c, item_points, sort_points, precomputed) derives from its
 // arguments (0 = automatic), with the argument checks of Msm::for_group (src/msm.rs:96-98) and the
 // index-range checks of this library's kernels: positions into the sorted point list, work-item
 // numbers and virtual point indices are 32-bit, so a plan whose W * nv entries or work items
 // reach 2^32 is rejected here (an Error with a message), before anything is allocated.
 struct MsmShape {
-    bool glv = false, pre = false, lds_sort = false;
-    uint32_t nv = 0, W = 0, wb = 0, nw = 0, B = 0, CH = 0, L = 0, T = 0, aw = 0, xw = 0;
+    bool glv = false, pre = false, gls = false, lds_sort = false;
+    // cw: digit bits of the widest window (B = 2^(cw - 1)); c, except where a GLS plan's windows
+    // are narrower (65 bits in five windows of 13 at c = 14)
+    uint32_t nv = 0, W = 0, wb = 0, nw = 0, cw = 0, B = 0, CH = 0, L = 0, T = 0, aw = 0, xw = 0;
     size_t max_items = 0;
     uint32_t merge_rounds = 0, sort_pts = 0, nblk = 0;
     size_t scan_blocks = 0;  // blocks of the slot scan
@@ -44,6 +46,9 @@ size_t msm_workspace_bytes(int group, size_t n, uint32_t c, uint32_t item_points
                            bool precomputed = false);
 // Whether plans use the GLV split (default on; ZKGPU_MSM_GLV=0 turns it off for comparisons).
 bool msm_glv_enabled(size_t n);
+// Whether G2 plans may use the GLS split (MsmPlan::gls; default on, ZKGPU_MSM_GLS=0 turns it off
+// for comparisons).
+bool msm_gls_enabled();
 // Whether plans use the 29-bit-limb field (field29.hpp): the whole G1 pipeline, and the G2 stages
 // after the bucket accumulation (default on; ZKGPU_MSM_F29=0 selects the 32-bit-limb kernels,
 // for comparisons). Same results either way.
@@ -74,14 +79,24 @@ struct MsmPlan {
     // windows. The same bucket additions as GLV alone, but half the windows: half the buckets to
     // finalize and reduce and a doubling chain of about 64 - c instead of 128 - c.
     bool pre = false;
+    // GLS (G2): psi(x, y) = (conj(x) gx, conj(y) gy) acts on G2 as mu = p mod r, so each scalar splits
+    // into four signed coefficients below 2^64 (msm.hip, GLS split) of Q, phi(Q), psi(Q) and
+    // phi(psi(Q)): nv = 4n virtual points over ceil(65 / c) windows. The run first writes a table
+    // of the bases and their psi images (gls_bases) and the phi images of both (endo_x), and the
+    // stages then run as over a precomputed table. Used where it takes no more windows of digits
+    // than the GLV split (2 ceil(65 / c) <= ceil(128 / c): c = 11, 13, 14): the same bucket
+    // additions, half the buckets to finalize and reduce, and a doubling chain of 65 - c bits; and
+    // for n <= 2^18, where the gathers from its larger table stay in the caches (msm_shape).
+    bool gls = false;
     bool f29 = false;        // 29-bit-limb field (field29.hpp)
-    uint32_t nv = 0;         // virtual points: 4n precomputed, 2n with GLV, else n
-    uint32_t W = 0;          // windows: ceil(64 / c) precomputed, ceil(128 / c) with GLV, else ceil(255 / c)
+    uint32_t nv = 0;         // virtual points: 4n precomputed or GLS, 2n with GLV, else n
+    uint32_t W = 0;          // windows: ceil(64 / c) precomputed, ceil(65 / c) GLS, ceil(128 / c) GLV, else ceil(255 / c)
+    uint32_t cw = 0;         // digit bits of the widest window (c, or less for GLS: MsmShape::cw)
     // Window w covers bits [w*wb + min(w, nw), +wb + (w < nw)): without GLV wb = c, nw = 0; with
     // GLV the 128 bits are split evenly (c = 15: two windows of 15 bits and seven of 14), so no
     // window is left with a few bits and a few heavy buckets.
     uint32_t wb = 0, nw = 0;
-    uint32_t B = 0;          // buckets per window, 2^(c-1)
+    uint32_t B = 0;          // buckets per window, 2^(cw-1)
     uint32_t CH = 0;         // points per work item
     uint32_t L = 0;          // buckets per running-sum thread
     uint32_t T = 0;          // running-sum threads per window, B / L
@@ -94,8 +109,9 @@ struct MsmPlan {
     uint32_t nblk = 0;       // sort blocks per window
 
     // workspace (device)
-    uint32_t* split = nullptr;       // 2n * 4  GLV half-scalars: |k| in 127 bits, sign in bit 127
-    uint32_t* endo_x = nullptr;      // n * aw / 2 (2n * aw / 2 precomputed)  beta * x of every base (Montgomery form)
+    uint32_t* split = nullptr;       // 2n * 4  GLV half-scalars: |k| in 127 bits, sign in bit 127 (not GLS)
+    uint32_t* endo_x = nullptr;      // n * aw / 2 (2n * aw / 2 precomputed or GLS)  beta * x of every base (Montgomery form)
+    uint32_t* gls_bases = nullptr;   // 2n * aw  GLS: the bases, then psi of each
     uint32_t* counts = nullptr;      // W*B  points per (window, bucket)
     uint32_t* cursors = nullptr;     // W*B  scatter cursors
     uint32_t* starts = nullptr;      // W*B + 1  exclusive scan of counts (global over windows)
@@ -125,7 +141,7 @@ struct MsmPlan {
     MsmPlan(Device* d, int group, size_t n, uint32_t c, const MsmShape& sh, void* workspace, uint32_t* result_words);
     // Virtual points below this index are entries of the bases buffer, the others phi of one
     // (2^32 - 1 without GLV).
-    uint32_t n_real() const { return pre ? 2 * n : glv ? n : 0xffffffffu; }
+    uint32_t n_real() const { return pre || gls ? 2 * n : glv ? n : 0xffffffffu; }
     // Points the bases buffer must hold: 2n for a precomputed plan, else n.
     size_t base_points() const { return pre ? 2 * (size_t)n : n; }
     // Whether the bucket accumulation cuts its lanes at multiples of CH of the sorted list (a
@@ -154,6 +170,9 @@ struct MsmPlan {
 // GLV split of n scalars (any 256-bit values) into split[0..4n) = k1 of each, split[4n..8n) = k2
 // of each: |k| in the low 127 bits, its sign in bit 127, s = k1 + k2 lambda (mod r).
 void msm_glv_split(const uint32_t* scalars, uint32_t n, uint32_t* split, hipStream_t stream);
+// GLS split of n scalars (any 256-bit values): out[12 i + 3 t ..] = |k_t| (2 words), sign of k_t,
+// t = 0..3, with s = k0 + k1 lambda + k2 mu + k3 lambda mu (mod r), mu = p mod r, |k_t| < 2^64.
+void msm_gls_split(const uint32_t* scalars, uint32_t n, uint32_t* out, hipStream_t stream);
 
 // Plain affine coordinates -> Montgomery form in place (`coords` elements of 8 words).
 void msm_convert_bases(uint32_t* xy, size_t coords, hipStream_t stream);

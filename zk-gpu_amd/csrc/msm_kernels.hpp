@@ -1231,7 +1231,7 @@ void launch_reduce(const MsmPlan& p, hipStream_t s) {
     constexpr uint32_t LN = FeLanes<Ft>::value;
     const ReduceGrid<Ft> g(p);
     const size_t wgs = (size_t)p.nw * g.gh + (size_t)(p.W - p.nw) * g.gl;
-    hipLaunchKernelGGL(msm_reduce_kernel<Ft>, dim3(wgs), dim3(g.rb * LN), 0, s, p.buckets, p.W, p.B, p.L, p.c,
+    hipLaunchKernelGGL(msm_reduce_kernel<Ft>, dim3(wgs), dim3(g.rb * LN), 0, s, p.buckets, p.W, p.B, p.L, p.cw,
                        p.wb, p.nw, g.gh, g.gl, g.P, p.partials);
 }
 

@@ -227,7 +227,7 @@ MsmTune tune_msm(Device* d, int group, size_t n) {
         if (cand.ms < best.ms) best = cand;
     }
     // 2. work-item size around the automatic one
-    const uint32_t auto_ch = msm_plan_item_points(n, best.c);
+    const uint32_t auto_ch = msm_plan_item_points(group, n, best.c);
     for (uint32_t ch : {auto_ch / 2, auto_ch * 2}) {
         if (ch < 4 || ch > 1024) continue;
         MsmTune cand = best;
@@ -236,8 +236,8 @@ MsmTune tune_msm(Device* d, int group, size_t n) {
         if (cand.ms < best.ms) best = cand;
     }
     // 3. points per counting-sort block around the default one
-    const uint32_t B = 1u << (best.c - 1);
-    const uint32_t auto_sp = msm_default_sort_points(msm_glv_enabled(n) ? 2 * n : n, best.c);
+    const MsmShape best_sh = msm_shape(group, n, best.c, 0);
+    const uint32_t B = best_sh.B, auto_sp = best_sh.sort_pts;
     for (uint32_t sp : {auto_sp / 2, auto_sp * 2, auto_sp * 4}) {
         if (sp < B) continue;
         MsmTune cand = best;

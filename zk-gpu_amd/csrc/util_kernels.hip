@@ -331,6 +331,22 @@ int zkg_test_glv_split(zkg_device* dev, const uint32_t* scalars, size_t n, uint3
     ZKG_GUARD_END
 }
 
+int zkg_test_gls_split(zkg_device* dev, const uint32_t* scalars, size_t n, uint32_t* out) {
+    if (!dev || !scalars || !out) return zkg::fail(zkg::ZKG_ERR_INVALID, "null argument");
+    if (n == 0) return zkg::ZKG_OK;
+    if (n > 0x3fffffffu) return zkg::fail(zkg::ZKG_ERR_INVALID, "too many scalars");
+    ZKG_GUARD_BEGIN
+    zkg::device_bind(dev->d);
+    zkg::DeviceArray<uint32_t> ds(8 * n), dout(12 * n);
+    hipStream_t s = dev->d->stream;
+    ZKG_HIP(hipMemcpyAsync(ds.ptr, scalars, 32 * n, hipMemcpyHostToDevice, s));
+    zkg::msm_gls_split(ds.ptr, (uint32_t)n, dout.ptr, s);
+    ZKG_HIP(hipMemcpyAsync(out, dout.ptr, 48 * n, hipMemcpyDeviceToHost, s));
+    ZKG_HIP(hipStreamSynchronize(s));
+    return zkg::ZKG_OK;
+    ZKG_GUARD_END
+}
+
 int zkg_util_generator_multiples(zkg_device* dev, int group, const zkg_buffer* ks, size_t n, zkg_buffer* out) {
     if (!dev || !ks || !out) return zkg::fail(zkg::ZKG_ERR_INVALID, "null argument");
     if (group != ZKG_G1 && group != ZKG_G2) return zkg::fail(zkg::ZKG_ERR_INVALID, "bad group");

@@ -127,6 +127,7 @@ SIGNATURES = {
                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "zkg_bench_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "zkg_test_glv_split": (ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_size_t, c_u32p]),
+    "zkg_test_gls_split": (ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_size_t, c_u32p]),
     "zkg_util_generator_multiples": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
                                                     ctypes.c_void_p]),
 }
