@@ -437,12 +437,25 @@ struct Fq2P {
 
 ZKG_DEV bool pair_odd() { return (__lane_id() & 1u) != 0; }
 
-ZKG_DEV Fq29 pair_other(const Fq29& x) {
+// Lane exchanges within each group of four lanes as DPP moves (quad_perm: lane i of a group takes
+// the value of lane sel_i, dpp_ctrl = sel0 | sel1 << 2 | sel2 << 4 | sel3 << 6): one VALU move per
+// word, where __shfl and __shfl_xor go through the LDS crossbar (ds_bpermute) and wait on its
+// latency. Every lane of the group must be active, as for the shuffles they replace.
+constexpr int DPP_XOR1 = 0xb1;  // lanes 1, 0, 3, 2
+constexpr int DPP_XOR2 = 0x4e;  // lanes 2, 3, 0, 1
+template <int CTRL>
+ZKG_DEV uint32_t dpp_quad(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+ZKG_DEV Fq29 fq29_dpp(const Fq29& x) {
     Fq29 r;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) r.l[i] = __shfl_xor(x.l[i], 1);
+    for (int i = 0; i < 9; ++i) r.l[i] = dpp_quad<CTRL>(x.l[i]);
     return r;
 }
+
+ZKG_DEV Fq29 pair_other(const Fq29& x) { return fq29_dpp<DPP_XOR1>(x); }
 
 ZKG_DEV Fq2P operator+(const Fq2P& x, const Fq2P& y) { return Fq2P{x.v + y.v}; }
 ZKG_DEV Fq2P operator-(const Fq2P& x, const Fq2P& y) { return Fq2P{x.v - y.v}; }
@@ -462,7 +475,7 @@ struct FeTraits<Fq2P> {
     static ZKG_DEV Fq2P one() { return Fq2P{fe_select(pair_odd(), FeTraits<Fq29>::zero(), FeTraits<Fq29>::one())}; }
     static ZKG_DEV bool is_zero(const Fq2P& x) {
         const int z = fq29_is_zero(x.v) ? 1 : 0;
-        return (z & __shfl_xor(z, 1)) != 0;
+        return (z & (int)dpp_quad<DPP_XOR1>((uint32_t)z)) != 0;
     }
     static ZKG_DEV Fq2P sqr(const Fq2P& x) {  // even: (a + b)(a - b); odd: (b + b) a
         const bool odd = pair_odd();
@@ -586,12 +599,7 @@ struct Fq2Q {
 ZKG_DEV bool quad_odd() { return (__lane_id() & 2u) != 0; }
 
 // the same component from the other pair of the group
-ZKG_DEV Fq2P quad_other(const Fq2P& x) {
-    Fq29 r;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) r.l[i] = __shfl_xor(x.v.l[i], 2);
-    return Fq2P{r};
-}
+ZKG_DEV Fq2P quad_other(const Fq2P& x) { return Fq2P{fq29_dpp<DPP_XOR2>(x.v)}; }
 
 ZKG_DEV Fq2Q operator+(const Fq2Q& x, const Fq2Q& y) { return Fq2Q{x.v + y.v}; }
 ZKG_DEV Fq2Q operator-(const Fq2Q& x, const Fq2Q& y) { return Fq2Q{x.v - y.v}; }

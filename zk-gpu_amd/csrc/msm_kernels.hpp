@@ -916,10 +916,19 @@ ZKG_DEV F29 f29_pick(uint32_t lane, const F29& a0, const F29& a1, const F29& a2,
     return f29_sel(lane == 0, a0, f29_sel(lane == 1, a1, f29_sel(lane == 2, a2, f29_sel(lane == 3, a3,
                                                                                       f29_sel(lane == 4, a4, a5)))));
 }
+// lane `src`'s value on every lane (src wave-uniform: a read of that lane into a scalar register)
 ZKG_DEV F29 f29_shfl(const F29& x, int src) {
     F29 r;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) r.l[i] = __shfl(x.l[i], src);
+    for (int i = 0; i < 9; ++i) r.l[i] = (uint32_t)__builtin_amdgcn_readlane((int)x.l[i], src);
+    return r;
+}
+// within each group of four lanes, lane i takes lane sel_i's value (dpp_quad's quad_perm control)
+template <int CTRL>
+ZKG_DEV F29 f29_dpp(const F29& x) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = dpp_quad<CTRL>(x.l[i]);
     return r;
 }
 // limbs 0..7 below 2^29 again (the value is unchanged)
@@ -958,9 +967,10 @@ ZKG_DEV Xyzz29 xyzz29_dbl_lanes(const Xyzz29& p, uint32_t lane) {
 
 // The doubling chain with fixed lane roles: between doublings lane 0 holds Y, lane 1 X and lane 2
 // ZZ and ZZZ (other fields, and lanes 3..63, hold values no one reads), and each product's lane
-// fetches only its own operands, in four rounds of lane shuffles per doubling instead of the
-// nine full broadcasts of xyzz29_dbl_lanes (the LDS round trips were a third of the chain's
-// cycles). The formulas, their order and hence the bounds are those of xyzz29_dbl_lanes:
+// fetches only its own operands, in four rounds of DPP moves within lanes 0..3 per doubling
+// instead of the nine full broadcasts of xyzz29_dbl_lanes (LDS round trips had been a third of
+// the chain's cycles). The formulas, their order and hence the bounds are those of
+// xyzz29_dbl_lanes:
 //   L1  lane 0: V = U^2 (U = 2Y)   lane 1: X^2
 //       round 1: lanes 0, 3 <- X^2 (lane 1); lanes 1, 2 <- V (lane 0)
 //   L2  lane 0: W = U V   lane 1: S = X V   lane 2: ZZ3 = V ZZ   lane 3: M^2 (M = 3 X^2)
@@ -971,13 +981,10 @@ ZKG_DEV Xyzz29 xyzz29_dbl_lanes(const Xyzz29& p, uint32_t lane) {
 //       round 4: lane 0 <- W Y (lane 1), Y3 = M D - W Y; lane 1 <- X3 (lane 0)
 struct DblRoles {
     uint32_t lane;
-    int src1, src2, src3, src4;  // the lane each round reads from
-    explicit ZKG_DEV DblRoles(uint32_t l) : lane(l) {
-        src1 = (l == 1 || l == 2) ? 0 : 1;
-        src2 = l == 0 ? 1 : 0;
-        src3 = l == 0 ? 3 : 0;
-        src4 = l == 0 ? 1 : 0;
-    }
+    // the lane each round reads from, as quad_perm controls: round 1 (1, 0, 0, 1), round 2 and 4
+    // (1, 0, 0, 0), round 3 (3, 0, 0, 0)
+    static constexpr int R1 = 0x41, R2 = 0x01, R3 = 0x03, R4 = 0x01;
+    explicit ZKG_DEV DblRoles(uint32_t l) : lane(l) {}
 };
 
 ZKG_DEV Xyzz29 xyzz29_dbl_roles(const Xyzz29& p, const DblRoles& ro) {
@@ -985,21 +992,21 @@ ZKG_DEV Xyzz29 xyzz29_dbl_roles(const Xyzz29& p, const DblRoles& ro) {
     const F29 u = f29_add(p.y, p.y);                        // lane 0: < 10, limbs < 2^30
     const F29 o1 = f29_sel(lane == 0, u, p.x);
     const F29 m1 = f29_mul(o1, o1);                         // lane 0: V, lane 1: X^2 (10 * 10, 9 * 9) -> < 2
-    const F29 r1 = f29_shfl(m1, ro.src1);                   // lanes 0, 3: X^2; lanes 1, 2: V
+    const F29 r1 = f29_dpp<DblRoles::R1>(m1);               // lanes 0, 3: X^2; lanes 1, 2: V
     const F29 mm = f29_carry(f29_add(f29_add(r1, r1), r1));  // lanes 0, 3: M = 3 X^2 < 6, normalised
     const F29 a2 = f29_pick(lane, u, p.x, p.zz, mm);
     const F29 b2 = f29_pick(lane, m1, r1, r1, mm);
     const F29 m2 = f29_mul(a2, b2);                         // W, S, ZZ3, M^2 (10 * 2, 9 * 2, 2 * 2, 6 * 6) -> < 2
-    const F29 r2 = f29_shfl(m2, ro.src2);                   // lane 0: S; lanes 1, 2: W
+    const F29 r2 = f29_dpp<DblRoles::R2>(m2);               // lane 0: S; lanes 1, 2: W
     const F29 q3 = f29_sel(lane == 3, m2, p.y);
-    const F29 r3 = f29_shfl(q3, ro.src3);                   // lane 0: M^2; lane 1: Y
+    const F29 r3 = f29_dpp<DblRoles::R3>(q3);               // lane 0: M^2; lane 1: Y
     const F29 x3 = f29_sub<6>(r3, f29_add(r2, r2));         // lane 0: 2S < 4 < 5: X3 < 8
     const F29 d = f29_sub<10>(r2, x3);                      // lane 0: < 12
     const F29 a3 = f29_sel(lane == 0, mm, r2);
     const F29 b3 = f29_pick(lane, d, r3, p.zzz);
     const F29 m3 = f29_mul(a3, b3);                         // M D, W Y, ZZZ3 (6 * 12, 2 * 5, 2 * 2) -> < 2
     const F29 q4 = f29_sel(lane == 0, x3, m3);
-    const F29 r4 = f29_shfl(q4, ro.src4);                   // lane 0: W Y; lane 1: X3
+    const F29 r4 = f29_dpp<DblRoles::R4>(q4);               // lane 0: W Y; lane 1: X3
     Xyzz29 o;
     o.y = f29_sub<3>(m3, r4);                               // lane 0: Y3 = M D - W Y < 5
     o.x = r4;                                               // lane 1: X3
