@@ -190,14 +190,14 @@ ZKG_DEV void glv_sub_mul(uint32_t acc[5], const uint32_t* a, G b) {
 }
 
 // 160-bit two's complement -> |k| (127 bits) with the sign in bit 127
-ZKG_DEV void glv_store(uint32_t* dst, uint32_t k[5]) {
+ZKG_DEV uint4 glv_pack(uint32_t k[5]) {
     const uint32_t neg = k[4] >> 31;
     if (neg) {
         uint32_t carry = 1;
 #pragma unroll
         for (int i = 0; i < 5; ++i) k[i] = __builtin_addc(~k[i], 0u, carry, &carry);
     }
-    *reinterpret_cast<uint4*>(dst) = make_uint4(k[0], k[1], k[2], (k[3] & 0x7fffffffu) | (neg << 31));
+    return make_uint4(k[0], k[1], k[2], (k[3] & 0x7fffffffu) | (neg << 31));
 }
 
 // Scalar i (any 256-bit value) reduced mod r (2^256 < 6r).
@@ -215,10 +215,8 @@ ZKG_DEV void load_scalar_mod_r(const uint32_t* __restrict__ scalars, uint32_t i,
     }
 }
 
-__global__ __launch_bounds__(256) void msm_glv_split_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
-                                                           uint32_t* __restrict__ split) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// The GLV halves (k1, k2) of scalar i, packed as glv_pack does.
+ZKG_DEV void glv_split(const uint32_t* __restrict__ scalars, uint32_t i, uint4* h1, uint4* h2) {
     uint32_t s[8];
     load_scalar_mod_r(scalars, i, s);
     uint32_t c1[3], c2[5];
@@ -235,8 +233,18 @@ __global__ __launch_bounds__(256) void msm_glv_split_kernel(const uint32_t* __re
     uint32_t borrow = 0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) k2[k] = __builtin_subc(k2[k], t[k], borrow, &borrow);
-    glv_store(split + 4 * (size_t)i, k1);
-    glv_store(split + 4 * ((size_t)n + i), k2);
+    *h1 = glv_pack(k1);
+    *h2 = glv_pack(k2);
+}
+
+__global__ __launch_bounds__(256) void msm_glv_split_kernel(const uint32_t* __restrict__ scalars, uint32_t n,
+                                                           uint32_t* __restrict__ split) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint4 h1, h2;
+    glv_split(scalars, i, &h1, &h2);
+    *reinterpret_cast<uint4*>(split + 4 * (size_t)i) = h1;
+    *reinterpret_cast<uint4*>(split + 4 * ((size_t)n + i)) = h2;
 }
 
 void msm_glv_split(const uint32_t* scalars, uint32_t n, uint32_t* split, hipStream_t stream) {
@@ -526,6 +534,42 @@ __global__ __launch_bounds__(256) void msm_digits_store_kernel(const uint32_t* _
         uint32_t mag, neg;
         signed_digit<GLV>(v, width, mag, neg, carry);
         digits[(size_t)w * n + i] = mag ? ((mag - 1) | ((neg ^ sign) << 31)) : NO_DIGIT;
+    }
+}
+
+// GLV plans on the LDS sort path, in one pass per point: the split, the digits of both halves
+// (virtual points i and n + i) and phi's x-coordinate of base i (what msm_glv_split_kernel,
+// msm_endo_kernel and msm_digits_store_kernel<true> do in three launches, without the halves'
+// round trip through HBM).
+__global__ __launch_bounds__(256) void msm_glv_digits_kernel(const uint32_t* __restrict__ scalars,
+                                                            const uint32_t* __restrict__ bases, uint32_t n,
+                                                            int group, uint32_t W, uint32_t wb, uint32_t nw,
+                                                            uint32_t* __restrict__ endo_x,
+                                                            uint32_t* __restrict__ digits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t nv = 2 * (size_t)n;
+    uint4 h[2];
+    glv_split(scalars, i, &h[0], &h[1]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        uint32_t s[8] = {h[t].x, h[t].y, h[t].z, h[t].w & 0x7fffffffu, 0, 0, 0, 0};
+        const uint32_t sign = h[t].w >> 31, v_idx = i + (t ? n : 0u);
+        uint32_t carry = 0;
+        for (uint32_t w = 0; w < W; ++w) {
+            const uint32_t width = window_width(w, wb, nw);
+            const uint32_t v = window_bits(s, window_pos(w, wb, nw), width) + carry;
+            uint32_t mag, neg;
+            signed_digit<true>(v, width, mag, neg, carry);
+            digits[(size_t)w * nv + v_idx] = mag ? ((mag - 1) | ((neg ^ sign) << 31)) : NO_DIGIT;
+        }
+    }
+    Fq beta;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) beta.w[j] = group == 1 ? GlvConst::beta_g1(j) : GlvConst::beta_g2(j);
+    for (int k = 0; k < group; ++k) {
+        const Fq x = fp_load<FqParams>(bases + 16 * group * (size_t)i + 8 * k);
+        fp_store(endo_x + 8 * ((size_t)group * i + k), x * beta);
     }
 }
 
@@ -1533,10 +1577,14 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
     // GLS: the table of the bases and their psi images, and phi of both; the stages then read the
     // table. GLV: split every scalar into two signed 127-bit halves and compute phi(P_i) = (beta x_i, y_i).
     const uint32_t* dscal = scalars;
+    // the plain GLV plan on the LDS sort path splits, writes its digits and phi(x) in one kernel
+    const bool glv_fused = p.glv && !p.pre && !p.gls && p.lds_sort;
     if (p.gls) {
         hipLaunchKernelGGL(msm_gls_prepare_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, bases, p.n, p.gls_bases,
                            p.endo_x);
         bases = p.gls_bases;
+    } else if (glv_fused) {
+        // below
     } else if (p.glv) {
         hipLaunchKernelGGL(msm_glv_split_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, scalars, p.n, p.split);
         const size_t nbase = p.base_points();
@@ -1554,6 +1602,9 @@ static void run_stages(const MsmPlan& p, const uint32_t* bases, const uint32_t* 
         if (p.gls)
             hipLaunchKernelGGL(msm_gls_digits_store_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, scalars, p.n,
                                p.W, p.wb, p.nw, dig);
+        else if (glv_fused)
+            hipLaunchKernelGGL(msm_glv_digits_kernel, dim3(ceil_div(p.n, 256)), dim3(256), 0, s, scalars, bases, p.n,
+                               p.group, p.W, p.wb, p.nw, p.endo_x, dig);
         else if (p.pre)
             hipLaunchKernelGGL(msm_digits_split_store_kernel, dim3(ceil_div(2 * (size_t)p.n, 256)), dim3(256), 0, s,
                                dscal, p.n, p.W, p.wb, p.nw, dig);
