@@ -106,3 +106,29 @@ def test_g2_msm_gls_witness_scalars(gpu):
     got = _gls("1", lambda: affine(gpu, Msm.for_group(gpu, Group.G2, n, 13).run(gpu, bb, sb), 2))
     assert np.array_equal(got, exp)
 
+
+
+def test_g2_msm_gls_with_poisoned_workspace(gpu):
+    """Every workspace array (the psi table and the phi table of both halves included) filled with
+    a poison value before the run: the result still equals the oracle's, so no stage reads what
+    it has not written."""
+    from test_gpu_msm import affine
+    from zkgpu import Group, Msm
+
+    n = 2500
+    rng = np.random.default_rng(1740)
+    ks, sc = uniform_fr_words(rng, n), uniform_fr_words(rng, n)
+    bases = O.multiples(2, O.generator_words(2), ks)
+    exp = O.msm(2, bases, sc)
+    bb, sb = Msm.upload_bases(gpu, bases), gpu.buffer(sc)
+    old = os.environ.get("ZKGPU_POISON")
+    try:
+        os.environ["ZKGPU_POISON"] = "1"
+        for c in (11, 13):
+            got = _gls("1", lambda: affine(gpu, Msm.for_group(gpu, Group.G2, n, c).run(gpu, bb, sb), 2))
+            assert np.array_equal(got, exp), c
+    finally:
+        if old is None:
+            os.environ.pop("ZKGPU_POISON", None)
+        else:
+            os.environ["ZKGPU_POISON"] = old

@@ -1533,7 +1533,8 @@ static void poison_workspace(const MsmPlan& p, hipStream_t s) {
     } arrays[] = {
         {p.counts, 4 * S}, {p.cursors, 4 * S}, {p.starts, 4 * (S + 1)}, {p.istarts, 4 * (S + 1)},
         {p.scan_tmp, 4 * (3 * nb + 1)}, {p.split, p.glv ? 32 * (size_t)p.n : 0},
-        {p.endo_x, p.glv ? 2 * p.base_points() * p.aw : 0}, {p.sorted, 4 * (size_t)p.W * p.nv},
+        {p.endo_x, p.glv ? 2 * (p.gls ? 2 * (size_t)p.n : p.base_points()) * p.aw : 0},
+        {p.gls_bases, p.gls ? 4 * 2 * (size_t)p.n * p.aw : 0}, {p.sorted, 4 * (size_t)p.W * p.nv},
         {p.digits, p.lds_sort ? 4 * (size_t)p.W * p.nv : 0}, {p.table, p.lds_sort ? 4 * (size_t)p.W * p.nblk * p.B : 0},
         {p.item_info, 8 * p.max_items}, {p.item_pts, 4 * p.max_items * p.xw}, {p.buckets, 4 * S * p.xw},
         {p.partials, 4 * (size_t)p.W * p.T * p.xw}, {p.wsums, 4 * (size_t)p.W * p.xw}, {p.result, 4 * (size_t)p.xw}};
