@@ -161,9 +161,10 @@ int zkg_msm_convert_bases(zkg_buffer* buf, size_t n_words);
  * smallest such index (0 when *bad = 0). Waits for the device.
  *
  * Why it matters: plans split scalars with the GLV endomorphism phi(x, y) = (beta x, y), which
- * acts as multiplication by lambda only on the order-r subgroup. G1 has cofactor 1 (on the curve
- * is enough), but the G2 twist has cofactor 2p - r: a G2 base on the twist but outside the
- * subgroup gives a different sum than the reference's plain Pippenger. Bases of a Groth16 proving
+ * acts as multiplication by lambda only on the order-r subgroup, and G2 plans up to 2^18 points
+ * also with psi (the Frobenius carried to the twist), which acts as p mod r only there. G1 has
+ * cofactor 1 (on the curve is enough), but the G2 twist has cofactor 2p - r: a G2 base on the
+ * twist but outside the subgroup gives a different sum than the reference's plain Pippenger. Bases of a Groth16 proving
  * key are subgroup points; zkg_zkey_upload and zkg_server_add_zkey run this check themselves. */
 int zkg_msm_check_bases(zkg_device* dev, int group, const zkg_buffer* bases, size_t n, uint64_t* bad,
                         uint64_t* first_bad);
@@ -171,8 +172,8 @@ int zkg_msm_check_bases(zkg_device* dev, int group, const zkg_buffer* bases, siz
  * NULL; zkg_msm_result gives the same handle at any time) receives the plan-owned buffer holding one XYZZ point (32 words G1, 64 words G2, Montgomery form), valid
  * until the next run on this plan. Errors: "scalars buffer holds ..." / "bases buffer holds ...".
  * Precondition (not checked here, see zkg_msm_check_bases): G2 bases lie in the order-r subgroup.
- * ZKGPU_MSM_GLV=0 in the environment at plan creation selects plans without the GLV split, which
- * sum any points of the twist exactly as the reference does. */
+ * ZKGPU_MSM_GLV=0 in the environment at plan creation selects plans without the GLV (and GLS)
+ * split, which sum any points of the twist exactly as the reference does. */
 int zkg_msm_run(zkg_msm* plan, const zkg_buffer* bases, const zkg_buffer* scalars, zkg_buffer** result);
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): the MSM that
