@@ -1466,7 +1466,8 @@ uint32_t msm_default_sort_points(size_t nv, uint32_t c) {
 
 // Buckets per lane of the bucket reduction: its serial chain is 2L additions plus the workgroup
 // tree, against W B / L lanes in all. Measured on MI355X: L = 4 for G1 (2^20: 2.90 ms, 3.00 with
-// L = 2; over precomputed bases at c = 16, 131072 buckets, L = 2 and 4 tie). G2 runs the
+// L = 2; over precomputed bases at c = 16, 131072 buckets, L = 2 and 4 tie; on lane pairs at 2^20,
+// c = 15, L = 2 doubles the busy waves to 1408 for 1024 SIMDs: reduce 0.19-0.20 -> 0.28-0.31 ms). G2 runs the
 // reduction on four lanes per point, and L = 4 there too (2^18, c = 13: reduce 0.46 -> 0.35 ms
 // against L = 2, which put a second wave on some SIMDs; precomputed 2^18 at c = 16: 1.05 -> 0.65
 // ms).
