@@ -132,3 +132,35 @@ def test_g2_msm_gls_with_poisoned_workspace(gpu):
             os.environ.pop("ZKGPU_POISON", None)
         else:
             os.environ["ZKGPU_POISON"] = old
+
+
+def test_g2_msm_gls_seeded_fuzz(gpu):
+    """Seeded random G2 MSMs on the GLS path: sizes 1..6000, every window that selects it, scalar
+    mixes (uniform, small, witness-like, multiples of lambda and mu, values near r) and repeated
+    bases, each against the oracle."""
+    from test_gpu_msm import affine
+    from zkgpu import Group, Msm
+
+    rng = np.random.default_rng(1750)
+    mu = O.P % R
+    for case in range(24):
+        n = int(rng.choice([1, 2, 5, 63, 64, 65, 511, 1500, 3000, 6000]))
+        c = int(rng.choice([11, 13, 14]))
+        ks = uniform_fr_words(rng, n)
+        kind = case % 4
+        if kind == 0:
+            sc = uniform_fr_words(rng, n)
+        elif kind == 1:
+            sc = witness_scalars(rng, n)
+        elif kind == 2:  # multiples of the eigenvalues and values near r
+            vals = [(int(rng.integers(0, 1 << 62)) * int(rng.choice([1, LAM, mu, LAM * mu % R])) +
+                     int(rng.integers(-3, 4))) % R for _ in range(n)]
+            sc = np.concatenate([O.int_to_words(v) for v in vals])
+        else:  # repeated bases with random scalars
+            ks = np.tile(ks[:8], n)
+            sc = uniform_fr_words(rng, n)
+        bases = O.multiples(2, O.generator_words(2), ks)
+        exp = O.msm(2, bases, sc)
+        bb, sb = Msm.upload_bases(gpu, bases), gpu.buffer(sc)
+        got = _gls("1", lambda: affine(gpu, Msm.for_group(gpu, Group.G2, n, c).run(gpu, bb, sb), 2))
+        assert np.array_equal(got, exp), (case, n, c, kind)
