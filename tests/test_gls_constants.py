@@ -170,10 +170,10 @@ def test_g2_plan_shapes_every_window():
             assert Msm.plan_check(Group.G2, n, c) > 0, (n, c)
 
 
-def test_gls_is_chosen_where_it_takes_no_more_digit_windows():
+def test_gls_is_chosen_for_g2_up_to_2_18_points():
     """G2 plans use the GLS split (a different workspace: the psi table, no GLV halves) exactly at
-    c = 11, 13, 14 and n <= 2^18, with the GLV split enabled; G1 never. Checked on the host through
-    the workspace sizes, with ZKGPU_MSM_GLS=0 as the comparison."""
+    c <= 16 and n <= 2^18, with the GLV split enabled; G1 never. Checked on the host through the
+    workspace sizes, with ZKGPU_MSM_GLS=0 as the comparison."""
     from zkgpu import Group, Msm
 
     def sizes(group, n, c):
@@ -193,7 +193,7 @@ def test_gls_is_chosen_where_it_takes_no_more_digit_windows():
     for n in (1, 3000, 1 << 16, 1 << 18, (1 << 18) + 1, 1 << 20):
         for c in range(9, 21):
             gls, glv = sizes(Group.G2, n, c)
-            expect = n <= 1 << 18 and c in (11, 13, 14)
+            expect = n <= 1 << 18 and c <= 16
             assert (gls != glv) == expect, (n, c)
             g1_on, g1_off = sizes(Group.G1, n, c)
             assert g1_on == g1_off

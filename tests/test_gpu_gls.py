@@ -1,7 +1,7 @@
 """The G2 MSM's GLS path (four coefficients below 2^64 of Q, phi(Q), psi(Q), phi(psi(Q)), where
 psi is the Frobenius carried to the twist): the device split against its defining properties and
 the kernel's integer steps restated in Python, and G2 MSMs with GLS on and off (ZKGPU_MSM_GLS=0,
-the GLV split alone) against the oracle at every window that selects it (c = 11, 13, 14), with
+the GLV split alone) against the oracle at every window (c = 9..16: 8, 7, 6 or 5 windows over 65 bits), with
 special scalars, infinity and repeated bases, and witness-like scalars."""
 import os
 
@@ -52,7 +52,7 @@ def test_device_split_properties(gpu):
 
 
 @pytest.mark.parametrize("n,c", [(1, 11), (2, 13), (3, 14), (255, 11), (1000, 13), (4000, 14), (7000, 11),
-                                 (20000, 13)])
+                                 (20000, 13), (3000, 9), (3000, 10), (5000, 12), (5000, 15), (6000, 16)])
 def test_g2_msm_gls_on_and_off_match_oracle(gpu, n, c):
     from test_gpu_msm import affine
     from zkgpu import Group, Msm
@@ -145,7 +145,7 @@ def test_g2_msm_gls_seeded_fuzz(gpu):
     mu = O.P % R
     for case in range(24):
         n = int(rng.choice([1, 2, 5, 63, 64, 65, 511, 1500, 3000, 6000]))
-        c = int(rng.choice([11, 13, 14]))
+        c = int(rng.choice([9, 10, 11, 12, 13, 14, 15, 16]))
         ks = uniform_fr_words(rng, n)
         kind = case % 4
         if kind == 0:

@@ -47,7 +47,8 @@ def test_profile_names_and_result(gpu, group, n, c, item_points, witness):
     assert plan.group() == Group(group)
     prof = plan.profile(gpu, bb, sb)
     names = [name for name, _ in prof]
-    nv = 2 * n  # virtual points of the GLV split
+    # virtual points: 4n for the G2 GLS split (c <= 16, n <= 2^18), else 2n for the GLV split
+    nv = 4 * n if group == 2 and c <= 16 and n <= 1 << 18 else 2 * n
     assert names == _names(_merge_rounds(nv, plan.item_points())), names
     assert all(ms >= 0 for _, ms in prof)
     if item_points == 1:

@@ -1194,19 +1194,23 @@ MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uin
         if (!sh.glv || npts > 0x1fffffffu)
             throw Error(ZKG_ERR_INVALID, "a plan over precomputed bases needs the GLV split and n < 2^29");
     }
-    // GLS where its windows are no more than the GLV split's (c = 11, 13, 14) and n <= 2^18: the
-    // accumulation gathers from a table of 2n points and 2n phi x-coordinates (384 bytes per base,
-    // twice the GLV plan's), and beyond 2^18 bases those gathers fall out of the caches. Measured
-    // (G2, uniform scalars, best c of each): 2^16 1.53 ms against 1.84 for the GLV plan, 2^18 3.08
-    // against 3.38; 2^19 at c = 14 5.63 against 5.02 and 2^20 accumulate 8.99 ms against 6.49.
-    sh.gls = group == 2 && sh.glv && !sh.pre && c <= 16 && npts <= ((size_t)1 << 18) && msm_gls_enabled() &&
-             2 * ((65 + c - 1) / c) <= (128 + c - 1) / c;
+    // GLS for G2 at every c <= 16 (the windows of the LDS-sort path) and n <= 2^18: the accumulation
+    // gathers from a table of 2n points and 2n phi x-coordinates (384 bytes per base, twice the GLV
+    // plan's), and beyond 2^18 bases those gathers fall out of the caches (2^19 at c = 14: 5.63 ms
+    // against 5.02 for the GLV plan; 2^20 accumulate 8.99 ms against 6.49).
+    sh.gls = group == 2 && sh.glv && !sh.pre && c <= 16 && npts <= ((size_t)1 << 18) && msm_gls_enabled();
     const uint64_t n = npts;
     const uint64_t nv = sh.pre || sh.gls ? 4 * n : sh.glv ? 2 * n : n;
     // bits per (half-)scalar the windows cover: |k| < 2^63.7 for GLS, so 65 bits of signed digits
     const uint32_t sbits = sh.pre ? 64 : sh.gls ? 65 : 128;
     sh.nv = (uint32_t)nv;
-    sh.W = sh.glv ? (sbits + c - 1) / c : (255 + c - 1) / c;
+    // GLS windows: the balanced layouts of 65 bits have 9, 10, 11 or 13 digit bits (8, 7, 6 or 5
+    // windows); c up to 11 takes its own width, c from 12 on five windows of 13 bits. Measured (G2,
+    // uniform scalars, against the GLV plan at the same c): 2^12 c = 10 0.96 vs 1.23 ms, 2^16
+    // c = 13 1.44 vs 1.71, 2^17 c = 12 2.03 vs 2.45 (with six windows of 11 bits 2.72: the reason
+    // c = 12 takes the 13-bit layout), 2^18 c = 13 2.94 vs 3.19, c = 15 2.91 vs 3.25, c = 16 2.93
+    // vs 3.74.
+    sh.W = sh.gls ? (c <= 11 ? (sbits + c - 1) / c : 5) : sh.glv ? (sbits + c - 1) / c : (255 + c - 1) / c;
     sh.wb = sh.glv ? sbits / sh.W : c;
     sh.nw = sh.glv ? sbits - sh.wb * sh.W : 0;
     const uint32_t cw = sh.gls ? sh.wb + (sh.nw ? 1 : 0) : c;

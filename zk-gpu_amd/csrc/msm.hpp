@@ -23,8 +23,8 @@ uint32_t msm_plan_item_points(int group, size_t n, uint32_t c);
 // reach 2^32 is rejected here (an Error with a message), before anything is allocated.
 struct MsmShape {
     bool glv = false, pre = false, gls = false, lds_sort = false;
-    // cw: digit bits of the widest window (B = 2^(cw - 1)); c, except where a GLS plan's windows
-    // are narrower (65 bits in five windows of 13 at c = 14)
+    // cw: digit bits of the widest window (B = 2^(cw - 1)); c, except for GLS plans (65 bits in
+    // windows of 9, 10, 11 or 13 bits)
     uint32_t nv = 0, W = 0, wb = 0, nw = 0, cw = 0, B = 0, CH = 0, L = 0, T = 0, aw = 0, xw = 0;
     size_t max_items = 0;
     uint32_t merge_rounds = 0, sort_pts = 0, nblk = 0;
@@ -81,16 +81,17 @@ struct MsmPlan {
     bool pre = false;
     // GLS (G2): psi(x, y) = (conj(x) gx, conj(y) gy) acts on G2 as mu = p mod r, so each scalar splits
     // into four signed coefficients below 2^64 (msm.hip, GLS split) of Q, phi(Q), psi(Q) and
-    // phi(psi(Q)): nv = 4n virtual points over ceil(65 / c) windows. The run first writes a table
-    // of the bases and their psi images (gls_bases) and the phi images of both (endo_x), and the
-    // stages then run as over a precomputed table. Used where it takes no more windows of digits
-    // than the GLV split (2 ceil(65 / c) <= ceil(128 / c): c = 11, 13, 14): the same bucket
-    // additions, half the buckets to finalize and reduce, and a doubling chain of 65 - c bits; and
-    // for n <= 2^18, where the gathers from its larger table stay in the caches (msm_shape).
+    // phi(psi(Q)): nv = 4n virtual points over 8, 7, 6 or 5 windows of 65 bits (9, 10, 11 or 13
+    // digit bits: c up to 11 its own, c from 12 on 13). The run first writes a table of the bases
+    // and their psi images (gls_bases) and the phi images of both (endo_x), and the stages then
+    // run as over a precomputed table: about as many bucket additions as the GLV split, half the
+    // buckets to finalize and reduce, and a doubling chain of about 52 bits instead of 115. Used
+    // for G2 at c <= 16 and n <= 2^18, where the gathers from its larger table stay in the caches
+    // (msm_shape).
     bool gls = false;
     bool f29 = false;        // 29-bit-limb field (field29.hpp)
     uint32_t nv = 0;         // virtual points: 4n precomputed or GLS, 2n with GLV, else n
-    uint32_t W = 0;          // windows: ceil(64 / c) precomputed, ceil(65 / c) GLS, ceil(128 / c) GLV, else ceil(255 / c)
+    uint32_t W = 0;          // windows: ceil(64 / c) precomputed, 8..5 GLS, ceil(128 / c) GLV, else ceil(255 / c)
     uint32_t cw = 0;         // digit bits of the widest window (c, or less for GLS: MsmShape::cw)
     // Window w covers bits [w*wb + min(w, nw), +wb + (w < nw)): without GLV wb = c, nw = 0; with
     // GLV the 128 bits are split evenly (c = 15: two windows of 15 bits and seven of 14), so no
