@@ -1,0 +1,250 @@
+"""The G1 combine's doubling chain with one 29-bit limb per lane (zk-gpu_amd/csrc/msm_kernels.hpp,
+namespace lpl), restated lane for lane: a wave of 64 lanes in four rows of 16, the DPP row moves
+(row_shr, row_shl, row_newbcast, zero for lanes with no source) and the gfx950 row swaps
+(v_permlane16_swap, v_permlane32_swap) as the ISA defines them. Every 32-bit lane value and
+64-bit column is checked against its register width, the product against Montgomery
+multiplication by 2^-261 mod p (random, lazy and worst-case limbs), the doubling's value bounds
+(X < 9p in, X < 8p, Y < 5p, ZZ, ZZZ < 2p out) for a fixed point, and chains of doublings against the
+textbook affine law. CPU only."""
+import os
+import random
+import re
+
+import pytest
+
+import oracle as O
+
+P = O.P
+R = 1 << 261
+MASK = (1 << 29) - 1
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KERNELS = open(os.path.join(ROOT, "zk-gpu_amd", "csrc", "msm_kernels.hpp")).read()
+FIELD29 = open(os.path.join(ROOT, "zk-gpu_amd", "csrc", "field29.hpp")).read()
+LANES = 64
+
+
+def _table(src, pattern):
+    m = re.search(pattern, src, re.S)
+    return [int(x, 16) for x in re.findall(r"0x([0-9a-f]{8})u", m.group(1))]
+
+
+PINV = _table(KERNELS, r"namespace lpl \{.*?constexpr uint32_t PINV\(int i\) \{(.*?)\}")
+PL = _table(FIELD29, r"constexpr uint32_t P\(int i\) \{  // p(.*?)\}")
+
+
+def _kp(c):
+    return _table(FIELD29, r"K<%d>\(int i\) \{(.*?)\}" % c)
+
+
+def test_constants():
+    assert sum(v << (29 * i) for i, v in enumerate(PL)) == P
+    assert sum(v << (29 * i) for i, v in enumerate(PINV)) == (-pow(P, -1, R)) % R
+    for c in (3, 6, 10):
+        assert sum(v << (29 * i) for i, v in enumerate(_kp(c))) == c * P
+
+
+# ---- the wave: lists of 64 lane values
+def rl(l):
+    return l & 15
+
+
+def w32(v):
+    assert all(0 <= x < 1 << 32 for x in v), hex(max(v))
+    return v
+
+
+def w64(v):
+    assert all(0 <= x < 1 << 64 for x in v), hex(max(v))
+    return v
+
+
+def shr(v, k):
+    return [v[l - k] if rl(l) >= k else 0 for l in range(LANES)]
+
+
+def shl(v, k):
+    return [v[l + k] if rl(l) + k <= 15 else 0 for l in range(LANES)]
+
+
+def bc(v, k):
+    return [v[(l & ~15) + k] for l in range(LANES)]
+
+
+def swap16(a, b):  # odd rows of a <-> even rows of b
+    a2, b2 = a[:], b[:]
+    for r in (0, 2):
+        for i in range(16):
+            a2[(r + 1) * 16 + i], b2[r * 16 + i] = b[r * 16 + i], a[(r + 1) * 16 + i]
+    return a2, b2
+
+
+def swap32(a, b):  # rows 2, 3 of a <-> rows 0, 1 of b
+    a2, b2 = a[:], b[:]
+    for i in range(32):
+        a2[32 + i], b2[i] = b[i], a[32 + i]
+    return a2, b2
+
+
+def rows(v):  # lpl::rows: each row's value over all four rows
+    s0, s1 = swap16(v, v)
+    e0, e1 = swap32(s0, s0)
+    o0, o1 = swap32(s1, s1)
+    return e0, o0, e1, o1
+
+
+def columns(acc, x, ys):
+    for k in range(9):
+        xk = shr(x, k)
+        acc = w64([acc[l] + xk[l] * ys[k][l] for l in range(LANES)])
+    return acc
+
+
+def carry2(c):
+    hi = [x >> 29 for x in c]
+    c1 = w64([(x & MASK) + y for x, y in zip(c, shr(hi, 1))])
+    return w32([(x & MASK) + y for x, y in zip(c1, shr([x >> 29 for x in c1], 1))])
+
+
+def mul(a, b, stats):
+    """lpl::mul, step for step."""
+    w32(a), w32(b)
+    t = columns([0] * LANES, a, [bc(b, k) for k in range(9)])
+    h = w64([x * y for x, y in zip(bc(a, 8), bc(b, 8))])
+    tn = carry2(t)
+    q = carry2(columns([0] * LANES, tn, [[PINV[k]] * LANES for k in range(9)]))
+    q = [q[l] if rl(l) < 8 else (q[l] & MASK if rl(l) == 8 else 0) for l in range(LANES)]
+    u = columns(t, q, [[PL[k]] * LANES for k in range(9)])
+    h = w64([x + y * PL[8] for x, y in zip(h, bc(q, 8))])
+    hi = [x >> 29 for x in u]
+    c1 = w64([(x & MASK) + y for x, y in zip(u, shr(hi, 1))])
+    h = w64([x + y for x, y in zip(h, bc(hi, 15))])
+    hi2 = w32([x >> 29 for x in c1])
+    uu = w32([(x & MASK) + y for x, y in zip(c1, shr(hi2, 1))])
+    h = w64([x + y for x, y in zip(h, bc(hi2, 15))])
+    stats["h"] = max(stats.get("h", 0), max(h))
+    nz = [any(uu[r * 16 + i] for i in range(9)) for r in range(4)]
+    r = shl(uu, 9)
+    out = []
+    for l in range(LANES):
+        v = r[l] + (1 if rl(l) == 0 and nz[l >> 4] else 0)
+        if rl(l) == 7:
+            v = h[l] & MASK
+        if rl(l) == 8:
+            v = h[l] >> 29
+        out.append(v)
+    return w32(out)
+
+
+def carry1(v):
+    out = w32([(x & MASK) + y for x, y in zip(v, shr([x >> 29 for x in v], 1))])
+    assert all(out[l] == 0 for l in range(LANES) if rl(l) >= 9)  # no carry leaves lane 8
+    return out
+
+
+def ksub(a, b, c):  # a + c p - b in borrowed limbs (b's limbs below 2^31 - 4)
+    k = _kp(c)
+    assert all(x < (1 << 31) - 4 for x in b)
+    return carry1(w32([x + (k[rl(l)] if rl(l) < 9 else 0) - y for l, (x, y) in enumerate(zip(a, b))]))
+
+
+def add(a, b):
+    return w32([x + y for x, y in zip(a, b)])
+
+
+def pick(*vs):
+    return [vs[l >> 4][l] for l in range(LANES)]
+
+
+def dbl(p, stats):
+    """lpl::dbl: three rounds of four products."""
+    x, y, zz, zzz = p
+    u = add(y, y)
+    v, x2, uy, uz = rows(mul(pick(u, x, u, u), pick(u, x, y, zzz), stats))
+    m = carry1(add(add(x2, x2), x2))
+    s, zz3, m2, wy = rows(mul(pick(x, v, m, v), pick(v, zz, m, uy), stats))
+    x3 = ksub(m2, add(s, s), 6)
+    d = ksub(s, x3, 10)
+    md, zzz3, _, _ = rows(mul(pick(m, v, m, v), pick(d, uz, d, uz), stats))
+    return x3, ksub(md, wy, 3), zz3, zzz3
+
+
+def row_value(v, r):
+    return sum(v[r * 16 + i] << (29 * i) for i in range(16))
+
+
+def replicated(x):
+    l9 = [(x >> (29 * i)) & MASK for i in range(8)] + [x >> 232]
+    return [l9[rl(l)] if rl(l) < 9 else 0 for l in range(LANES)]
+
+
+def lazy_limbs(x, rng, top=(1 << 29) + (1 << 7)):
+    """Limbs 0..7 in [0, top), value at most x and close to it."""
+    while True:
+        l = [rng.randrange(top) for _ in range(8)]
+        low = sum(v << (29 * i) for i, v in enumerate(l))
+        if low <= x:
+            return l + [(x - low) >> 232]
+
+
+def test_product_random_and_lazy():
+    rng = random.Random(261)
+    stats = {}
+    for it in range(120):
+        a_l, b_l, vals = [0] * LANES, [0] * LANES, []
+        for r in range(4):
+            ka, kb = rng.choice([(10, 10), (9, 9), (10, 5), (6, 12), (12, 12), (2, 2)])
+            a, b = (ka * P - 1, kb * P - 1) if it % 5 == 0 else (rng.randrange(ka * P), rng.randrange(kb * P))
+            la, lb = lazy_limbs(a, rng), lazy_limbs(b, rng)
+            for i in range(9):
+                a_l[r * 16 + i], b_l[r * 16 + i] = la[i], lb[i]
+            vals.append((row_value(a_l, r), row_value(b_l, r)))
+        out = mul(a_l, b_l, stats)
+        for r, (a, b) in enumerate(vals):
+            v = row_value(out, r)
+            assert v % P == a * b * pow(R, -1, P) % P
+            assert v < a * b // R + P + (P >> 40)
+            assert all(out[r * 16 + i] < (1 << 29) + (1 << 7) + 2 for i in range(8))
+    assert stats["h"] < 1 << 53
+
+
+@pytest.mark.parametrize("bound", [10, 12])
+def test_product_worst_case_limbs(bound):
+    # U = 2Y: limbs up to 2 (2^29 + 2^7 + 1), every lane at its maximum
+    top = 2 * ((1 << 29) + (1 << 7) + 1)
+    a_l = [0] * LANES
+    for r in range(4):
+        low = sum(top << (29 * i) for i in range(8))
+        for i in range(8):
+            a_l[r * 16 + i] = top
+        a_l[r * 16 + 8] = (bound * P - 1 - low) >> 232
+    a = row_value(a_l, 0)
+    assert a < bound * P
+    out = mul(a_l, a_l, {})
+    assert row_value(out, 0) % P == a * a * pow(R, -1, P) % P and row_value(out, 0) < 2 * P
+
+
+def _xyzz(pt, extra):
+    Rm = R % P
+    return [replicated(v + e * P) for v, e in zip((pt[0] * Rm % P, pt[1] * Rm % P, Rm, Rm), extra)]
+
+
+def _affine(p):
+    X, Y, ZZ, ZZZ = (row_value(v, 0) % P for v in p)
+    return X * pow(ZZ, -1, P) % P, Y * pow(ZZZ, -1, P) % P
+
+
+def test_doubling_chain_matches_textbook_at_the_bounds():
+    rng = random.Random(2610)
+    for trial in range(4):
+        a = O.py_g1_mul(O.G1, rng.randrange(1, O.R))
+        # the running point as an addition leaves it (X < 9p) with Y, ZZ, ZZZ at their bounds
+        p = _xyzz(a, (8, 4, 1, 1))
+        exp = a
+        for k in range(15):
+            p = dbl(p, {})
+            exp = O.py_g1_add(exp, exp)
+            for r in range(1, 4):  # every coordinate the same on all four rows
+                assert all(row_value(c, r) == row_value(c, 0) for c in p)
+            for c, bound in zip(p, (8, 5, 2, 2)):
+                assert row_value(c, 0) < bound * P
+            assert _affine(p) == exp

@@ -973,60 +973,181 @@ ZKG_DEV Xyzz29 xyzz29_dbl_lanes(const Xyzz29& p, uint32_t lane) {
     return Xyzz29{x3, y3, zz3, f29_shfl(m, 2)};
 }
 
-// The doubling chain with fixed lane roles: between doublings lane 0 holds Y, lane 1 X and lane 2
-// ZZ and ZZZ (other fields, and lanes 3..63, hold values no one reads), and each product's lane
-// fetches only its own operands, in four rounds of DPP moves within lanes 0..3 per doubling
-// instead of the nine full broadcasts of xyzz29_dbl_lanes (LDS round trips had been a third of
-// the chain's cycles). The formulas, their order and hence the bounds are those of
-// xyzz29_dbl_lanes:
-//   L1  lane 0: V = U^2 (U = 2Y)   lane 1: X^2
-//       round 1: lanes 0, 3 <- X^2 (lane 1); lanes 1, 2 <- V (lane 0)
-//   L2  lane 0: W = U V   lane 1: S = X V   lane 2: ZZ3 = V ZZ   lane 3: M^2 (M = 3 X^2)
-//       round 2: lane 0 <- S (lane 1); lanes 1, 2 <- W (lane 0)
-//       round 3: lane 0 <- M^2 (lane 3); lane 1 <- Y (lane 0)
-//   lane 0: X3 = M^2 - 2S, D = S - X3
-//   L3  lane 0: M D   lane 1: W Y   lane 2: ZZZ3 = W ZZZ
-//       round 4: lane 0 <- W Y (lane 1), Y3 = M D - W Y; lane 1 <- X3 (lane 0)
-struct DblRoles {
-    uint32_t lane;
-    // the lane each round reads from, as quad_perm controls: round 1 (1, 0, 0, 1), round 2 and 4
-    // (1, 0, 0, 0), round 3 (3, 0, 0, 0)
-    static constexpr int R1 = 0x41, R2 = 0x01, R3 = 0x03, R4 = 0x01;
-    explicit ZKG_DEV DblRoles(uint32_t l) : lane(l) {}
+// ---------------------------------------------------------------- limb-per-lane doubling chain
+// The combine's doublings run on one wave, so their cost is the instruction count of one dependent
+// chain. Here each field element is one VGPR: lane i of a 16-lane row holds limb i (lanes 9..15
+// hold 0), and the four rows of the wave carry four independent products. A product then costs
+// about 115 instructions for four (nine multiply-adds per column, the operands moved by DPP row
+// shifts and row broadcasts) instead of about 230 for each product held on one lane.
+//
+// Product (Montgomery, R = 2^261, same contract as f29_mul: limbs < 2^30 + 2^8, v(a) v(b) below
+// 169 p^2, result below a b / 2^261 + p (1 + 2^-50), limbs 0..7 below 2^29 + 2^7 + 1):
+//   T = a b as seventeen 64-bit columns, column j on lane j (j < 16) and column 16 as H, a value
+//       every lane of the row holds (each column at most nine products below 2^60.1);
+//   q = (T mod 2^261) (-p^-1) mod 2^261, from T's low nine columns after two carry rounds;
+//   U = T + q p, whose low 261 bits are 0 mod 2^261; after two carry rounds (lane 15's carries
+//       go into H) limbs 0..8 are below 2^30, so their value is 0 or exactly 2^261, and it is
+//       2^261 iff one of them is non-zero: that bit is the carry into the result's limb 0;
+//   result = (U_9 .. U_15 moved down nine lanes, H's low 29 bits, H's high bits) + carry.
+// H stays below 2^53 because H 2^203 is at most the result (below 3p). Every bound was checked on
+// a lane-level model of these steps (random and extreme limbs against integer arithmetic).
+namespace lpl {
+
+template <int CTRL>
+ZKG_DEV uint32_t dpp(uint32_t v) {  // lanes with no source lane in the row read 0 (bound_ctrl)
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
+}
+template <int K>
+ZKG_DEV uint32_t shr(uint32_t v) {  // lane i of a row takes lane i - K (row_shr)
+    if constexpr (K == 0) return v;
+    else return dpp<0x110 + K>(v);
+}
+template <int K>
+ZKG_DEV uint32_t bc(uint32_t v) {  // every lane of a row takes the row's lane K (row_newbcast)
+    return dpp<0x150 + K>(v);
+}
+ZKG_DEV uint64_t shr1_64(uint64_t v) {
+    return ((uint64_t)shr<1>((uint32_t)(v >> 32)) << 32) | shr<1>((uint32_t)v);
+}
+ZKG_DEV uint64_t bc15_64(uint64_t v) {
+    return ((uint64_t)bc<15>((uint32_t)(v >> 32)) << 32) | bc<15>((uint32_t)v);
+}
+
+// -p^-1 mod 2^261 in 29-bit limbs
+ZKG_DEV constexpr uint32_t PINV(int i) {
+    return i == 0 ? 0x04866389u : i == 1 ? 0x1e903c17u : i == 2 ? 0x129ab261u : i == 3 ? 0x1cfaca3du
+         : i == 4 ? 0x1da809edu : i == 5 ? 0x05e80c19u : i == 6 ? 0x11af62bfu : i == 7 ? 0x16f23111u : 0x0ff57a22u;
+}
+
+// acc + sum over k of shr<k>(x) * y_k, with y_k = bc<k>(b) or the constant limbs C(k) (one chain of
+// multiply-adds: three independent chains measured no faster)
+template <int K = 0, class Y>
+ZKG_DEV uint64_t columns(uint64_t acc, uint32_t x, const Y& y) {
+    acc += (uint64_t)shr<K>(x) * y.template get<K>();
+    if constexpr (K < 8) return columns<K + 1>(acc, x, y);
+    else return acc;
+}
+struct BcastOf {
+    uint32_t b;
+    template <int K>
+    ZKG_DEV uint32_t get() const { return bc<K>(b); }
+};
+struct PinvLimbs {
+    template <int K>
+    ZKG_DEV uint32_t get() const { return PINV(K); }
+};
+struct PLimbs {
+    template <int K>
+    ZKG_DEV uint32_t get() const { return f29::P(K); }
 };
 
-ZKG_DEV Xyzz29 xyzz29_dbl_roles(const Xyzz29& p, const DblRoles& ro) {
-    const uint32_t lane = ro.lane;
-    const F29 u = f29_add(p.y, p.y);                        // lane 0: < 10, limbs < 2^30
-    const F29 o1 = f29_sel(lane == 0, u, p.x);
-    const F29 m1 = f29_mul(o1, o1);                         // lane 0: V, lane 1: X^2 (10 * 10, 9 * 9) -> < 2
-    const F29 r1 = f29_dpp<DblRoles::R1>(m1);               // lanes 0, 3: X^2; lanes 1, 2: V
-    const F29 mm = f29_carry(f29_add(f29_add(r1, r1), r1));  // lanes 0, 3: M = 3 X^2 < 6, normalised
-    const F29 a2 = f29_pick(lane, u, p.x, p.zz, mm);
-    const F29 b2 = f29_pick(lane, m1, r1, r1, mm);
-    const F29 m2 = f29_mul(a2, b2);                         // W, S, ZZ3, M^2 (10 * 2, 9 * 2, 2 * 2, 6 * 6) -> < 2
-    const F29 r2 = f29_dpp<DblRoles::R2>(m2);               // lane 0: S; lanes 1, 2: W
-    const F29 q3 = f29_sel(lane == 3, m2, p.y);
-    const F29 r3 = f29_dpp<DblRoles::R3>(q3);               // lane 0: M^2; lane 1: Y
-    const F29 x3 = f29_sub<6>(r3, f29_add(r2, r2));         // lane 0: 2S < 4 < 5: X3 < 8
-    const F29 d = f29_sub<10>(r2, x3);                      // lane 0: < 12
-    const F29 a3 = f29_sel(lane == 0, mm, r2);
-    const F29 b3 = f29_pick(lane, d, r3, p.zzz);
-    const F29 m3 = f29_mul(a3, b3);                         // M D, W Y, ZZZ3 (6 * 12, 2 * 5, 2 * 2) -> < 2
-    const F29 q4 = f29_sel(lane == 0, x3, m3);
-    const F29 r4 = f29_dpp<DblRoles::R4>(q4);               // lane 0: W Y; lane 1: X3
-    Xyzz29 o;
-    o.y = f29_sub<3>(m3, r4);                               // lane 0: Y3 = M D - W Y < 5
-    o.x = r4;                                               // lane 1: X3
-    o.zz = m2;                                              // lane 2: ZZ3
-    o.zzz = m3;                                             // lane 2: ZZZ3
-    return o;
+// two carry rounds of 64-bit columns into limbs below 2^29 + 2^7 (carries out of lane 15 dropped)
+ZKG_DEV uint32_t carry2(uint64_t c) {
+    const uint64_t c1 = (c & f29::MASK) + shr1_64(c >> 29);  // < 2^29 + 2^35
+    return ((uint32_t)c1 & f29::MASK) + shr<1>((uint32_t)(c1 >> 29));
 }
 
-// The whole point on every lane from the roles' lanes (Y: lane 0, X: lane 1, ZZ, ZZZ: lane 2).
-ZKG_DEV Xyzz29 xyzz29_from_roles(const Xyzz29& p) {
-    return Xyzz29{f29_shfl(p.x, 1), f29_shfl(p.y, 0), f29_shfl(p.zz, 2), f29_shfl(p.zzz, 2)};
+// Where a lane sits: its row (0..3) and its lane within the row, as the flags the selects use
+// (written as two-way selects so that the compiler keeps them branch-free).
+struct Where {
+    uint32_t lane, rl;
+    bool row_b0, row_b1, low8, is8, is7, is0;
+    ZKG_DEV explicit Where(uint32_t l)
+        : lane(l), rl(l & 15), row_b0((l & 16) != 0), row_b1((l & 32) != 0), low8((l & 15) < 8), is8((l & 15) == 8),
+          is7((l & 15) == 7), is0((l & 15) == 0) {}
+};
+
+// limbs 0..7 of the result below 2^29 + 2^7 + 1, limb 8 below 2^24
+ZKG_DEV uint32_t mul(uint32_t a, uint32_t b, const Where& at) {
+    const uint64_t t = columns(0, a, BcastOf{b});
+    uint64_t h = (uint64_t)bc<8>(a) * bc<8>(b);               // column 16
+    const uint32_t tn = carry2(t);                             // = T mod 2^261 on lanes 0..8
+    uint32_t q = carry2(columns(0, tn, PinvLimbs{}));
+    q = at.low8 ? q : (at.is8 ? q & f29::MASK : 0u);           // q mod 2^261, lanes 9..15 cleared
+    const uint64_t u = columns(t, q, PLimbs{});
+    h += (uint64_t)bc<8>(q) * f29::P(8);
+    const uint64_t hi = u >> 29;
+    const uint64_t c1 = (u & f29::MASK) + shr1_64(hi);
+    h += bc15_64(hi);
+    const uint32_t hi2 = (uint32_t)(c1 >> 29);
+    const uint32_t uu = ((uint32_t)c1 & f29::MASK) + shr<1>(hi2);
+    h += bc<15>(hi2);
+    // per row: is one of lanes 0..8 non-zero? (bit 9 of each row's 16-bit group after adding 0x1ff)
+    constexpr uint64_t LOW9 = 0x01ff01ff01ff01ffull;
+    const uint64_t nz = (__builtin_amdgcn_ballot_w64(uu != 0) & LOW9) + LOW9;
+    const uint32_t carry = at.is0 ? (uint32_t)((nz >> 9) >> at.lane) & 1u : 0u;
+    const uint32_t r = dpp<0x100 + 9>(uu) + carry;             // row_shl:9: limbs 9..15 to lanes 0..6
+    const uint32_t r78 = at.is7 ? (uint32_t)h & f29::MASK : (uint32_t)(h >> 29);
+    return at.is7 || at.is8 ? r78 : r;
 }
+
+ZKG_DEV uint32_t carry1(uint32_t v) {  // one carry round (no carry leaves lane 8 for values < 2^261)
+    return (v & f29::MASK) + shr<1>(v >> 29);
+}
+template <int C>
+ZKG_DEV uint32_t kp(uint32_t rl) {  // lane rl's limb of C p in borrowed limbs (f29::K)
+    uint32_t k = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) k = rl == (uint32_t)i ? f29::K<C>(i) : k;
+    return k;
+}
+
+struct Rows {
+    uint32_t r0, r1, r2, r3;
+};
+// each row's value replicated over all four rows
+ZKG_DEV Rows rows(uint32_t v) {
+    const auto s = __builtin_amdgcn_permlane16_swap(v, v, false, false);  // rows (0 0 2 2), (1 1 3 3)
+    const auto e = __builtin_amdgcn_permlane32_swap(s[0], s[0], false, false);
+    const auto o = __builtin_amdgcn_permlane32_swap(s[1], s[1], false, false);
+    return Rows{e[0], o[0], e[1], o[1]};
+}
+ZKG_DEV uint32_t pick(const Where& at, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {  // row r: a_r
+    const uint32_t lo = at.row_b0 ? a1 : a0, hi = at.row_b0 ? a3 : a2;
+    return at.row_b1 ? hi : lo;
+}
+
+struct SubConsts {  // C p in borrowed limbs for the doubling's three subtractions
+    uint32_t k3, k6, k10;
+    ZKG_DEV explicit SubConsts(uint32_t rl) : k3(kp<3>(rl)), k6(kp<6>(rl)), k10(kp<10>(rl)) {}
+};
+
+struct Point {  // XYZZ, every coordinate replicated over the four rows
+    uint32_t x, y, zz, zzz;
+};
+
+// 2P (dbl-2008-s-1, as xyzz29_dbl_lanes; same bounds: X < 8, Y < 5, ZZ, ZZZ < 2 in and out). W Y
+// and ZZZ3 are taken as V (U Y) and V (U ZZZ), so that three rounds of four products suffice:
+//   round 1: V = U^2 (U = 2Y), X^2, U Y, U ZZZ        (10 * 10, 8 * 8, 10 * 5, 10 * 2)
+//   round 2: S = X V, ZZ3 = V ZZ, M^2 (M = 3X^2), W Y  (8 * 2, 2 * 2, 6 * 6, 2 * 2)
+//   round 3: M D (D = S - X3), ZZZ3 = V (U ZZZ)       (6 * 12, 2 * 2)
+ZKG_DEV Point dbl(const Point& p, const Where& at, const SubConsts& k) {
+    const uint32_t u = p.y + p.y;                                      // < 10, limbs < 2^30 + 2^8
+    const Rows m1 = rows(mul(pick(at, u, p.x, u, u), pick(at, u, p.x, p.y, p.zzz), at));
+    const uint32_t v = m1.r0, uy = m1.r2, uz = m1.r3;
+    const uint32_t m = carry1(m1.r1 + m1.r1 + m1.r1);                  // M = 3 X^2 < 6
+    const Rows m2 = rows(mul(pick(at, p.x, v, m, v), pick(at, v, p.zz, m, uy), at));
+    const uint32_t s = m2.r0;
+    const uint32_t x3 = carry1(m2.r2 + k.k6 - (s + s));                // M^2 + 6p - 2S < 8
+    const uint32_t d = carry1(s + k.k10 - x3);                         // < 12
+    const Rows m3 = rows(mul(at.row_b0 ? v : m, at.row_b0 ? uz : d, at));
+    return Point{x3, carry1(m3.r0 + k.k3 - m2.r3), m2.r1, m3.r1};      // Y3 = M D + 3p - W Y < 5
+}
+
+ZKG_DEV uint32_t from_f29(const F29& a, uint32_t rl) {  // a's limbs (the same on every lane)
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v = rl == (uint32_t)i ? a.l[i] : v;
+    return v;
+}
+ZKG_DEV F29 to_f29(uint32_t v) {  // limbs 0..7 below 2^29 again, on every lane
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.l[i] = (uint32_t)__builtin_amdgcn_readlane((int)v, i);
+    return f29_carry(r);
+}
+
+}  // namespace lpl
 
 // P + Q (add-2008-s) with every special case; P finite (bounds as above), Q finite with all
 // coordinates below 2. *inf is set when the sum is infinity.
@@ -1079,12 +1200,15 @@ __global__ __launch_bounds__(64) void msm_combine29_kernel(const uint32_t* __res
     };
     bool inf;
     Xyzz29 acc = load(W - 1, &inf);
-    const DblRoles roles(lane);
+    const lpl::Where at(lane);
+    const lpl::SubConsts kp(at.rl);
     for (int w = (int)W - 2; w >= 0; --w) {
         const uint32_t width = wb + ((uint32_t)w < nw ? 1u : 0u);  // bits between window w and w + 1
         if (!inf) {  // no 2-torsion: a finite point stays finite
-            for (uint32_t k = 0; k < width; ++k) acc = xyzz29_dbl_roles(acc, roles);
-            acc = xyzz29_from_roles(acc);
+            lpl::Point d{lpl::from_f29(acc.x, at.rl), lpl::from_f29(acc.y, at.rl), lpl::from_f29(acc.zz, at.rl),
+                         lpl::from_f29(acc.zzz, at.rl)};
+            for (uint32_t k = 0; k < width; ++k) d = lpl::dbl(d, at, kp);
+            acc = Xyzz29{lpl::to_f29(d.x), lpl::to_f29(d.y), lpl::to_f29(d.zz), lpl::to_f29(d.zzz)};
         }
         bool q_inf;
         const Xyzz29 q = load(w, &q_inf);
