@@ -1,11 +1,13 @@
-"""The G1 combine's doubling chain with one 29-bit limb per lane (zk-gpu_amd/csrc/msm_kernels.hpp,
-namespace lpl), restated lane for lane: a wave of 64 lanes in four rows of 16, the DPP row moves
-(row_shr, row_shl, row_newbcast, zero for lanes with no source) and the gfx950 row swaps
-(v_permlane16_swap, v_permlane32_swap) as the ISA defines them. Every 32-bit lane value and
-64-bit column is checked against its register width, the product against Montgomery
-multiplication by 2^-261 mod p (random, lazy and worst-case limbs), the doubling's value bounds
-(X < 9p in, X < 8p, Y < 5p, ZZ, ZZZ < 2p out) for a fixed point, and chains of doublings against the
-textbook affine law. CPU only."""
+"""The G1 combine's Horner chain with one 29-bit limb per lane (zk-gpu_amd/csrc/msm_kernels.hpp,
+namespace lpl: the product, the doubling and the addition), restated lane for lane: a wave of 64
+lanes in four rows of 16, the DPP row moves (row_shr, row_shl, row_newbcast, zero for lanes with
+no source) and the gfx950 row swaps (v_permlane16_swap, v_permlane32_swap) as the ISA defines
+them. Every 32-bit lane value and 64-bit column is checked against its register width, the
+product against Montgomery multiplication by 2^-261 mod p (random, lazy and worst-case limbs),
+the value bounds of the doubling (X < 9p in, X < 8p, Y < 5p, ZZ, ZZZ < 2p out) and of the
+addition (the same, with window sums below 2p) for a fixed point, and chains of doublings and
+additions, with the addition's doubling and cancellation cases, against the textbook affine law.
+CPU only."""
 import os
 import random
 import re
@@ -39,7 +41,7 @@ def _kp(c):
 def test_constants():
     assert sum(v << (29 * i) for i, v in enumerate(PL)) == P
     assert sum(v << (29 * i) for i, v in enumerate(PINV)) == (-pow(P, -1, R)) % R
-    for c in (3, 6, 10):
+    for c in (3, 6, 7, 10):
         assert sum(v << (29 * i) for i, v in enumerate(_kp(c))) == c * P
 
 
@@ -248,3 +250,61 @@ def test_doubling_chain_matches_textbook_at_the_bounds():
             for c, bound in zip(p, (8, 5, 2, 2)):
                 assert row_value(c, 0) < bound * P
             assert _affine(p) == exp
+
+
+def is_zero(v):  # the kernel's test: the value (below 12p) is 0 mod p
+    x = row_value(v, 0)
+    assert x < 12 * P
+    return x % P == 0
+
+
+def add_pts(p, q, stats):
+    """lpl::add: (0, sum), (1, None) for infinity, (2, None) when Q = P."""
+    x1, y1, zz1, zzz1 = p
+    x2, y2, zz2, zzz2 = q
+    u1, u2, s1, s2 = rows(mul(pick(x1, x2, y1, y2), pick(zz2, zz1, zzz2, zzz1), stats))
+    pp = ksub(u2, u1, 3)
+    r = ksub(s2, s1, 3)
+    if is_zero(pp):
+        return (2, None) if is_zero(r) else (1, None)
+    pp2, r2, zz12, zzz12 = rows(mul(pick(pp, r, zz1, zzz1), pick(pp, r, zz2, zzz2), stats))
+    ppp, qq, zz3, _ = rows(mul(pick(pp, u1, zz12, pp), pp2, stats))
+    x3 = ksub(r2, add(add(ppp, qq), qq), 7)
+    d = ksub(qq, x3, 10)
+    rd, s1ppp, zzz3, _ = rows(mul(pick(r, s1, zzz12, r), pick(d, ppp, ppp, d), stats))
+    return 0, (x3, ksub(rd, s1ppp, 3), zz3, zzz3)
+
+
+def test_horner_chain_with_additions_matches_textbook():
+    rng = random.Random(2611)
+    hits = set()
+    for trial in range(6):
+        a = O.py_g1_mul(O.G1, rng.randrange(1, O.R))
+        p, exp = _xyzz(a, (1, 1, 1, 1)), a
+        for step in range(3):
+            for _ in range(rng.choice((1, 3))):
+                p = dbl(p, {})
+                exp = O.py_g1_add(exp, exp)
+            if trial == 4 and step == 1:
+                qpt = exp                          # Q = P: the caller doubles
+            elif trial == 5 and step == 1:
+                qpt = (exp[0], (P - exp[1]) % P)   # Q = -P: infinity
+            else:
+                qpt = O.py_g1_mul(O.G1, rng.randrange(1, O.R))
+            # window sums as stored: below 2p, some raised by p
+            q = _xyzz(qpt, (1, 1, 1, 1) if rng.random() < 0.5 else (0, 0, 0, 0))
+            how, s = add_pts(p, q, {})
+            exp = O.py_g1_add(exp, qpt)
+            if how == 1:
+                hits.add("inf")
+                assert exp is None
+                break
+            if how == 2:
+                hits.add("doubling")
+                s = dbl(p, {})
+            for c, bound in zip(s, (9, 5, 2, 2)):
+                assert row_value(c, 0) < bound * P
+                assert all(row_value(c, r) == row_value(c, 0) for r in range(1, 4))
+            p = s
+            assert _affine(p) == exp, (trial, step)
+    assert hits == {"inf", "doubling"}

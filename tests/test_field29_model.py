@@ -351,103 +351,3 @@ def test_ntt_product_is_the_twiddle_product_below_2r():
     one = pow(2, 256, R_FR)
     for a in (0, 1, R_FR - 1, R_FR, 4 * R_FR - 1):
         assert ntt_mul(a, one) % R_FR == a % R_FR
-
-
-# ---- the G1 window combine on lazy limbs (msm_kernels.hpp: xyzz29_dbl_lanes, xyzz29_add_lanes)
-
-def _pmul(a, b):
-    """The product with the limit of the kernel's bounds asserted: a b < p 2^261 (169 p^2)."""
-    assert value(a) * value(b) < P << 261, (value(a) / P, value(b) / P)
-    return mul(a, b)
-
-
-def _carry(a):
-    a = list(a)
-    for i in range(8):
-        a[i + 1] += a[i] >> 29
-        a[i] &= MASK
-    assert all(x < 1 << 29 for x in a[:8])
-    return a
-
-
-def dbl29(p):
-    X, Y, ZZ, ZZZ = p
-    u = add(Y, Y)
-    assert all(x < 1 << 30 for x in u)
-    v, x2 = _pmul(u, u), _pmul(X, X)
-    mm = _carry(add(add(x2, x2), x2))
-    w, s, zz3, m2 = _pmul(u, v), _pmul(X, v), _pmul(v, ZZ), _pmul(mm, mm)
-    x3 = sub(6, m2, add(s, s))
-    d = sub(10, s, x3)
-    y3 = sub(3, _pmul(mm, d), _pmul(w, Y))
-    out = [x3, y3, zz3, _pmul(w, ZZZ)]
-    assert value(x3) < 8 * P and value(y3) < 5 * P and value(out[2]) < 2 * P and value(out[3]) < 2 * P
-    return out
-
-
-def add29(p, q):
-    """Returns the sum, "inf", or the doubling of p."""
-    X, Y, ZZ, ZZZ = p
-    qx, qy, qzz, qzzz = q
-    u1, u2, s1, s2 = _pmul(X, qzz), _pmul(qx, ZZ), _pmul(Y, qzzz), _pmul(qy, ZZZ)
-    zz12, zzz12 = _pmul(ZZ, qzz), _pmul(ZZZ, qzzz)
-    pp, r = sub(3, u2, u1), sub(3, s2, s1)
-    assert value(pp) < 12 * P and value(r) < 12 * P
-    if value(pp) % P == 0:
-        return dbl29(p) if value(r) % P == 0 else "inf"
-    ppsq, r2 = _pmul(pp, pp), _pmul(r, r)
-    ppp, qq, zz3 = _pmul(pp, ppsq), _pmul(u1, ppsq), _pmul(zz12, ppsq)
-    t = add(ppp, add(qq, qq))
-    x3 = sub(7, r2, t)
-    d = sub(10, qq, x3)
-    y3 = sub(3, _pmul(r, d), _pmul(s1, ppp))
-    out = [x3, y3, zz3, _pmul(zzz12, ppp)]
-    assert value(x3) < 9 * P and value(y3) < 5 * P and value(out[2]) < 2 * P and value(out[3]) < 2 * P
-    return out
-
-
-def _xyzz29(pt, extra=(0, 0, 0, 0)):
-    """Affine (x, y) as XYZZ limbs in R = 2^261 form (Z = 1), each value raised by extra[i] p."""
-    Rm = pow(2, 261, P)
-    vals = [pt[0] * Rm % P, pt[1] * Rm % P, Rm, Rm]
-    return [limbs(v + e * P) for v, e in zip(vals, extra)]
-
-
-def _affine29(p):
-    X, Y, ZZ, ZZZ = (value(v) % P for v in p)
-    return (X * pow(ZZ, -1, P) % P, Y * pow(ZZZ, -1, P) % P)
-
-
-def test_combine_lazy_dbl_and_add_match_textbook_at_the_bounds():
-    rng = random.Random(2914)
-    hits = set()
-    for trial in range(8):
-        a = O.py_g1_mul(O.G1, rng.randrange(1, O.R))
-        # the running point at its bounds (X < 9, Y < 5, ZZ, ZZZ < 2) and window sums below 2
-        acc = _xyzz29(a, (8, 4, 1, 1))
-        exp = a
-        for step in range(3):
-            width = rng.choice((1, 13, 15))
-            for _ in range(width):
-                acc = dbl29(acc)
-                exp = O.py_g1_add(exp, exp)
-            assert _affine29(acc) == exp
-            kind = rng.random()
-            if trial == 6 and step == 1:
-                qpt = exp  # Q = P: the doubling case
-            elif trial == 7 and step == 1:
-                qpt = (exp[0], (P - exp[1]) % P)  # Q = -P: infinity
-            else:
-                qpt = O.py_g1_mul(O.G1, rng.randrange(1, O.R))
-            q = _xyzz29(qpt, (1, 1, 1, 1) if kind < 0.5 else (0, 0, 0, 0))
-            res = add29(acc, q)
-            exp = O.py_g1_add(exp, qpt)
-            if res == "inf":
-                hits.add("inf")
-                assert exp is None
-                break
-            if qpt == _affine29(acc):
-                hits.add("doubling")
-            acc = res
-            assert _affine29(acc) == exp, (trial, step)
-    assert hits == {"inf", "doubling"}

@@ -903,42 +903,10 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
 // ---------------------------------------------------------------- G1 combine on lazy 29-bit limbs
 // The same Horner chain for the 29-bit G1 pipeline, with the values kept lazily reduced between
 // products as in msm_accumulate29_kernel (no renormalisation after each addition or subtraction,
-// which the reduced Fq29 type pays for). Bounds in units of p, checked against the product's limit
-// (169) in tests/test_field29_model.py: the running point keeps X < 9, Y < 5, ZZ, ZZZ < 2 through
-// every doubling and addition; window sums (Fq29 storage) are below 2.
+// which the reduced Fq29 type pays for). Bounds in units of p, the product's limit being 169: the
+// running point keeps X < 9, Y < 5, ZZ, ZZZ < 2 through every doubling and addition; window sums
+// (Fq29 storage) are below 2.
 
-ZKG_DEV F29 f29_sel(bool c, const F29& a, const F29& b) {
-    F29 r;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) r.l[i] = c ? a.l[i] : b.l[i];
-    return r;
-}
-ZKG_DEV F29 f29_pick(uint32_t lane, const F29& a0, const F29& a1, const F29& a2) {
-    return f29_sel(lane == 0, a0, f29_sel(lane == 1, a1, a2));
-}
-ZKG_DEV F29 f29_pick(uint32_t lane, const F29& a0, const F29& a1, const F29& a2, const F29& a3) {
-    return f29_sel(lane == 0, a0, f29_sel(lane == 1, a1, f29_sel(lane == 2, a2, a3)));
-}
-ZKG_DEV F29 f29_pick(uint32_t lane, const F29& a0, const F29& a1, const F29& a2, const F29& a3, const F29& a4,
-                     const F29& a5) {
-    return f29_sel(lane == 0, a0, f29_sel(lane == 1, a1, f29_sel(lane == 2, a2, f29_sel(lane == 3, a3,
-                                                                                      f29_sel(lane == 4, a4, a5)))));
-}
-// lane `src`'s value on every lane (src wave-uniform: a read of that lane into a scalar register)
-ZKG_DEV F29 f29_shfl(const F29& x, int src) {
-    F29 r;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) r.l[i] = (uint32_t)__builtin_amdgcn_readlane((int)x.l[i], src);
-    return r;
-}
-// within each group of four lanes, lane i takes lane sel_i's value (dpp_quad's quad_perm control)
-template <int CTRL>
-ZKG_DEV F29 f29_dpp(const F29& x) {
-    F29 r;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) r.l[i] = dpp_quad<CTRL>(x.l[i]);
-    return r;
-}
 // limbs 0..7 below 2^29 again (the value is unchanged)
 ZKG_DEV F29 f29_carry(F29 a) {
 #pragma unroll
@@ -947,30 +915,6 @@ ZKG_DEV F29 f29_carry(F29 a) {
         a.l[i] &= f29::MASK;
     }
     return a;
-}
-
-struct Xyzz29 {
-    F29 x, y, zz, zzz;
-};
-
-// 2P (dbl-2008-s-1) with each level's products on lanes 0..3; P finite, X < 9, Y < 5, ZZ, ZZZ < 2.
-ZKG_DEV Xyzz29 xyzz29_dbl_lanes(const Xyzz29& p, uint32_t lane) {
-    const F29 u = f29_add(p.y, p.y);                      // < 10, limbs < 2^30
-    F29 a = f29_sel(lane == 0, u, p.x);
-    F29 m = f29_mul(a, a);                                // 10 * 10, 9 * 9 -> < 2
-    const F29 v = f29_shfl(m, 0), x2 = f29_shfl(m, 1);
-    const F29 mm = f29_carry(f29_add(f29_add(x2, x2), x2));  // 3 X^2 < 6, normalised
-    a = f29_pick(lane, u, p.x, v, mm);
-    F29 b = f29_pick(lane, v, v, p.zz, mm);
-    m = f29_mul(a, b);                                    // 10 * 2, 9 * 2, 2 * 2, 6 * 6 -> < 2
-    const F29 w = f29_shfl(m, 0), s = f29_shfl(m, 1), zz3 = f29_shfl(m, 2), m2 = f29_shfl(m, 3);
-    const F29 x3 = f29_sub<6>(m2, f29_add(s, s));         // 2S < 4 < 5: X3 < 8
-    const F29 d = f29_sub<10>(s, x3);                     // < 12
-    a = f29_pick(lane, mm, w, w);
-    b = f29_pick(lane, d, p.y, p.zzz);
-    m = f29_mul(a, b);                                    // 6 * 12, 2 * 5, 2 * 2 -> < 2
-    const F29 y3 = f29_sub<3>(f29_shfl(m, 0), f29_shfl(m, 1));  // < 5
-    return Xyzz29{x3, y3, zz3, f29_shfl(m, 2)};
 }
 
 // ---------------------------------------------------------------- limb-per-lane doubling chain
@@ -1107,16 +1051,16 @@ ZKG_DEV uint32_t pick(const Where& at, uint32_t a0, uint32_t a1, uint32_t a2, ui
     return at.row_b1 ? hi : lo;
 }
 
-struct SubConsts {  // C p in borrowed limbs for the doubling's three subtractions
-    uint32_t k3, k6, k10;
-    ZKG_DEV explicit SubConsts(uint32_t rl) : k3(kp<3>(rl)), k6(kp<6>(rl)), k10(kp<10>(rl)) {}
+struct SubConsts {  // C p in borrowed limbs for the subtractions of the doubling and the addition
+    uint32_t k3, k6, k7, k10;
+    ZKG_DEV explicit SubConsts(uint32_t rl) : k3(kp<3>(rl)), k6(kp<6>(rl)), k7(kp<7>(rl)), k10(kp<10>(rl)) {}
 };
 
 struct Point {  // XYZZ, every coordinate replicated over the four rows
     uint32_t x, y, zz, zzz;
 };
 
-// 2P (dbl-2008-s-1, as xyzz29_dbl_lanes; same bounds: X < 8, Y < 5, ZZ, ZZZ < 2 in and out). W Y
+// 2P (dbl-2008-s-1) for P finite with X < 9, Y < 5, ZZ, ZZZ < 2 (2P: X < 8, Y < 5, ZZ, ZZZ < 2). W Y
 // and ZZZ3 are taken as V (U Y) and V (U ZZZ), so that three rounds of four products suffice:
 //   round 1: V = U^2 (U = 2Y), X^2, U Y, U ZZZ        (10 * 10, 8 * 8, 10 * 5, 10 * 2)
 //   round 2: S = X V, ZZ3 = V ZZ, M^2 (M = 3X^2), W Y  (8 * 2, 2 * 2, 6 * 6, 2 * 2)
@@ -1147,93 +1091,85 @@ ZKG_DEV F29 to_f29(uint32_t v) {  // limbs 0..7 below 2^29 again, on every lane
     return f29_carry(r);
 }
 
-}  // namespace lpl
-
-// P + Q (add-2008-s) with every special case; P finite (bounds as above), Q finite with all
-// coordinates below 2. *inf is set when the sum is infinity.
-ZKG_DEV Xyzz29 xyzz29_add_lanes(const Xyzz29& p, const Xyzz29& q, uint32_t lane, bool* inf) {
-    F29 a = f29_pick(lane, p.x, q.x, p.y, q.y, p.zz, p.zzz);
-    F29 b = f29_pick(lane, q.zz, p.zz, q.zzz, p.zzz, q.zz, q.zzz);
-    F29 m = f29_mul(a, b);                                // 9 * 2, 2 * 2, 5 * 2, 2 * 2, 2 * 2, 2 * 2 -> < 2
-    const F29 u1 = f29_shfl(m, 0), u2 = f29_shfl(m, 1), s1 = f29_shfl(m, 2), s2 = f29_shfl(m, 3);
-    const F29 zz12 = f29_shfl(m, 4), zzz12 = f29_shfl(m, 5);
-    const F29 pp = f29_sub<3>(u2, u1), r = f29_sub<3>(s2, s1);  // < 5
-    *inf = false;
-    if (f29_is_zero_below12p(pp)) {
-        if (f29_is_zero_below12p(r)) return xyzz29_dbl_lanes(p, lane);
-        *inf = true;
-        return p;
-    }
-    a = f29_sel(lane == 0, pp, r);
-    m = f29_mul(a, a);                                    // 5 * 5 -> < 2
-    const F29 ppsq = f29_shfl(m, 0), r2 = f29_shfl(m, 1);
-    a = f29_pick(lane, pp, u1, zz12);
-    m = f29_mul(a, ppsq);                                 // 5 * 2, 2 * 2, 2 * 2 -> < 2
-    const F29 ppp = f29_shfl(m, 0), qq = f29_shfl(m, 1), zz3 = f29_shfl(m, 2);
-    const F29 x3 = f29_sub<7>(r2, f29_add(ppp, f29_add(qq, qq)));  // PPP + 2Q < 6: X3 < 9
-    const F29 d = f29_sub<10>(qq, x3);                   // < 12
-    a = f29_pick(lane, r, s1, zzz12);
-    b = f29_pick(lane, d, ppp, ppp);
-    m = f29_mul(a, b);                                    // 5 * 12, 2 * 2, 2 * 2 -> < 2
-    const F29 y3 = f29_sub<3>(f29_shfl(m, 0), f29_shfl(m, 1));  // < 5
-    return Xyzz29{x3, y3, zz3, f29_shfl(m, 2)};
+// P + Q (add-2008-s with its special cases) for P with X < 9, Y < 5, ZZ, ZZZ < 2 and Q with
+// coordinates below 2 (the sum: X < 9, Y < 5, ZZ, ZZZ < 2), in four rounds of products:
+//   round 1: U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1      (9 * 2, 2 * 2, 5 * 2, 2 * 2)
+//   round 2: PP = P^2, R^2, ZZ1 ZZ2, ZZZ1 ZZZ2  (P = U2 - U1, R = S2 - S1)   (5 * 5, 5 * 5, 2 * 2, 2 * 2)
+//   round 3: PPP = P PP, Q = U1 PP, ZZ3 = ZZ1 ZZ2 PP                   (5 * 2, 2 * 2, 2 * 2)
+//   round 4: R (Q - X3), S1 PPP, ZZZ3 = ZZZ1 ZZZ2 PPP                  (5 * 12, 2 * 2, 2 * 2)
+// Returns 0 with the sum in *out, 1 if the sum is infinity (Q = -P), 2 if Q = P (the caller
+// doubles P).
+ZKG_DEV int add(const Point& p, const Point& q, const Where& at, const SubConsts& k, Point* out) {
+    const Rows m1 = rows(mul(pick(at, p.x, q.x, p.y, q.y), pick(at, q.zz, p.zz, q.zzz, p.zzz), at));
+    const uint32_t pp = carry1(m1.r1 + k.k3 - m1.r0);                     // P = U2 - U1 < 5
+    const uint32_t r = carry1(m1.r3 + k.k3 - m1.r2);                      // R = S2 - S1 < 5
+    if (f29_is_zero_below12p(to_f29(pp))) return f29_is_zero_below12p(to_f29(r)) ? 2 : 1;
+    const Rows m2 = rows(mul(pick(at, pp, r, p.zz, p.zzz), pick(at, pp, r, q.zz, q.zzz), at));
+    const uint32_t pp2 = m2.r0;
+    const Rows m3 = rows(mul(pick(at, pp, m1.r0, m2.r2, pp), pp2, at));
+    const uint32_t x3 = carry1(m2.r1 + k.k7 - (m3.r0 + m3.r1 + m3.r1));  // R^2 - PPP - 2Q < 9
+    const uint32_t d = carry1(m3.r1 + k.k10 - x3);                        // Q - X3 < 12
+    const Rows m4 = rows(mul(pick(at, r, m1.r2, m2.r3, r), pick(at, d, m3.r0, m3.r0, d), at));
+    *out = Point{x3, carry1(m4.r0 + k.k3 - m4.r1), m3.r2, m4.r2};          // Y3 = R D - S1 PPP < 5
+    return 0;
 }
+
+}  // namespace lpl
 
 template <class Fe>  // Fq29 (a template, so that every translation unit may instantiate it)
 __global__ __launch_bounds__(64) void msm_combine29_kernel(const uint32_t* __restrict__ wsums, uint32_t W,
                                                           uint32_t wb, uint32_t nw, uint32_t* __restrict__ result) {
     constexpr int XW = 4 * FeTraits<Fq29>::WORDS;
     const uint32_t lane = threadIdx.x;
-    const uint32_t zero = lane >> 6;  // every lane loads the same sums through a lane-dependent address
+    const lpl::Where at(lane);
+    const lpl::SubConsts kp(at.rl);
+    // window w's sum (Fq29 storage, below 2p) with one limb per lane; every lane reads the whole
+    // sum through a lane-dependent address (vector registers) and keeps its limb
+    const uint32_t zero = lane >> 6;
+    auto limbs = [&](const Fq29& v) {
+        F29 f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f.l[i] = v.l[i];
+        return lpl::from_f29(f, at.rl);
+    };
     auto load = [&](uint32_t w, bool* is_inf) {
         const Xyzz<Fq29> v = load_xyzz<Fq29>(wsums + (size_t)XW * (w + zero));
         *is_inf = fq29_is_zero(v.zz);
-        Xyzz29 r;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            r.x.l[i] = v.x.l[i];
-            r.y.l[i] = v.y.l[i];
-            r.zz.l[i] = v.zz.l[i];
-            r.zzz.l[i] = v.zzz.l[i];
-        }
-        return r;
+        return lpl::Point{limbs(v.x), limbs(v.y), limbs(v.zz), limbs(v.zzz)};
     };
     bool inf;
-    Xyzz29 acc = load(W - 1, &inf);
-    const lpl::Where at(lane);
-    const lpl::SubConsts kp(at.rl);
+    lpl::Point acc = load(W - 1, &inf);
     for (int w = (int)W - 2; w >= 0; --w) {
         const uint32_t width = wb + ((uint32_t)w < nw ? 1u : 0u);  // bits between window w and w + 1
-        if (!inf) {  // no 2-torsion: a finite point stays finite
-            lpl::Point d{lpl::from_f29(acc.x, at.rl), lpl::from_f29(acc.y, at.rl), lpl::from_f29(acc.zz, at.rl),
-                         lpl::from_f29(acc.zzz, at.rl)};
-            for (uint32_t k = 0; k < width; ++k) d = lpl::dbl(d, at, kp);
-            acc = Xyzz29{lpl::to_f29(d.x), lpl::to_f29(d.y), lpl::to_f29(d.zz), lpl::to_f29(d.zzz)};
-        }
+        if (!inf)  // no 2-torsion: a finite point stays finite
+            for (uint32_t k = 0; k < width; ++k) acc = lpl::dbl(acc, at, kp);
         bool q_inf;
-        const Xyzz29 q = load(w, &q_inf);
+        const lpl::Point q = load(w, &q_inf);
         if (q_inf) continue;
         if (inf) {
             acc = q;
             inf = false;
             continue;
         }
-        acc = xyzz29_add_lanes(acc, q, lane, &inf);
-    }
-    if (lane != 0) return;
-    if (inf) {
-        store_xyzz_result(result, xyzz_inf<Fq29>());
-        return;
+        lpl::Point sum;
+        const int how = lpl::add(acc, q, at, kp, &sum);
+        if (how == 0) acc = sum;
+        else if (how == 1) inf = true;
+        else acc = lpl::dbl(acc, at, kp);
     }
     Xyzz<Fq29> out;
+    if (!inf) {
+        const F29 x = lpl::to_f29(acc.x), y = lpl::to_f29(acc.y), zz = lpl::to_f29(acc.zz), zzz = lpl::to_f29(acc.zzz);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {  // values below 12 p: store_result reduces them to canonical words
-        out.x.l[i] = acc.x.l[i];
-        out.y.l[i] = acc.y.l[i];
-        out.zz.l[i] = acc.zz.l[i];
-        out.zzz.l[i] = acc.zzz.l[i];
+        for (int i = 0; i < 9; ++i) {  // values below 12 p: store_result reduces them to canonical words
+            out.x.l[i] = x.l[i];
+            out.y.l[i] = y.l[i];
+            out.zz.l[i] = zz.l[i];
+            out.zzz.l[i] = zzz.l[i];
+        }
     }
-    store_xyzz_result(result, out);
+    if (lane != 0) return;
+    store_xyzz_result(result, inf ? xyzz_inf<Fq29>() : out);
 }
 
 // ---------------------------------------------------------------- stage launchers
