@@ -1,13 +1,15 @@
-"""The G1 combine's Horner chain with one 29-bit limb per lane (zk-gpu_amd/csrc/msm_kernels.hpp,
-namespace lpl: the product, the doubling and the addition), restated lane for lane: a wave of 64
+"""The combines' Horner chains with one 29-bit limb per lane (zk-gpu_amd/csrc/msm_kernels.hpp,
+namespace lpl: the product, the doubling and the addition for G1, and for G2 the Fq2 product
+combined on the 64-bit columns, its doubling and addition), restated lane for lane: a wave of 64
 lanes in four rows of 16, the DPP row moves (row_shr, row_shl, row_newbcast, zero for lanes with
 no source) and the gfx950 row swaps (v_permlane16_swap, v_permlane32_swap) as the ISA defines
 them. Every 32-bit lane value and 64-bit column is checked against its register width, the
 product against Montgomery multiplication by 2^-261 mod p (random, lazy and worst-case limbs),
 the value bounds of the doubling (X < 9p in, X < 8p, Y < 5p, ZZ, ZZZ < 2p out) and of the
 addition (the same, with window sums below 2p) for a fixed point, and chains of doublings and
-additions, with the addition's doubling and cancellation cases, against the textbook affine law.
-CPU only."""
+additions, with the addition's doubling and cancellation cases, against the textbook affine law;
+for G2 the same over Fq2 inside the bound box the chain keeps, with no 64-bit column going
+negative where the real part subtracts a1 b1. CPU only."""
 import os
 import random
 import re
@@ -307,4 +309,177 @@ def test_horner_chain_with_additions_matches_textbook():
                 assert all(row_value(c, r) == row_value(c, 0) for r in range(1, 4))
             p = s
             assert _affine(p) == exp, (trial, step)
+    assert hits == {"inf", "doubling"}
+
+
+# ---- G2 (msm_combine_g2_lanes_kernel): Fq2 products combined on the 64-bit columns
+# p 2^261 in borrowed columns, as lpl::mcol and lpl::M16 build it from p's limbs
+MC = [(1 << 62) - ((1 << 33) if j else 0) + (PL[j - 9] if 9 <= j <= 15 else 0) for j in range(16)]
+M16 = PL[7] + (PL[8] << 29) - (1 << 33)
+
+
+def test_g2_column_constant():
+    assert sum(v << (29 * j) for j, v in enumerate(MC + [M16])) == P * R
+    assert all(v >= (1 << 62) - (1 << 33) for v in MC) and M16 > 0
+
+
+def reduce_cols(u, h, stats):
+    """lpl::reduce: mul without its first step."""
+    tn = carry2(u)
+    q = carry2(columns([0] * LANES, tn, [[PINV[k]] * LANES for k in range(9)]))
+    q = [q[l] if rl(l) < 8 else (q[l] & MASK if rl(l) == 8 else 0) for l in range(LANES)]
+    u = columns(u, q, [[PL[k]] * LANES for k in range(9)])
+    h = w64([x + y * PL[8] for x, y in zip(h, bc(q, 8))])
+    hi = [x >> 29 for x in u]
+    c1 = w64([(x & MASK) + y for x, y in zip(u, shr(hi, 1))])
+    h = w64([x + y for x, y in zip(h, bc(hi, 15))])
+    hi2 = w32([x >> 29 for x in c1])
+    uu = w32([(x & MASK) + y for x, y in zip(c1, shr(hi2, 1))])
+    h = w64([x + y for x, y in zip(h, bc(hi2, 15))])
+    stats["h"] = max(stats.get("h", 0), max(h))
+    nz = [any(uu[r * 16 + i] for i in range(9)) for r in range(4)]
+    r = shl(uu, 9)
+    out = [r[l] + (1 if rl(l) == 0 and nz[l >> 4] else 0) for l in range(LANES)]
+    return w32([h[l] & MASK if rl(l) == 7 else h[l] >> 29 if rl(l) == 8 else out[l] for l in range(LANES)])
+
+
+def mul2(x, y, stats):
+    """lpl::mul2: rows a0 b0, a1 b1, a0 b1, a1 b0; rows 0, 1 -> a0 b0 + p 2^261 - a1 b1, rows 2, 3 ->
+    a0 b1 + a1 b0 on the columns; one reduction; each component replicated over the rows."""
+    for v in x + y:  # inputs normalised
+        assert all(v[l] < (1 << 29) + (1 << 8) for l in range(LANES) if rl(l) < 8)
+    a, b = pick(x[0], x[1], x[0], x[1]), pick(y[0], y[1], y[1], y[0])
+    t = columns([0] * LANES, a, [bc(b, k) for k in range(9)])
+    h = w64([p * q for p, q in zip(bc(a, 8), bc(b, 8))])
+    t0, t1 = swap16(t, t)
+    h0, h1 = swap16(h, h)
+    u, hu = [], []
+    for l in range(LANES):
+        if l >> 5:
+            u.append(t0[l] + t1[l])
+            hu.append(h0[l] + h1[l])
+        else:
+            assert t1[l] <= t0[l] + MC[rl(l)] and h1[l] <= h0[l] + M16  # no column goes negative
+            u.append(t0[l] + MC[rl(l)] - t1[l])
+            hu.append(h0[l] + M16 - h1[l])
+    r = reduce_cols(w64(u), w64(hu), stats)
+    e0, e1 = swap32(r, r)
+    return e0, e1
+
+
+def add2(x, y):
+    return add(x[0], y[0]), add(x[1], y[1])
+
+
+def norm2(x):
+    return carry1(x[0]), carry1(x[1])
+
+
+def sub2(x, y, c):
+    return ksub(x[0], y[0], c), ksub(x[1], y[1], c)
+
+
+def dbl_g2(p, st):
+    x, y, zz, zzz = p
+    u = norm2(add2(y, y))
+    v, x2, uy, uz = mul2(u, u, st), mul2(x, x, st), mul2(u, y, st), mul2(u, zzz, st)
+    m = norm2(add2(add2(x2, x2), x2))
+    s, zz3, m2, wy = mul2(x, v, st), mul2(v, zz, st), mul2(m, m, st), mul2(v, uy, st)
+    x3 = sub2(m2, add2(s, s), 6)
+    d = sub2(s, x3, 10)
+    md, zzz3 = mul2(m, d, st), mul2(v, uz, st)
+    return x3, sub2(md, wy, 4), zz3, zzz3
+
+
+def is_zero2(x):
+    return is_zero(x[0]) and is_zero(x[1])
+
+
+def add_g2(p, q, st):
+    x1, y1, zz1, zzz1 = p
+    x2, y2, zz2, zzz2 = q
+    u1, u2, s1, s2 = mul2(x1, zz2, st), mul2(x2, zz1, st), mul2(y1, zzz2, st), mul2(y2, zzz1, st)
+    pp, r = sub2(u2, u1, 4), sub2(s2, s1, 4)
+    if is_zero2(pp):
+        return (2, None) if is_zero2(r) else (1, None)
+    pp2, r2, zz12, zzz12 = mul2(pp, pp, st), mul2(r, r, st), mul2(zz1, zz2, st), mul2(zzz1, zzz2, st)
+    ppp, qq, zz3 = mul2(pp, pp2, st), mul2(u1, pp2, st), mul2(zz12, pp2, st)
+    x3 = sub2(r2, add2(ppp, add2(qq, qq)), 8)
+    d = sub2(qq, x3, 12)
+    rd, s1ppp, zzz3 = mul2(r, d, st), mul2(s1, ppp, st), mul2(zzz12, ppp, st)
+    return 0, (x3, sub2(rd, s1ppp, 4), zz3, zzz3)
+
+
+RINV = pow(R, -1, P)
+
+
+def fq2_val(x):
+    return row_value(x[0], 0), row_value(x[1], 0)
+
+
+def test_fq2_product_at_the_chain_bounds():
+    rng = random.Random(2620)
+    st = {}
+    # operand bounds (real, imaginary) in units of p that the chain multiplies: U U, X X, M M, M D, R D
+    pairs = [((13.2, 12), (13.2, 12)), ((10.3, 9.4), (10.3, 9.4)), ((7.9, 6.4), (7.9, 6.4)),
+             ((7.9, 6.4), (12.2, 11.4)), ((6.1, 5.1), (14.1, 13.1)), ((2, 2), (2, 2))]
+    for it in range(90):
+        ba, bb = pairs[it % len(pairs)]
+        val = (lambda b: int(b * P) - 1 - rng.randrange(1 << 30)) if it % 3 == 0 else (lambda b: rng.randrange(int(b * P)))
+        a = (val(ba[0]), val(ba[1]))
+        b = (val(bb[0]), val(bb[1]))
+        c = fq2_val(mul2((replicated(a[0]), replicated(a[1])), (replicated(b[0]), replicated(b[1])), st))
+        assert c[0] % P == (a[0] * b[0] - a[1] * b[1]) * RINV % P
+        assert c[1] % P == (a[0] * b[1] + a[1] * b[0]) * RINV % P
+        assert c[0] < a[0] * b[0] // R + 2 * P + (P >> 40)
+        assert c[1] < (a[0] * b[1] + a[1] * b[0]) // R + P + (P >> 40)
+    assert st["h"] < 1 << 53
+
+
+def _xyzz2(pt, extra):
+    Rm = R % P
+    x, y = pt
+    coords = [(x[0] * Rm % P, x[1] * Rm % P), (y[0] * Rm % P, y[1] * Rm % P), (Rm, 0), (Rm, 0)]
+    return [(replicated(c[0] + e * P), replicated(c[1] + e * P)) for c, e in zip(coords, extra)]
+
+
+def _affine2(p):
+    X, Y, ZZ, ZZZ = ((row_value(c[0], 0) % P, row_value(c[1], 0) % P) for c in p)
+    return O.py_f2_mul(X, O.py_f2_inv(ZZ)), O.py_f2_mul(Y, O.py_f2_inv(ZZZ))
+
+
+BOX = ((10.3, 9.4), (6.6, 6.0), (2.1, 2.0), (2.1, 2.0))  # the running point's bounds
+
+
+def test_g2_horner_chain_matches_textbook_at_the_bounds():
+    rng = random.Random(2621)
+    hits = set()
+    for trial in range(5):
+        a = O.py_g2_mul(O.G2, rng.randrange(1, 1 << 64))
+        p, exp = _xyzz2(a, (1, 1, 1, 1)), a
+        for step in range(3):
+            for _ in range(rng.choice((1, 2))):
+                p = dbl_g2(p, {})
+                exp = O.py_g2_add(exp, exp)
+            if trial == 3 and step == 1:
+                qpt = exp
+            elif trial == 4 and step == 1:
+                qpt = (exp[0], O.py_f2_sub((0, 0), exp[1]))
+            else:
+                qpt = O.py_g2_mul(O.G2, rng.randrange(1, 1 << 64))
+            q = _xyzz2(qpt, (1, 1, 1, 1) if rng.random() < 0.5 else (0, 0, 0, 0))
+            how, s = add_g2(p, q, {})
+            exp = O.py_g2_add(exp, qpt)
+            if how == 1:
+                hits.add("inf")
+                assert exp is None
+                break
+            if how == 2:
+                hits.add("doubling")
+                s = dbl_g2(p, {})
+            for c, bound in zip(s, BOX):
+                assert row_value(c[0], 0) < bound[0] * P and row_value(c[1], 0) < bound[1] * P
+                assert all(row_value(c[i], r) == row_value(c[i], 0) for i in (0, 1) for r in range(1, 4))
+            p = s
+            assert _affine2(p) == exp, (trial, step)
     assert hits == {"inf", "doubling"}

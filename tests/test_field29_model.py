@@ -44,7 +44,7 @@ def test_constants_follow_from_p():
     assert header_table("C256") == limbs(pow(2, 256, P))
     kp0 = header_table("KP0")
     assert [0] + kp0 == [limbs(k * P)[0] for k in range(12)]  # the k = 0 entry is written as 0u
-    for c in (3, 6, 7, 10):
+    for c in (2, 3, 4, 6, 7, 8, 10, 12):
         k = header_k(c)
         assert value(k) == c * P
         assert k[0] >= 1 << 31 and all(x >= (1 << 31) - 4 for x in k[1:8]) and all(x < 1 << 32 for x in k)

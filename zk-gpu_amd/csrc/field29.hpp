@@ -52,6 +52,11 @@ ZKG_DEV constexpr uint32_t K<3>(int i) {
          : i == 4 ? 0x80918c64u : i == 5 ? 0x8891c23eu : i == 6 ? 0x81f4a3bfu : i == 7 ? 0x8b147298u : 0x00912ce7u;
 }
 template <>
+ZKG_DEV constexpr uint32_t K<4>(int i) {
+    return i == 0 ? 0x81f3f51cu : i == 1 ? 0x841182d7u : i == 2 ? 0x91ca8d38u : i == 3 ? 0x8b548b3fu
+         : i == 4 ? 0x961765dcu : i == 5 ? 0x8b6d02feu : i == 6 ? 0x829b8500u : i == 7 ? 0x997098ccu : 0x00c19135u;
+}
+template <>
 ZKG_DEV constexpr uint32_t K<6>(int i) {
     return i == 0 ? 0x92edefaau : i == 1 ? 0x861a4444u : i == 2 ? 0x8aafd3d6u : i == 3 ? 0x90fed0e1u
          : i == 4 ? 0x812318ccu : i == 5 ? 0x91238480u : i == 6 ? 0x83e94782u : i == 7 ? 0x9628e534u : 0x012259d2u;
@@ -62,9 +67,20 @@ ZKG_DEV constexpr uint32_t K<7>(int i) {
          : i == 4 ? 0x96a8f244u : i == 5 ? 0x93fec540u : i == 6 ? 0x849028c3u : i == 7 ? 0x84850b68u : 0x0152be21u;
 }
 template <>
+ZKG_DEV constexpr uint32_t K<8>(int i) {
+    return i == 0 ? 0x83e7ea38u : i == 1 ? 0x882305b2u : i == 2 ? 0x83951a74u : i == 3 ? 0x96a91683u
+         : i == 4 ? 0x8c2ecbbcu : i == 5 ? 0x96da0601u : i == 6 ? 0x85370a04u : i == 7 ? 0x92e1319cu : 0x0183226fu;
+}
+template <>
 ZKG_DEV constexpr uint32_t K<10>(int i) {
     return i == 0 ? 0x94e1e4c6u : i == 1 ? 0x8a2bc71fu : i == 2 ? 0x9c7a6112u : i == 3 ? 0x9c535c24u
          : i == 4 ? 0x973a7eacu : i == 5 ? 0x9c908782u : i == 6 ? 0x8684cc86u : i == 7 ? 0x8f997e04u : 0x01e3eb0cu;
+}
+
+template <>
+ZKG_DEV constexpr uint32_t K<12>(int i) {
+    return i == 0 ? 0x85dbdf54u : i == 1 ? 0x8c34888du : i == 2 ? 0x955fa7b0u : i == 3 ? 0x81fda1c6u
+         : i == 4 ? 0x8246319du : i == 5 ? 0x82470904u : i == 6 ? 0x87d28f09u : i == 7 ? 0x8c51ca6cu : 0x0244b3a9u;
 }
 
 ZKG_DEV constexpr uint32_t TWO_P(int i) {  // 2p, normalised

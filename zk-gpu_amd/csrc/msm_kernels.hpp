@@ -868,19 +868,13 @@ struct CombineField {
     static ZKG_DEV Fe conv(const Fe& x) { return x; }
 };
 
-template <>
-struct CombineField<Fq2_29> {
-    // G2 (29-bit stages): the chain on lane pairs (Fq2P, the same memory form), each Fq2 product
-    // by one pair. Measured at G2 2^18 against three lanes per product in the 32-bit Fq2 (0.94
-    // ms) and in Fq2_29 (1.16 ms, Karatsuba's partial products gathered by shuffles).
-    using type = Fq2P;
-};
-
 template <class Fe>
 ZKG_DEV Xyzz<typename CombineField<Fe>::type> load_combine(const uint32_t* p) {
     return load_xyzz<typename CombineField<Fe>::type>(p);
 }
 
+// The chain of the 32-bit pipelines (ZKGPU_MSM_F29=0: Fq, and Fq2 with three lanes per product);
+// the 29-bit pipelines run msm_combine29_kernel (G1) and msm_combine_g2_lanes_kernel (G2).
 template <class Fe>
 __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restrict__ wsums, uint32_t W, uint32_t wb,
                                                         uint32_t nw, uint32_t* __restrict__ result) {
@@ -897,7 +891,7 @@ __global__ __launch_bounds__(64) void msm_combine_kernel(const uint32_t* __restr
             if (!xyzz_is_inf(acc)) acc = xyzz_dbl_lanes(acc, lane);
         acc = xyzz_add_lanes(acc, load_combine<Fe>(wsums + (size_t)XW * (w + zero)), lane);
     }
-    if (lane < FeLanes<Fc>::value) store_xyzz_result(result, acc);  // Fq2P: each lane of the pair its component
+    if (lane < FeLanes<Fc>::value) store_xyzz_result(result, acc);
 }
 
 // ---------------------------------------------------------------- G1 combine on lazy 29-bit limbs
@@ -1001,10 +995,9 @@ struct Where {
           is7((l & 15) == 7), is0((l & 15) == 0) {}
 };
 
-// limbs 0..7 of the result below 2^29 + 2^7 + 1, limb 8 below 2^24
-ZKG_DEV uint32_t mul(uint32_t a, uint32_t b, const Where& at) {
-    const uint64_t t = columns(0, a, BcastOf{b});
-    uint64_t h = (uint64_t)bc<8>(a) * bc<8>(b);               // column 16
+// Montgomery reduction of the 64-bit columns t (lane j: column j) and h (column 16, the same on
+// every lane of a row): limbs 0..7 of the result below 2^29 + 2^7 + 1, limb 8 below 2^24
+ZKG_DEV uint32_t reduce(const uint64_t t, uint64_t h, const Where& at) {
     const uint32_t tn = carry2(t);                             // = T mod 2^261 on lanes 0..8
     uint32_t q = carry2(columns(0, tn, PinvLimbs{}));
     q = at.low8 ? q : (at.is8 ? q & f29::MASK : 0u);           // q mod 2^261, lanes 9..15 cleared
@@ -1023,6 +1016,10 @@ ZKG_DEV uint32_t mul(uint32_t a, uint32_t b, const Where& at) {
     const uint32_t r = dpp<0x100 + 9>(uu) + carry;             // row_shl:9: limbs 9..15 to lanes 0..6
     const uint32_t r78 = at.is7 ? (uint32_t)h & f29::MASK : (uint32_t)(h >> 29);
     return at.is7 || at.is8 ? r78 : r;
+}
+
+ZKG_DEV uint32_t mul(uint32_t a, uint32_t b, const Where& at) {
+    return reduce(columns(0, a, BcastOf{b}), (uint64_t)bc<8>(a) * bc<8>(b), at);
 }
 
 ZKG_DEV uint32_t carry1(uint32_t v) {  // one carry round (no carry leaves lane 8 for values < 2^261)
@@ -1114,6 +1111,99 @@ ZKG_DEV int add(const Point& p, const Point& q, const Where& at, const SubConsts
     return 0;
 }
 
+// ---- G2: Fq2 = Fq[u]/(u^2 + 1), each component one limb per lane
+// An Fq2 product puts its four Fq products on the four rows (a0 b0, a1 b1, a0 b1, a1 b0) and
+// combines them on the 64-bit columns before one reduction per component: rows 0, 1 take
+// a0 b0 + p 2^261 - a1 b1 and rows 2, 3 a0 b1 + a1 b0 (the rows' columns exchanged by a row
+// swap). p 2^261 is added in borrowed columns (every column 0..15 at least 2^62 - 2^33, column 16
+// p's bits from 203), so that no column goes negative for a1 b1 < p 2^261 (about 169 p^2) with
+// limbs below 2^29 + 2^8: the real part is below a0 b0 / 2^261 + 2p and the imaginary part below
+// (a0 b1 + a1 b0) / 2^261 + p, which keeps the chain's bounds closed where separately reduced
+// products (each carrying its own + p) did not.
+struct Fq2L {
+    uint32_t a, b;
+};
+struct Point2 {  // XYZZ over Fq2, every component replicated over the four rows
+    Fq2L x, y, zz, zzz;
+};
+
+// lane rl's column of p 2^261 in borrowed 64-bit columns (columns 0..15; column 16 is M16)
+ZKG_DEV uint64_t mcol(uint32_t rl) {
+    uint32_t pl = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) pl = rl == (uint32_t)(9 + i) ? f29::P(i) : pl;
+    return (1ull << 62) - (rl ? (1ull << 33) : 0ull) + pl;
+}
+constexpr uint64_t M16 = (uint64_t)f29::P(7) + ((uint64_t)f29::P(8) << 29) - (1ull << 33);
+
+ZKG_DEV uint64_t swap16_64(uint64_t v, bool second) {
+    const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(v >> 32), (uint32_t)(v >> 32), false, false);
+    return second ? ((uint64_t)hi[1] << 32) | lo[1] : ((uint64_t)hi[0] << 32) | lo[0];
+}
+
+// x y for inputs with limbs below 2^29 + 2^8 and x.b y.b < 169 p^2
+ZKG_DEV Fq2L mul2(const Fq2L& x, const Fq2L& y, const Where& at, uint64_t mc) {
+    const uint32_t a = pick(at, x.a, x.b, x.a, x.b), b = pick(at, y.a, y.b, y.b, y.a);
+    const uint64_t t = columns(0, a, BcastOf{b});
+    const uint64_t h = (uint64_t)bc<8>(a) * bc<8>(b);
+    const uint64_t t0 = swap16_64(t, false), t1 = swap16_64(t, true);  // rows (0 0 2 2), (1 1 3 3)
+    const uint64_t h0 = swap16_64(h, false), h1 = swap16_64(h, true);
+    const uint64_t u = at.row_b1 ? t0 + t1 : t0 + mc - t1;
+    const uint64_t hu = at.row_b1 ? h0 + h1 : h0 + M16 - h1;
+    const uint32_t r = reduce(u, hu, at);                              // rows 0, 1: real; 2, 3: imaginary
+    const auto e = __builtin_amdgcn_permlane32_swap(r, r, false, false);
+    return Fq2L{e[0], e[1]};
+}
+
+ZKG_DEV Fq2L add2(const Fq2L& x, const Fq2L& y) { return Fq2L{x.a + y.a, x.b + y.b}; }
+ZKG_DEV Fq2L norm2(const Fq2L& x) { return Fq2L{carry1(x.a), carry1(x.b)}; }
+ZKG_DEV Fq2L sub2(const Fq2L& x, const Fq2L& y, uint32_t k) {  // x + C p - y, k = C p's limb
+    return Fq2L{carry1(x.a + k - y.a), carry1(x.b + k - y.b)};
+}
+ZKG_DEV bool is_zero2(const Fq2L& x) { return f29_is_zero_below12p(to_f29(x.a)) && f29_is_zero_below12p(to_f29(x.b)); }
+
+struct SubConsts2 {  // C p in borrowed limbs for the G2 chain's subtractions, and p 2^261's column
+    uint32_t k4, k6, k8, k10, k12;
+    uint64_t mc;
+    ZKG_DEV explicit SubConsts2(uint32_t rl)
+        : k4(kp<4>(rl)), k6(kp<6>(rl)), k8(kp<8>(rl)), k10(kp<10>(rl)), k12(kp<12>(rl)), mc(mcol(rl)) {}
+};
+
+// 2P (dbl-2008-s-1) over Fq2, W Y and ZZZ3 taken as V (U Y) and V (U ZZZ) as in dbl. Bounds in
+// units of p per component (real, imaginary), a box closed under this doubling and add_g2: running
+// point X < (10.3, 9.4), Y < (6.6, 6), ZZ, ZZZ < (2.1, 2); window sums below 2.
+ZKG_DEV Point2 dbl_g2(const Point2& p, const Where& at, const SubConsts2& k) {
+    const Fq2L u = norm2(add2(p.y, p.y));                                   // < (13.2, 12)
+    const Fq2L v = mul2(u, u, at, k.mc), x2 = mul2(p.x, p.x, at, k.mc);
+    const Fq2L uy = mul2(u, p.y, at, k.mc), uz = mul2(u, p.zzz, at, k.mc);
+    const Fq2L m = norm2(add2(add2(x2, x2), x2));                           // M = 3 X^2 < (7.9, 6.4)
+    const Fq2L s = mul2(p.x, v, at, k.mc), zz3 = mul2(v, p.zz, at, k.mc);
+    const Fq2L m2 = mul2(m, m, at, k.mc), wy = mul2(v, uy, at, k.mc);
+    const Fq2L x3 = sub2(m2, add2(s, s), k.k6);                            // M^2 + 6p - 2S
+    const Fq2L d = sub2(s, x3, k.k10);                                      // S + 10p - X3
+    const Fq2L md = mul2(m, d, at, k.mc), zzz3 = mul2(v, uz, at, k.mc);
+    return Point2{x3, sub2(md, wy, k.k4), zz3, zzz3};                       // Y3 = M D + 4p - W Y
+}
+
+// P + Q (add-2008-s with its special cases) over Fq2, as add; the same return codes.
+ZKG_DEV int add_g2(const Point2& p, const Point2& q, const Where& at, const SubConsts2& k, Point2* out) {
+    const Fq2L u1 = mul2(p.x, q.zz, at, k.mc), u2 = mul2(q.x, p.zz, at, k.mc);
+    const Fq2L s1 = mul2(p.y, q.zzz, at, k.mc), s2 = mul2(q.y, p.zzz, at, k.mc);
+    const Fq2L pp = sub2(u2, u1, k.k4), r = sub2(s2, s1, k.k4);             // P, R < (6.1, 5.1)
+    if (is_zero2(pp)) return is_zero2(r) ? 2 : 1;
+    const Fq2L pp2 = mul2(pp, pp, at, k.mc), r2 = mul2(r, r, at, k.mc);
+    const Fq2L zz12 = mul2(p.zz, q.zz, at, k.mc), zzz12 = mul2(p.zzz, q.zzz, at, k.mc);
+    const Fq2L ppp = mul2(pp, pp2, at, k.mc), qq = mul2(u1, pp2, at, k.mc);
+    const Fq2L zz3 = mul2(zz12, pp2, at, k.mc);
+    const Fq2L x3 = sub2(r2, add2(ppp, add2(qq, qq)), k.k8);               // R^2 + 8p - PPP - 2Q
+    const Fq2L d = sub2(qq, x3, k.k12);                                     // Q + 12p - X3
+    const Fq2L rd = mul2(r, d, at, k.mc), s1ppp = mul2(s1, ppp, at, k.mc);
+    const Fq2L zzz3 = mul2(zzz12, ppp, at, k.mc);
+    *out = Point2{x3, sub2(rd, s1ppp, k.k4), zz3, zzz3};                    // Y3 = R D + 4p - S1 PPP
+    return 0;
+}
+
 }  // namespace lpl
 
 template <class Fe>  // Fq29 (a template, so that every translation unit may instantiate it)
@@ -1170,6 +1260,66 @@ __global__ __launch_bounds__(64) void msm_combine29_kernel(const uint32_t* __res
     }
     if (lane != 0) return;
     store_xyzz_result(result, inf ? xyzz_inf<Fq29>() : out);
+}
+
+// The G2 (Fq2_29 pipeline) Horner chain on the same limb-per-lane products: an Fq2 product per
+// instruction stream, the independent Fq2 products of a doubling or addition round interleaved by
+// the compiler. Window sums (Fq2_29 storage) are below 2p per component; the result, below 12p per
+// component, is stored canonical.
+template <class Fe>  // Fq2_29
+__global__ __launch_bounds__(64) void msm_combine_g2_lanes_kernel(const uint32_t* __restrict__ wsums, uint32_t W,
+                                                                 uint32_t wb, uint32_t nw,
+                                                                 uint32_t* __restrict__ result) {
+    constexpr int XW = 4 * FeTraits<Fq2_29>::WORDS;
+    const uint32_t lane = threadIdx.x;
+    const lpl::Where at(lane);
+    const lpl::SubConsts2 kp(at.rl);
+    const uint32_t zero = lane >> 6;  // every lane reads the whole sum (vector registers) and keeps its limb
+    auto limbs = [&](const Fq29& v) {
+        F29 f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f.l[i] = v.l[i];
+        return lpl::from_f29(f, at.rl);
+    };
+    auto load = [&](uint32_t w, bool* is_inf) {
+        const Xyzz<Fq2_29> v = load_xyzz<Fq2_29>(wsums + (size_t)XW * (w + zero));
+        *is_inf = FeTraits<Fq2_29>::is_zero(v.zz);
+        return lpl::Point2{{limbs(v.x.a), limbs(v.x.b)}, {limbs(v.y.a), limbs(v.y.b)},
+                           {limbs(v.zz.a), limbs(v.zz.b)}, {limbs(v.zzz.a), limbs(v.zzz.b)}};
+    };
+    bool inf;
+    lpl::Point2 acc = load(W - 1, &inf);
+    for (int w = (int)W - 2; w >= 0; --w) {
+        const uint32_t width = wb + ((uint32_t)w < nw ? 1u : 0u);  // bits between window w and w + 1
+        if (!inf)
+            for (uint32_t k = 0; k < width; ++k) acc = lpl::dbl_g2(acc, at, kp);
+        bool q_inf;
+        const lpl::Point2 q = load(w, &q_inf);
+        if (q_inf) continue;
+        if (inf) {
+            acc = q;
+            inf = false;
+            continue;
+        }
+        lpl::Point2 sum;
+        const int how = lpl::add_g2(acc, q, at, kp, &sum);
+        if (how == 0) acc = sum;
+        else if (how == 1) inf = true;
+        else acc = lpl::dbl_g2(acc, at, kp);
+    }
+    auto fq29 = [](uint32_t v) {
+        const F29 f = lpl::to_f29(v);
+        Fq29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = f.l[i];
+        return r;
+    };
+    Xyzz<Fq2_29> out;
+    if (!inf)
+        out = Xyzz<Fq2_29>{{fq29(acc.x.a), fq29(acc.x.b)}, {fq29(acc.y.a), fq29(acc.y.b)},
+                           {fq29(acc.zz.a), fq29(acc.zz.b)}, {fq29(acc.zzz.a), fq29(acc.zzz.b)}};
+    if (lane != 0) return;
+    store_xyzz_result(result, inf ? xyzz_inf<Fq2_29>() : out);  // below 12p: reduced to canonical words
 }
 
 // ---------------------------------------------------------------- stage launchers
@@ -1327,6 +1477,9 @@ template <class Fe>
 void launch_combine(const MsmPlan& p, hipStream_t s) {
     if constexpr (std::is_same<Fe, Fq29>::value)
         hipLaunchKernelGGL(msm_combine29_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.wb, p.nw, p.result);
+    else if constexpr (std::is_same<Fe, Fq2_29>::value)
+        hipLaunchKernelGGL(msm_combine_g2_lanes_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.wb, p.nw,
+                           p.result);
     else
         hipLaunchKernelGGL(msm_combine_kernel<Fe>, dim3(1), dim3(64), 0, s, p.wsums, p.W, p.wb, p.nw, p.result);
 }
