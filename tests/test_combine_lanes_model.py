@@ -312,6 +312,29 @@ def test_horner_chain_with_additions_matches_textbook():
     assert hits == {"inf", "doubling"}
 
 
+
+def test_addition_of_two_running_points_for_the_window_sums():
+    """msm_window_sum29_lanes_kernel adds sums to sums: both operands at the running bounds."""
+    rng = random.Random(2612)
+    hits = set()
+    for trial in range(8):
+        a = O.py_g1_mul(O.G1, rng.randrange(1, O.R))
+        b = a if trial == 6 else (a[0], (P - a[1]) % P) if trial == 7 else O.py_g1_mul(O.G1, rng.randrange(1, O.R))
+        p, q = _xyzz(a, (8, 4, 1, 1)), _xyzz(b, (8, 4, 1, 1))
+        how, s = add_pts(p, q, {})
+        exp = O.py_g1_add(a, b)
+        if how == 1:
+            hits.add("inf")
+            assert exp is None
+            continue
+        if how == 2:
+            hits.add("doubling")
+            s = dbl(p, {})
+        for c, bound in zip(s, (9, 5, 2, 2)):
+            assert row_value(c, 0) < bound * P
+        assert _affine(s) == exp
+    assert hits == {"inf", "doubling"}
+
 # ---- G2 (msm_combine_g2_lanes_kernel): Fq2 products combined on the 64-bit columns
 # p 2^261 in borrowed columns, as lpl::mcol and lpl::M16 build it from p's limbs
 MC = [(1 << 62) - ((1 << 33) if j else 0) + (PL[j - 9] if 9 <= j <= 15 else 0) for j in range(16)]
@@ -482,4 +505,29 @@ def test_g2_horner_chain_matches_textbook_at_the_bounds():
                 assert all(row_value(c[i], r) == row_value(c[i], 0) for i in (0, 1) for r in range(1, 4))
             p = s
             assert _affine2(p) == exp, (trial, step)
+    assert hits == {"inf", "doubling"}
+
+
+def test_g2_addition_of_two_running_points_for_the_window_sums():
+    """msm_window_sum_g2_lanes_kernel adds sums to sums: both operands at the G2 chain's bounds."""
+    rng = random.Random(2622)
+    hits = set()
+    for trial in range(5):
+        a = O.py_g2_mul(O.G2, rng.randrange(1, 1 << 64))
+        b = a if trial == 3 else (a[0], O.py_f2_sub((0, 0), a[1])) if trial == 4 else O.py_g2_mul(O.G2, rng.randrange(1, 1 << 64))
+        # both operands raised towards the box: X by 9p, Y by 5p, ZZ and ZZZ by p (real and imaginary)
+        p = _xyzz2(a, (9, 5, 1, 1))
+        q = _xyzz2(b, (9, 5, 1, 1))
+        how, s = add_g2(p, q, {})
+        exp = O.py_g2_add(a, b)
+        if how == 1:
+            hits.add("inf")
+            assert exp is None
+            continue
+        if how == 2:
+            hits.add("doubling")
+            s = dbl_g2(p, {})
+        for c, bound in zip(s, BOX):
+            assert row_value(c[0], 0) < bound[0] * P and row_value(c[1], 0) < bound[1] * P
+        assert _affine2(s) == exp
     assert hits == {"inf", "doubling"}

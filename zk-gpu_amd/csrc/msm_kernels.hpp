@@ -1081,10 +1081,10 @@ ZKG_DEV uint32_t from_f29(const F29& a, uint32_t rl) {  // a's limbs (the same o
     for (int i = 0; i < 9; ++i) v = rl == (uint32_t)i ? a.l[i] : v;
     return v;
 }
-ZKG_DEV F29 to_f29(uint32_t v) {  // limbs 0..7 below 2^29 again, on every lane
+ZKG_DEV F29 to_f29(uint32_t v, int row = 0) {  // row's value, limbs 0..7 below 2^29 again, on every lane
     F29 r;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) r.l[i] = (uint32_t)__builtin_amdgcn_readlane((int)v, i);
+    for (int i = 0; i < 9; ++i) r.l[i] = (uint32_t)__builtin_amdgcn_readlane((int)v, 16 * row + i);
     return f29_carry(r);
 }
 
@@ -1322,6 +1322,167 @@ __global__ __launch_bounds__(64) void msm_combine_g2_lanes_kernel(const uint32_t
     store_xyzz_result(result, inf ? xyzz_inf<Fq2_29>() : out);  // below 12p: reduced to canonical words
 }
 
+// G1 window sums with one limb per lane: a workgroup of NW waves per window; wave i adds its
+// window's partials i, i + NW, ... (each an lpl::add, both operands in the running bounds X < 9p,
+// Y < 5p, ZZ, ZZZ < 2p), then the waves fold their sums through LDS in log2 NW rounds. The sum is
+// multiplied by Montgomery's one (the four coordinates in one round of products) to bring it below
+// 2p, the form the window sums are stored in.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void msm_window_sum29_lanes_kernel(const uint32_t* __restrict__ partials,
+                                                                        uint32_t P, uint32_t nw, uint32_t gh,
+                                                                        uint32_t gl, uint32_t* __restrict__ wsums) {
+    constexpr int XW = 4 * FeTraits<Fq29>::WORDS;
+    __shared__ uint32_t pts[NW / 2][4][64];  // a wave's point, one limb per lane
+    __shared__ uint32_t pinf[NW / 2];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const lpl::Where at(lane);
+    const lpl::SubConsts kp(at.rl);
+    const uint32_t w = blockIdx.x, pw = w < nw ? gh : gl;
+    const uint32_t* src = partials + (size_t)XW * w * P;
+    const uint32_t zero = threadIdx.x >> 16;  // a lane-dependent address: vector loads of the whole point
+    auto limbs = [&](const Fq29& v) {
+        F29 f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f.l[i] = v.l[i];
+        return lpl::from_f29(f, at.rl);
+    };
+    bool inf = true;
+    lpl::Point acc{0, 0, 0, 0};
+    auto fold = [&](const lpl::Point& q, bool q_inf) {
+        if (q_inf) return;
+        if (inf) {
+            acc = q;
+            inf = false;
+            return;
+        }
+        lpl::Point sum;
+        const int how = lpl::add(acc, q, at, kp, &sum);
+        if (how == 0) acc = sum;
+        else if (how == 1) inf = true;
+        else acc = lpl::dbl(acc, at, kp);
+    };
+    for (uint32_t g = wave; g < pw; g += NW) {
+        const Xyzz<Fq29> v = load_xyzz<Fq29>(src + (size_t)XW * (g + zero));
+        fold(lpl::Point{limbs(v.x), limbs(v.y), limbs(v.zz), limbs(v.zzz)}, fq29_is_zero(v.zz));
+    }
+    for (uint32_t off = NW / 2; off > 0; off >>= 1) {
+        if (wave >= off && wave < 2 * off) {
+            uint32_t(*d)[64] = pts[wave - off];
+            d[0][lane] = acc.x;
+            d[1][lane] = acc.y;
+            d[2][lane] = acc.zz;
+            d[3][lane] = acc.zzz;
+            if (lane == 0) pinf[wave - off] = inf ? 1u : 0u;
+        }
+        __syncthreads();
+        if (wave < off) {
+            const uint32_t(*d)[64] = pts[wave];
+            fold(lpl::Point{d[0][lane], d[1][lane], d[2][lane], d[3][lane]}, pinf[wave] != 0);
+        }
+        __syncthreads();
+    }
+    if (wave != 0) return;
+    Xyzz<Fq29> out = xyzz_inf<Fq29>();
+    if (!inf) {
+        // rows x, y, zz, zzz times Montgomery's one: each coordinate below (9 / 169 + 1) p < 2p
+        const uint32_t r = lpl::mul(lpl::pick(at, acc.x, acc.y, acc.zz, acc.zzz), lpl::from_f29(f29_const_one(), at.rl), at);
+        auto fq29 = [&](int row) {
+            const F29 f = lpl::to_f29(r, row);
+            Fq29 o;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) o.l[i] = f.l[i];
+            return o;
+        };
+        out = Xyzz<Fq29>{fq29(0), fq29(1), fq29(2), fq29(3)};
+    }
+    if (lane == 0) store_xyzz(wsums + (size_t)XW * w, out);
+}
+
+// G2 window sums (Fq2_29 pipeline) the same way, with lpl::add_g2 (sums added to sums stay in the
+// G2 chain's bounds); the sum's eight components are multiplied by Montgomery's one in two rounds
+// of products, which brings each below 2p for storage.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void msm_window_sum_g2_lanes_kernel(const uint32_t* __restrict__ partials,
+                                                                         uint32_t P, uint32_t nw, uint32_t gh,
+                                                                         uint32_t gl, uint32_t* __restrict__ wsums) {
+    constexpr int XW = 4 * FeTraits<Fq2_29>::WORDS;
+    __shared__ uint32_t pts[NW / 2][8][64];  // a wave's point, one limb per lane
+    __shared__ uint32_t pinf[NW / 2];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const lpl::Where at(lane);
+    const lpl::SubConsts2 kp(at.rl);
+    const uint32_t w = blockIdx.x, pw = w < nw ? gh : gl;
+    const uint32_t* src = partials + (size_t)XW * w * P;
+    const uint32_t zero = threadIdx.x >> 16;  // a lane-dependent address: vector loads of the whole point
+    auto limbs = [&](const Fq29& v) {
+        F29 f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f.l[i] = v.l[i];
+        return lpl::from_f29(f, at.rl);
+    };
+    bool inf = true;
+    lpl::Point2 acc{};
+    auto fold = [&](const lpl::Point2& q, bool q_inf) {
+        if (q_inf) return;
+        if (inf) {
+            acc = q;
+            inf = false;
+            return;
+        }
+        lpl::Point2 sum;
+        const int how = lpl::add_g2(acc, q, at, kp, &sum);
+        if (how == 0) acc = sum;
+        else if (how == 1) inf = true;
+        else acc = lpl::dbl_g2(acc, at, kp);
+    };
+    for (uint32_t g = wave; g < pw; g += NW) {
+        const Xyzz<Fq2_29> v = load_xyzz<Fq2_29>(src + (size_t)XW * (g + zero));
+        fold(lpl::Point2{{limbs(v.x.a), limbs(v.x.b)}, {limbs(v.y.a), limbs(v.y.b)},
+                         {limbs(v.zz.a), limbs(v.zz.b)}, {limbs(v.zzz.a), limbs(v.zzz.b)}},
+             FeTraits<Fq2_29>::is_zero(v.zz));
+    }
+    for (uint32_t off = NW / 2; off > 0; off >>= 1) {
+        if (wave >= off && wave < 2 * off) {
+            uint32_t(*d)[64] = pts[wave - off];
+            d[0][lane] = acc.x.a;
+            d[1][lane] = acc.x.b;
+            d[2][lane] = acc.y.a;
+            d[3][lane] = acc.y.b;
+            d[4][lane] = acc.zz.a;
+            d[5][lane] = acc.zz.b;
+            d[6][lane] = acc.zzz.a;
+            d[7][lane] = acc.zzz.b;
+            if (lane == 0) pinf[wave - off] = inf ? 1u : 0u;
+        }
+        __syncthreads();
+        if (wave < off) {
+            const uint32_t(*d)[64] = pts[wave];
+            fold(lpl::Point2{{d[0][lane], d[1][lane]}, {d[2][lane], d[3][lane]}, {d[4][lane], d[5][lane]},
+                             {d[6][lane], d[7][lane]}},
+                 pinf[wave] != 0);
+        }
+        __syncthreads();
+    }
+    if (wave != 0) return;
+    Xyzz<Fq2_29> out = xyzz_inf<Fq2_29>();
+    if (!inf) {
+        // components times Montgomery's one, four per round: each below (10.3 / 169 + 1) p < 2p
+        const uint32_t one = lpl::from_f29(f29_const_one(), at.rl);
+        const uint32_t r0 = lpl::mul(lpl::pick(at, acc.x.a, acc.x.b, acc.y.a, acc.y.b), one, at);
+        const uint32_t r1 = lpl::mul(lpl::pick(at, acc.zz.a, acc.zz.b, acc.zzz.a, acc.zzz.b), one, at);
+        auto fq29 = [&](uint32_t r, int row) {
+            const F29 f = lpl::to_f29(r, row);
+            Fq29 o;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) o.l[i] = f.l[i];
+            return o;
+        };
+        out = Xyzz<Fq2_29>{{fq29(r0, 0), fq29(r0, 1)}, {fq29(r0, 2), fq29(r0, 3)},
+                           {fq29(r1, 0), fq29(r1, 1)}, {fq29(r1, 2), fq29(r1, 3)}};
+    }
+    if (lane == 0) store_xyzz(wsums + (size_t)XW * w, out);
+}
+
 // ---------------------------------------------------------------- stage launchers
 // Each group's stages are instantiated in their own translation units (msm_g1.hip,
 // msm_g2_*.hip) so that the large, fully inlined G2 kernels compile in parallel.
@@ -1467,10 +1628,18 @@ void launch_tree(const MsmPlan& p, hipStream_t s) {
     using Ft = typename ReduceField<Fe>::type;
     constexpr uint32_t LN = FeLanes<Ft>::value;
     const ReduceGrid<Ft> g(p);
-    uint32_t top = 1;
-    while (top < std::min<uint32_t>(std::max(g.gh, g.gl), 64 / LN)) top <<= 1;
-    hipLaunchKernelGGL(msm_window_sum_kernel<Ft>, dim3(p.W), dim3(64), 0, s, p.partials, g.P, p.nw, g.gh, g.gl,
-                       top / 2, p.wsums);
+    if constexpr (std::is_same<Fe, Fq29>::value) {
+        hipLaunchKernelGGL(msm_window_sum29_lanes_kernel<16>, dim3(p.W), dim3(64 * 16), 0, s, p.partials, g.P, p.nw,
+                           g.gh, g.gl, p.wsums);
+    } else if constexpr (std::is_same<Fe, Fq2_29>::value) {
+        hipLaunchKernelGGL(msm_window_sum_g2_lanes_kernel<16>, dim3(p.W), dim3(64 * 16), 0, s, p.partials, g.P,
+                           p.nw, g.gh, g.gl, p.wsums);
+    } else {
+        uint32_t top = 1;
+        while (top < std::min<uint32_t>(std::max(g.gh, g.gl), 64 / LN)) top <<= 1;
+        hipLaunchKernelGGL(msm_window_sum_kernel<Ft>, dim3(p.W), dim3(64), 0, s, p.partials, g.P, p.nw, g.gh, g.gl,
+                           top / 2, p.wsums);
+    }
 }
 
 template <class Fe>
