@@ -195,6 +195,25 @@ def test_montgomery_256_bases(gpu, group):
     assert np.array_equal(gpu_msm(gpu, group, mont, sc, 11, None, mont256=True), exp)
 
 
+
+# (group, c, width of window 0): G1 with the GLV split (128-bit halves: 8 windows of 16 bits at
+# c = 16; two of 15 and seven of 14 at c = 15), G2 with the GLS split (65 bits: 5 windows of 13 at
+# c = 13; five of 11 and one of 10 at c = 11) and with the GLV split at c = 17 (8 windows of 16)
+@pytest.mark.parametrize("group,c,w0", [(1, 16, 16), (1, 15, 15), (2, 13, 13), (2, 11, 11), (2, 17, 16)])
+def test_combine_addition_doubles_and_cancels(gpu, group, c, w0):
+    """The combine's window addition at its special cases: with bases Q and 2^w0 Q and scalars 2^w0
+    and 1, window 1 holds Q and window 0 holds 2^w0 Q, so after w0 doublings the chain adds a point
+    to itself (the doubling case); with -2^w0 Q the two cancel (infinity)."""
+    rng = np.random.default_rng(300 + c)
+    q = int(rng.integers(1, 1 << 62))
+    sc = O.ints_to_words([1 << w0, 1]).reshape(2, 8)
+    for k2, expect_inf in ((q << w0) % O.R, False), ((O.R - (q << w0) % O.R) % O.R, True):
+        ks = O.ints_to_words([q, k2])
+        bases = O.multiples(group, O.generator_words(group), ks).reshape(2, 16 * group)
+        exp = O.msm(group, bases, sc)
+        assert (not exp.any()) == expect_inf
+        assert np.array_equal(gpu_msm(gpu, group, bases.reshape(-1), sc.reshape(-1), c), exp)
+
 def test_msm_linearity_and_scaling(gpu):
     """tests/properties.rs:150-185: MSM(a s + t) = a MSM(s) + MSM(t); MSM(k s) = k MSM(s)."""
     n = 2000
