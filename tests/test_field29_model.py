@@ -351,3 +351,55 @@ def test_ntt_product_is_the_twiddle_product_below_2r():
     one = pow(2, 256, R_FR)
     for a in (0, 1, R_FR - 1, R_FR, 4 * R_FR - 1):
         assert ntt_mul(a, one) % R_FR == a % R_FR
+
+
+# ---- the one-reduction sum of two products (field29.hpp: f29_mul2), used by the G2 stages'
+# Fq2 product on lane pairs: (a b - c d) or (a b + c d) times 2^-261, p 2^257 added in borrowed
+# 64-bit columns for the difference
+
+def m2col_table():
+    m = re.search(r"constexpr uint64_t M2COL\(int k\) \{(.*?)\}", HDR, re.S)
+    return [int(x, 16) for x in re.findall(r"0x([0-9a-f]{16})ull", m.group(1))]
+
+
+def mul2(a, b, c, d, sub):
+    mc = m2col_table()
+    pl, pinv = limbs(P), (-pow(P, -1, 1 << 29)) % (1 << 29)
+    acc, m, r = 0, [0] * 9, [0] * 9
+    for k in range(17):
+        lo, hi = max(0, k - 8), min(k, 8)
+        s = 0
+        for i in range(lo, hi + 1):
+            acc += a[i] * b[k - i]
+            s += c[i] * d[k - i]
+        assert s <= mc[k] or not sub, "a column of c d above the borrowed constant"
+        acc += mc[k] - s if sub else s
+        if k < 9:
+            acc += sum(m[j] * pl[k - j] for j in range(k))
+            m[k] = ((acc & 0xFFFFFFFF) * pinv) & MASK
+            acc += m[k] * pl[0]
+            assert acc & MASK == 0
+            acc >>= 29
+        else:
+            acc += sum(m[j] * pl[k - j] for j in range(k - 8, 9))
+            r[k - 9] = acc & MASK
+            acc >>= 29
+        assert 0 <= acc < 1 << 64
+    r[8] = acc
+    return r
+
+
+def test_mul2_sum_of_products_with_one_reduction():
+    mc = m2col_table()
+    assert len(mc) == 17 and value(mc) == P << 257  # a multiple of p: the reduction is unchanged mod p
+    rng = random.Random(2625)
+    rinv = pow(2, -261, P)
+    edge = [0, 1, P - 1, P, 2 * P - 1, (1 << 232) - 1]
+    cases = [tuple(rng.choice(edge) for _ in range(4)) for _ in range(60)]
+    cases += [tuple(rng.randrange(2 * P) for _ in range(4)) for _ in range(400)]
+    for a, b, c, d in cases:
+        for sub in (True, False):
+            r = mul2(limbs(a), limbs(b), limbs(c), limbs(d), sub)
+            v = value(r)
+            assert v % P == ((a * b - c * d) if sub else (a * b + c * d)) * rinv % P
+            assert all(x < 1 << 29 for x in r[:8]) and v < (a * b + c * d) // (1 << 261) + P + P // 16 + 1
