@@ -140,8 +140,9 @@ def test_value_bounds_are_a_fixed_point():
     x3 = rr + 7
     d = q + 10
     assert x3 <= 9 and x3 <= 10 - 1
-    y3 = prod(r, d) + 3
-    assert prod(Y, 2) <= 2 and y3 <= 5
+    # Y3 = R D - Y PPP with one reduction (f29_mul2<true>): below (r d + Y ppp) p / 2^261 + 2p
+    y3 = (r * d + Y * 2) / lim + 2
+    assert r * d < lim and y3 <= 2.8 and y3 <= 5
     assert prod(ZZ, 2) <= 2 and prod(ZZZ, 2) <= 2
     assert prod(X, 1) <= 2  # the final conversion by 2^256 mod p < p
 
@@ -203,9 +204,9 @@ def test_kernel_point_law_matches_textbook_on_random_and_edge_chains():
             t = add(ppp, add(q, q))
             x3 = sub(7, rr, t)
             d = sub(10, q, x3)
-            y3 = sub(3, mul(r, d), mul(Y, ppp))
+            y3 = mul2(r, d, Y, ppp, True, wide=True)
             acc = [x3, y3, mul(ZZ, ppsq), mul(ZZZ, ppp)]
-            assert value(x3) < 9 * P and value(y3) < 5 * P
+            assert value(x3) < 9 * P and value(y3) < 2.8 * P
             assert value(acc[2]) < 2 * P and value(acc[3]) < 2 * P
         if exp is None or acc is None:
             assert acc is None and exp is None
@@ -362,8 +363,15 @@ def m2col_table():
     return [int(x, 16) for x in re.findall(r"0x([0-9a-f]{16})ull", m.group(1))]
 
 
-def mul2(a, b, c, d, sub):
-    mc = m2col_table()
+def m2col_wide():
+    """M2COL_WIDE as the header builds it from p's limbs: p 2^261 with the same borrows."""
+    pl = limbs(P)
+    return [((1 << 62) if k <= 15 else 0) - ((1 << 33) if k >= 1 else 0) + (pl[k - 9] if 9 <= k <= 15 else 0)
+            + ((pl[7] + (pl[8] << 29)) if k == 16 else 0) for k in range(17)]
+
+
+def mul2(a, b, c, d, sub, wide=False):
+    mc = m2col_wide() if wide else m2col_table()
     pl, pinv = limbs(P), (-pow(P, -1, 1 << 29)) % (1 << 29)
     acc, m, r = 0, [0] * 9, [0] * 9
     for k in range(17):
@@ -392,6 +400,7 @@ def mul2(a, b, c, d, sub):
 def test_mul2_sum_of_products_with_one_reduction():
     mc = m2col_table()
     assert len(mc) == 17 and value(mc) == P << 257  # a multiple of p: the reduction is unchanged mod p
+    assert value(m2col_wide()) == P << 261 and all(v > 0 for v in m2col_wide())
     rng = random.Random(2625)
     rinv = pow(2, -261, P)
     edge = [0, 1, P - 1, P, 2 * P - 1, (1 << 232) - 1]
@@ -403,3 +412,19 @@ def test_mul2_sum_of_products_with_one_reduction():
             v = value(r)
             assert v % P == ((a * b - c * d) if sub else (a * b + c * d)) * rinv % P
             assert all(x < 1 << 29 for x in r[:8]) and v < (a * b + c * d) // (1 << 261) + P + P // 16 + 1
+
+
+
+def test_mul2_wide_at_the_accumulation_bounds():
+    """f29_mul2<true> as the bucket accumulation uses it: R D - Y PPP with R < 9p, D < 12p, Y < 5p,
+    PPP < 2p (normalised limbs), below (R D + Y PPP) / 2^261 + 2p < 2.8p."""
+    rng = random.Random(2626)
+    rinv = pow(2, -261, P)
+    tops = (9 * P - 1, 12 * P - 1, 5 * P - 1, 2 * P - 1)
+    cases = [tops, (0, 0, 5 * P - 1, 2 * P - 1), (9 * P - 1, 12 * P - 1, 0, 0)]
+    cases += [tuple(rng.randrange(t + 1) for t in tops) for _ in range(400)]
+    for a, b, c, d in cases:
+        r = mul2(limbs(a), limbs(b), limbs(c), limbs(d), True, wide=True)
+        v = value(r)
+        assert v % P == (a * b - c * d) * rinv % P
+        assert all(x < 1 << 29 for x in r[:8]) and v < 2.8 * P

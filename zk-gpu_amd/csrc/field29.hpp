@@ -118,6 +118,14 @@ ZKG_DEV constexpr uint64_t M2COL(int k) {  // p 2^257 in borrowed 64-bit columns
          : 0x000060c69ce5c263ull;
 }
 
+// p 2^261 in borrowed 64-bit columns (f29_mul2<true>, for lazier inputs): p's limbs at columns
+// 9..15 and the rest of p in column 16, with the same 2^62 / 2^33 borrows
+ZKG_DEV constexpr uint64_t M2COL_WIDE(int k) {
+    return (k <= 15 ? (1ull << 62) : 0ull) - (k >= 1 ? (1ull << 33) : 0ull) +
+           (k >= 9 && k <= 15 ? (uint64_t)P(k - 9) : 0ull) +
+           (k == 16 ? (uint64_t)P(7) + ((uint64_t)P(8) << 29) : 0ull);
+}
+
 }  // namespace f29
 
 ZKG_DEV F29 f29_const_one() {
@@ -174,11 +182,13 @@ ZKG_DEV F29 f29_mul(const F29& a, const F29& b) {
 
 // (a b - c d) 2^-261 (sub) or (a b + c d) 2^-261 mod p with the two products summed on the columns
 // and one Montgomery reduction (FIPS, as f29_mul): 243 multiply-adds where two products and an
-// addition take 324. For a b - c d the multiple p 2^257 of p is added in borrowed columns (every
-// column at least 2^62 - 2^33, the last p's bits from 207 on), which keeps each column above c d's
-// (at most nine products below 2^58, and below 2^45.2 in the last column) for inputs normalised
-// below 2p; columns stay below 2^63.2. The result is normalised and below
-// (a b + c d) / 2^261 + p + p / 16 < 1.1 p.
+// addition take 324. For a b - c d a multiple of p is added in borrowed columns (every column at
+// least 2^62 - 2^33), which keeps each column above c d's: p 2^257 (the last column p's bits from
+// 207 on) for inputs normalised below 2p, where c d's last column is below 2^45.2; p 2^261 (WIDE:
+// the last column p's bits from 203 on) for normalised inputs with c d below about 12p each,
+// whose top limbs' product stays below 2^50.5. Columns stay below 2^63.2. The result is normalised
+// and below (a b + c d) / 2^261 + p + p / 16 < 1.1p, or + 2p with WIDE.
+template <bool WIDE = false>
 ZKG_DEV F29 f29_mul2(const F29& a, const F29& b, const F29& c, const F29& d, bool sub) {
     uint64_t acc = 0;
     uint32_t m[9];
@@ -192,7 +202,7 @@ ZKG_DEV F29 f29_mul2(const F29& a, const F29& b, const F29& c, const F29& d, boo
             acc += (uint64_t)a.l[i] * b.l[k - i];
             s += (uint64_t)c.l[i] * d.l[k - i];
         }
-        acc += sub ? f29::M2COL(k) - s : s;
+        acc += sub ? (WIDE ? f29::M2COL_WIDE(k) : f29::M2COL(k)) - s : s;
         if (k < 9) {
 #pragma unroll
             for (int j = 0; j < k; ++j) acc += (uint64_t)m[j] * f29::P(k - j);

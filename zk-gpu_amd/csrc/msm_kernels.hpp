@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
         const F29 t = f29_add(ppp, f29_add(q, q));     // < 6, limbs < 3 * 2^29
         const F29 x3 = f29_sub<7>(rr, t);              // < 9
         const F29 d = f29_sub<10>(q, x3);              // < 12
-        const F29 y3 = f29_sub<3>(f29_mul(r, d), f29_mul(Y, ppp));  // 9 * 12, 5 * 2 -> < 5
+        const F29 y3 = f29_mul2<true>(r, d, Y, ppp, true);  // R D - Y PPP: (9 * 12 + 5 * 2) / 169 + 2 -> < 2.8
         ZZ = f29_mul(ZZ, ppsq);                        // < 2
         ZZZ = f29_mul(ZZZ, ppp);                       // < 2
         X = x3;
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
         const F29 t = f29_add(ppp, f29_add(q, q));     // < 6, limbs < 3 * 2^29
         const F29 x3 = f29_sub<7>(rr, t);              // < 9
         const F29 d = f29_sub<10>(q, x3);              // < 12
-        const F29 y3 = f29_sub<3>(f29_mul(r, d), f29_mul(Y, ppp));  // 9 * 12, 5 * 2 -> < 5
+        const F29 y3 = f29_mul2<true>(r, d, Y, ppp, true);  // R D - Y PPP: (9 * 12 + 5 * 2) / 169 + 2 -> < 2.8
         ZZ = f29_mul(ZZ, ppsq);                        // < 2
         ZZZ = f29_mul(ZZZ, ppp);                       // < 2
         X = x3;
