@@ -119,6 +119,13 @@ ZKG_DEV Xyzz<Fe> xyzz_neg(const Xyzz<Fe>& p) {
 }
 
 // 2P, dbl-2008-s-1 (a = 0): U = 2Y, V = U^2, W = UV, S = XV, M = 3X^2,
+// a b - c d for the point laws' Y3; a field type with a cheaper form (both products under one
+// reduction) overloads it, found by argument-dependent lookup where the laws are instantiated
+template <class Fe>
+ZKG_DEV Fe fe_mul_sub(const Fe& a, const Fe& b, const Fe& c, const Fe& d) {
+    return a * b - c * d;
+}
+
 // X3 = M^2 - 2S, Y3 = M(S - X3) - WY, ZZ3 = V ZZ, ZZZ3 = W ZZZ. Infinity maps to infinity.
 template <class Fe>
 ZKG_DEV Xyzz<Fe> xyzz_dbl(const Xyzz<Fe>& p) {
@@ -130,7 +137,7 @@ ZKG_DEV Xyzz<Fe> xyzz_dbl(const Xyzz<Fe>& p) {
     const Fe x2 = T::sqr(p.x);
     const Fe m = (x2 + x2) + x2;
     const Fe x3 = T::sqr(m) - (s + s);
-    const Fe y3 = m * (s - x3) - w * p.y;
+    const Fe y3 = fe_mul_sub(m, s - x3, w, p.y);
     return Xyzz<Fe>{x3, y3, v * p.zz, w * p.zzz};
 }
 
@@ -151,7 +158,7 @@ ZKG_DEV Xyzz<Fe> xyzz_madd_nonzero(const Xyzz<Fe>& p, const Affine<Fe>& q) {
     const Fe ppp = pp * ppsq;
     const Fe qq = p.x * ppsq;
     const Fe x3 = (T::sqr(r) - ppp) - (qq + qq);
-    const Fe y3 = r * (qq - x3) - p.y * ppp;
+    const Fe y3 = fe_mul_sub(r, qq - x3, p.y, ppp);
     return Xyzz<Fe>{x3, y3, p.zz * ppsq, p.zzz * ppp};
 }
 
@@ -181,7 +188,7 @@ ZKG_DEV Xyzz<Fe> xyzz_add(const Xyzz<Fe>& p, const Xyzz<Fe>& q) {
     const Fe ppp = pp * ppsq;
     const Fe qq = u1 * ppsq;
     const Fe x3 = (T::sqr(r) - ppp) - (qq + qq);
-    const Fe y3 = r * (qq - x3) - s1 * ppp;
+    const Fe y3 = fe_mul_sub(r, qq - x3, s1, ppp);
     return Xyzz<Fe>{x3, y3, (p.zz * q.zz) * ppsq, (p.zzz * q.zzz) * ppp};
 }
 
