@@ -911,12 +911,13 @@ ZKG_DEV F29 f29_carry(F29 a) {
     return a;
 }
 
-// ---------------------------------------------------------------- limb-per-lane doubling chain
-// The combine's doublings run on one wave, so their cost is the instruction count of one dependent
-// chain. Here each field element is one VGPR: lane i of a 16-lane row holds limb i (lanes 9..15
-// hold 0), and the four rows of the wave carry four independent products. A product then costs
-// about 115 instructions for four (nine multiply-adds per column, the operands moved by DPP row
-// shifts and row broadcasts) instead of about 230 for each product held on one lane.
+// ---------------------------------------------------------------- limb-per-lane point chains
+// The combines' Horner chains (and the window sums' short chains of additions) run one dependent
+// chain per wave, so their cost is that chain's instruction count. Here each field element is one
+// VGPR: lane i of a 16-lane row holds limb i (lanes 9..15 hold 0), and the four rows of the wave
+// carry four independent products. A product then costs about 115 instructions for four (nine
+// multiply-adds per column, the operands moved by DPP row shifts and row broadcasts) instead of
+// about 230 for each product held on one lane.
 //
 // Product (Montgomery, R = 2^261, same contract as f29_mul: limbs < 2^30 + 2^8, v(a) v(b) below
 // 169 p^2, result below a b / 2^261 + p (1 + 2^-50), limbs 0..7 below 2^29 + 2^7 + 1):
@@ -928,7 +929,8 @@ ZKG_DEV F29 f29_carry(F29 a) {
 //       2^261 iff one of them is non-zero: that bit is the carry into the result's limb 0;
 //   result = (U_9 .. U_15 moved down nine lanes, H's low 29 bits, H's high bits) + carry.
 // H stays below 2^53 because H 2^203 is at most the result (below 3p). Every bound was checked on
-// a lane-level model of these steps (random and extreme limbs against integer arithmetic).
+// a lane-level model of these steps (tests/test_combine_lanes_model.py: random and extreme limbs
+// against integer arithmetic, and the point laws below against the textbook ones).
 namespace lpl {
 
 template <int CTRL>
