@@ -214,6 +214,23 @@ def test_combine_addition_doubles_and_cancels(gpu, group, c, w0):
         assert (not exp.any()) == expect_inf
         assert np.array_equal(gpu_msm(gpu, group, bases.reshape(-1), sc.reshape(-1), c), exp)
 
+
+@pytest.mark.parametrize("group,c", [(1, 16), (2, 13)])
+def test_window_sum_partials_double_and_cancel(gpu, group, c):
+    """The window sums' fold at its special cases: scalars 11 and 1001 put P1 and P2 in buckets 10
+    and 1000 of window 0, which the bucket reduction gives to different workgroups (512 buckets per
+    G1 workgroup, 128 per G2), so the window sum adds the partials 11 P1 and 1001 P2; with
+    P2 = +-(11 / 1001) P1 they are equal (doubling) or opposite (infinity)."""
+    a = 123456789
+    w = 11 * pow(1001, -1, O.R) % O.R
+    sc = O.ints_to_words([11, 1001]).reshape(2, 8)
+    for k2, expect_inf in ((a * w % O.R, False), ((O.R - a * w % O.R) % O.R, True)):
+        bases = O.multiples(group, O.generator_words(group), O.ints_to_words([a, k2])).reshape(2, 16 * group)
+        exp = O.msm(group, bases, sc)
+        assert (not exp.any()) == expect_inf
+        assert np.array_equal(gpu_msm(gpu, group, bases.reshape(-1), sc.reshape(-1), c), exp)
+
+
 def test_msm_linearity_and_scaling(gpu):
     """tests/properties.rs:150-185: MSM(a s + t) = a MSM(s) + MSM(t); MSM(k s) = k MSM(s)."""
     n = 2000
