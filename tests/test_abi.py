@@ -135,24 +135,6 @@ def test_kernels_contain_no_device_calls():
     assert r.returncode == 0, r.stderr
 
 
-def test_gls_item_size_fills_rounds_of_resident_waves():
-    """G2 plans with the GLS split take the GLV plan's item size (a quarter of the mean bucket,
-    src/msm.rs:99-100) or a larger one below twice it that fits the work items into fewer rounds
-    of the accumulation's resident waves (two per SIMD on 1024 SIMDs): 45 points per item at 2^18
-    (c = 13), 23 at 2^17 (c = 12); 2^16 (c = 11), well under one round, keeps 32, and G1 keeps the
-    rule. Compared through the plans' workspaces, which grow with the number of work items."""
-    from zkgpu import Group, Msm
-
-    def same(group, n, c, ch):
-        return Msm.plan_check(group, n, c) == Msm.plan_check(group, n, c, ch)
-
-    assert same(Group.G2, 1 << 18, 13, 45)
-    assert not same(Group.G2, 1 << 18, 13, 32) and not same(Group.G2, 1 << 18, 13, 44)
-    assert same(Group.G2, 1 << 17, 12, 23) and not same(Group.G2, 1 << 17, 12, 16)
-    assert same(Group.G2, 1 << 16, 11, 32)
-    assert same(Group.G1, 1 << 16, 11, 32) and same(Group.G1, 1 << 18, 13, 32)
-
-
 def test_plan_bounds_reject_32_bit_index_overflow():
     """Plans whose sorted (point, digit) entries W * nv would reach 2^32 are rejected on the host,
     before any allocation; the reference's own argument errors keep their text (src/msm.rs:96-98)."""

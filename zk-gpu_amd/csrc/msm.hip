@@ -25,7 +25,6 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
-#include <cmath>
 #include <cstring>
 
 #include "curve.hpp"
@@ -58,29 +57,6 @@ uint32_t msm_auto_item_points(size_t n, uint32_t c) {
     if (p < 8) p = 8;
     if (p > 1024) p = 1024;
     return (uint32_t)p;
-}
-
-// Item size of a G2 plan with the GLS split and one lane per work item. Its accumulation keeps two
-// waves on each of MI355X's 1024 SIMDs (256 VGPRs each), and the items beyond those 2048 waves run
-// as a second round that leaves most SIMDs with one wave or none. Measured at G2 2^18, c = 13 (an
-// estimated 2720 waves at the reference rule's 32 points per item): accumulate 1.89 ms at 32, 2.38
-// to 2.53 at 40-42 (just over one round), 1.67-1.75 at 43-45 (one round). The size taken is the
-// rule's or larger, below twice it, minimising size x rounds, with W (nv / ch + B / 2) items
-// (ceil(count / ch) over buckets of Poisson-distributed counts: an over-estimate for the top window,
-// whose digits use fewer buckets) and rounds of 97% of 2048 waves.
-static uint32_t msm_gls_item_points(uint32_t W, uint64_t nv, uint32_t B, uint32_t auto_ch) {
-    constexpr double ROUND = 0.97 * 2048 * 64;  // work items per round of resident waves
-    uint32_t best = auto_ch;
-    double best_cost = 0;
-    for (uint32_t ch = auto_ch; ch < 2 * auto_ch; ++ch) {
-        const double items = (double)W * ((double)nv / ch + 0.5 * B);
-        const double cost = ch * std::ceil(items / ROUND);
-        if (ch == auto_ch || cost < best_cost) {
-            best = ch;
-            best_cost = cost;
-        }
-    }
-    return best;
 }
 
 // ---------------------------------------------------------------- small utility kernels
@@ -1250,9 +1226,7 @@ MsmShape msm_shape(int group, size_t npts, uint32_t c, uint32_t item_points, uin
     // rule's larger items halve the work items, and the G2 accumulation (two waves per SIMD) then
     // no longer fills the GPU (G2 2^18, c = 13: accumulate 2.50 ms at 64 points per item, 1.91 at 32).
     sh.CH = item_points ? item_points : msm_auto_item_points(sh.gls ? nv / 2 : nv, cw);
-    if (!item_points && sh.gls && !msm_lanes_enabled((uint64_t)sh.W * nv))
-        sh.CH = msm_gls_item_points(sh.W, nv, sh.B, sh.CH);
-    sh.L =msm_reduce_chunk(group, cw, sh.W);
+    sh.L = msm_reduce_chunk(group, cw, sh.W);
     sh.T = sh.B / sh.L;
     sh.aw = group == 1 ? 16 : 32;
     sh.xw = 2 * sh.aw;
